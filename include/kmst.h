@@ -1,0 +1,153 @@
+/* kmst.h -- C ABI of libkmst, the MI355X-native Boruvka / filter-Boruvka
+ * minimum spanning forest library.
+ *
+ * The two solve entry points replace KaMSTa's algorithm templates
+ *   hybridMST::boruvka(InputEdges, const AlgorithmConfig&)
+ *       include/algorithms/hybrid_boruvka.hpp:263-301
+ *   hybridMST::filter_boruvka(InputEdges, const AlgorithmConfig&)
+ *       include/algorithms/hybrid_boruvka_edgefilter.hpp:420-462
+ * which the driver calls at benchmarks/mst.cpp:215-228.  Inputs follow their
+ * contract: every undirected edge stored in both directions, records sorted
+ * by (src, dst), no self-loops (include/util/utils.hpp:391-395).  Unlike the
+ * reference, the contract is checked and violations are reported as status
+ * codes instead of undefined behaviour or std::abort.
+ *
+ * Results are canonical: each MST edge once as (min, max, weight), sorted by
+ * (min, max).  Ties are broken by the strict key (weight, position of the
+ * (min, max) record), which equals the reference check's Kruskal order
+ * (weight, src, dst) (include/util/utils.hpp:81-86); the forest is therefore
+ * unique and identical for both algorithms and any filter threshold.
+ *
+ * Every call is blocking.  A context owns device workspaces and is not
+ * thread-safe; calls without a context use one default context per device,
+ * serialised by a lock.
+ */
+#ifndef KMST_H
+#define KMST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* == hybridMST::WEdge (include/definitions.hpp:66-84): 24 bytes. */
+typedef struct kmst_wedge {
+  uint64_t src;
+  uint64_t dst;
+  uint32_t weight;
+  uint32_t _pad;
+} kmst_wedge;
+
+/* Mirrors AlgorithmConfig (include/algorithms/algorithm_configurations.hpp:28-34)
+ * plus the choice of entry point and device. */
+typedef enum kmst_algorithm {
+  KMST_ALGO_BORUVKA = 0,        /* hybridMST::boruvka         */
+  KMST_ALGO_FILTER_BORUVKA = 1  /* hybridMST::filter_boruvka  */
+} kmst_algorithm;
+
+typedef struct kmst_config {
+  int32_t algorithm;            /* kmst_algorithm */
+  int32_t local_preprocessing;  /* LocalPreprocessing: accepted, no effect on one GPU
+                                   (the whole graph is local) */
+  int32_t compression;          /* Compression: only 0 (NO_COMPRESSION) */
+  int32_t device;               /* HIP device ordinal */
+  uint64_t filter_threshold;    /* filter-Boruvka base case when m/n0 <= this (default 4) */
+} kmst_config;
+
+typedef enum kmst_status {
+  KMST_OK = 0,
+  KMST_EINVAL = 1,       /* bad argument, self-loop, id >= 2^32-1, too many records */
+  KMST_EUNSORTED = 2,    /* records not sorted by (src, dst) */
+  KMST_EASYMMETRIC = 3,  /* some record has no reverse twin of equal weight */
+  KMST_ENOMEM = 4,       /* device allocation failed / admission refused */
+  KMST_EDEVICE = 5,      /* HIP runtime error or a kernel's bounded wait expired */
+  KMST_EBUSY = 6         /* serving: queue full */
+} kmst_status;
+
+/* Device-resident input, structure of arrays (12 bytes per directed record). */
+typedef struct kmst_dev_edges {
+  const uint32_t* src;
+  const uint32_t* dst;
+  const uint32_t* weight;
+  uint64_t m_directed;   /* number of records (2 x undirected edges) */
+  uint64_t n;            /* vertex ids are < n */
+} kmst_dev_edges;
+
+/* Device-resident output, capacity >= n - 1 entries each. */
+typedef struct kmst_dev_mst {
+  uint32_t* src;         /* min endpoint */
+  uint32_t* dst;         /* max endpoint */
+  uint32_t* weight;
+  uint64_t capacity;
+} kmst_dev_mst;
+
+typedef struct kmst_context kmst_context;
+
+/* Fills cfg with the reference defaults (AlgorithmConfig{}, threshold 4). */
+void kmst_config_default(kmst_config* cfg);
+
+kmst_status kmst_context_create(int device, kmst_context** out);
+void kmst_context_destroy(kmst_context* ctx);
+/* Frees the context's cached device workspaces. */
+void kmst_context_trim(kmst_context* ctx);
+
+/* Host edge list in, host MST out.  `out` must hold at least
+ * (number of distinct vertices - 1) entries; m_directed / 2 always suffices.
+ * Replaces hybridMST::boruvka / filter_boruvka (selected by cfg->algorithm). */
+kmst_status kmst_boruvka(const kmst_wedge* in, uint64_t m_directed,
+                         const kmst_config* cfg, kmst_wedge* out,
+                         uint64_t* out_n, uint64_t* total_weight);
+kmst_status kmst_filter_boruvka(const kmst_wedge* in, uint64_t m_directed,
+                                const kmst_config* cfg, kmst_wedge* out,
+                                uint64_t* out_n, uint64_t* total_weight);
+
+/* Device edge list in, device MST out; the headline timing path.  The input
+ * must stay valid for the call; nothing is copied to the host except counts. */
+kmst_status kmst_solve_device(kmst_context* ctx, const kmst_dev_edges* in,
+                              const kmst_config* cfg, kmst_dev_mst* out,
+                              uint64_t* out_n, uint64_t* total_weight);
+
+/* Per-kernel HIP-event timing of later solves on this context (measurement
+ * hook; off by default).  Totals appear in kmst_context_stats_json. */
+void kmst_context_set_timing(kmst_context* ctx, int on);
+
+/* Thread-local message for the last non-OK status. */
+const char* kmst_last_error(void);
+
+/* Per-phase statistics of the last solve on a context (rounds, records,
+ * vertices, algorithmic bytes per kernel class).  Returns the number of
+ * bytes written to buf (a JSON object), or the size needed if buf is small. */
+size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap);
+
+/* ---- Input side: deterministic generators (KaGen replacement). ----------
+ * GNM: m distinct undirected pairs on n vertices, the first m distinct
+ * pairs of a counter-based hash sequence; weight = 1 + H(seed, min, max) mod
+ * max_weight.  Output is symmetric and sorted by (src, dst).  The device
+ * and host versions produce identical arrays. */
+kmst_status kmst_gen_gnm_device(kmst_context* ctx, uint64_t n, uint64_t m,
+                                uint32_t max_weight, uint64_t seed,
+                                uint32_t* src, uint32_t* dst, uint32_t* weight);
+kmst_status kmst_gen_gnm_host(uint64_t n, uint64_t m, uint32_t max_weight,
+                              uint64_t seed, uint32_t* src, uint32_t* dst,
+                              uint32_t* weight);
+
+/* ---- Kernel-level entry points (device pointers), for per-kernel tests. */
+/* K1: per-vertex min of val over records grouped by src (sorted);
+ * best_val[v] = UINT64_MAX and best_dst[v] untouched for empty vertices. */
+kmst_status kmst_k_segmented_argmin(kmst_context* ctx, const uint32_t* src,
+                                    const uint32_t* dst, const uint64_t* val,
+                                    uint64_t m, uint64_t n, uint64_t* best_val,
+                                    uint32_t* best_dst);
+/* K6: stable LSD radix sort of (src, dst, val) by (src, dst) over
+ * `bits` bits of each endpoint, in place (uses context scratch). */
+kmst_status kmst_k_sort_pairs(kmst_context* ctx, uint32_t* src, uint32_t* dst,
+                              uint64_t* val, uint64_t m, uint32_t bits);
+/* K3: parent[v] <- root of v, in place. */
+kmst_status kmst_k_pointer_jump(kmst_context* ctx, uint32_t* parent, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KMST_H */

@@ -1,0 +1,226 @@
+"""kamsta_amd -- MI355X-native Boruvka / filter-Boruvka minimum spanning forest.
+
+Python mirror of KaMSTa's algorithm entry points over the C ABI of
+``libkmst.so`` (declared in include/kmst.h):
+
+* ``boruvka(edges, config)``        <- hybridMST::boruvka
+  (include/algorithms/hybrid_boruvka.hpp:263-301)
+* ``filter_boruvka(edges, config)`` <- hybridMST::filter_boruvka
+  (include/algorithms/hybrid_boruvka_edgefilter.hpp:420-462)
+* ``AlgorithmConfig``               <- AlgorithmConfig
+  (include/algorithms/algorithm_configurations.hpp:28-34)
+
+``edges`` is a numpy array of ``WEDGE_DTYPE`` (the 24-byte hybridMST::WEdge,
+include/definitions.hpp:66-84): every undirected edge in both directions,
+sorted by (src, dst).  The result is the MST as canonical (min, max, weight)
+records sorted by (min, max), plus its u64 total weight.  Unlike the
+reference, which aborts, contract violations raise ``KmstError`` with the C
+status code.  All compute runs on the GPU through libkmst; there is no CPU
+fallback, and a missing library raises ``ImportError``.
+"""
+import ctypes
+import dataclasses
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkmst.so")
+
+WEDGE_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("weight", "<u4"), ("_pad", "<u4")])
+
+KMST_OK, KMST_EINVAL, KMST_EUNSORTED, KMST_EASYMMETRIC, KMST_ENOMEM, KMST_EDEVICE, KMST_EBUSY = range(7)
+STATUS_NAMES = {0: "OK", 1: "EINVAL", 2: "EUNSORTED", 3: "EASYMMETRIC", 4: "ENOMEM",
+                5: "EDEVICE", 6: "EBUSY"}
+
+BORUVKA, FILTER_BORUVKA = 0, 1
+
+# every symbol include/kmst.h declares
+EXPORTED_SYMBOLS = [
+    "kmst_config_default", "kmst_context_create", "kmst_context_destroy", "kmst_context_trim",
+    "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_context_set_timing",
+    "kmst_last_error", "kmst_context_stats_json", "kmst_gen_gnm_device", "kmst_gen_gnm_host",
+    "kmst_k_segmented_argmin", "kmst_k_sort_pairs", "kmst_k_pointer_jump",
+]
+
+
+class KmstError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("algorithm", ctypes.c_int32), ("local_preprocessing", ctypes.c_int32),
+                ("compression", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("filter_threshold", ctypes.c_uint64)]
+
+
+class _DevEdges(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("m_directed", ctypes.c_uint64), ("n", ctypes.c_uint64)]
+
+
+class _DevMst(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("capacity", ctypes.c_uint64)]
+
+
+@dataclasses.dataclass
+class AlgorithmConfig:
+    """Mirror of hybridMST::AlgorithmConfig; local_preprocessing is accepted
+    and has no effect on one GPU (the whole graph is local)."""
+    local_preprocessing: int = 0
+    compression: int = 0
+    filter_threshold: int = 4
+    device: int = 0
+
+    def _c(self, algorithm):
+        return _Config(algorithm, self.local_preprocessing, self.compression, self.device,
+                       self.filter_threshold)
+
+
+_lib = None
+
+
+def load():
+    """Load libkmst.so (built by kamsta_amd.build); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m kamsta_amd.build` first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+    pu64 = ctypes.POINTER(ctypes.c_uint64)
+    sig = {
+        "kmst_config_default": (None, [vp]),
+        "kmst_context_create": (i32, [i32, ctypes.POINTER(vp)]),
+        "kmst_context_destroy": (None, [vp]),
+        "kmst_context_trim": (None, [vp]),
+        "kmst_boruvka": (i32, [vp, u64, vp, vp, pu64, pu64]),
+        "kmst_filter_boruvka": (i32, [vp, u64, vp, vp, pu64, pu64]),
+        "kmst_solve_device": (i32, [vp, vp, vp, vp, pu64, pu64]),
+        "kmst_context_set_timing": (None, [vp, i32]),
+        "kmst_last_error": (ctypes.c_char_p, []),
+        "kmst_context_stats_json": (ctypes.c_size_t, [vp, vp, ctypes.c_size_t]),
+        "kmst_gen_gnm_device": (i32, [vp, u64, u64, u32, u64, vp, vp, vp]),
+        "kmst_gen_gnm_host": (i32, [u64, u64, u32, u64, vp, vp, vp]),
+        "kmst_k_segmented_argmin": (i32, [vp, vp, vp, vp, u64, u64, vp, vp]),
+        "kmst_k_sort_pairs": (i32, [vp, vp, vp, vp, u64, u32]),
+        "kmst_k_pointer_jump": (i32, [vp, vp, u64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != KMST_OK:
+        msg = load().kmst_last_error()
+        raise KmstError(status, msg.decode() if msg else "")
+
+
+def _host_solve(fn, edges, config):
+    edges = np.ascontiguousarray(edges, dtype=WEDGE_DTYPE)
+    m = len(edges)
+    out = np.zeros(max(m // 2, 1), dtype=WEDGE_DTYPE)
+    n_out, total = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    cfg = config._c(BORUVKA)
+    _check(fn(edges.ctypes.data if m else None, m, ctypes.byref(cfg), out.ctypes.data,
+              ctypes.byref(n_out), ctypes.byref(total)))
+    return out[: n_out.value].copy(), total.value
+
+
+def boruvka(edges, config=None):
+    """hybridMST::boruvka: returns (canonical MST records, u64 total weight)."""
+    return _host_solve(load().kmst_boruvka, edges, config or AlgorithmConfig())
+
+
+def filter_boruvka(edges, config=None):
+    """hybridMST::filter_boruvka: returns (canonical MST records, u64 total weight)."""
+    return _host_solve(load().kmst_filter_boruvka, edges, config or AlgorithmConfig())
+
+
+def gen_gnm_host(n, m, max_weight=254, seed=1):
+    """Deterministic GNM on the host: (src, dst, weight) uint32 arrays, 2m records."""
+    s = np.empty(2 * m, np.uint32)
+    d = np.empty(2 * m, np.uint32)
+    w = np.empty(2 * m, np.uint32)
+    _check(load().kmst_gen_gnm_host(n, m, max_weight, seed, s.ctypes.data, d.ctypes.data,
+                                    w.ctypes.data))
+    return s, d, w
+
+
+def to_wedges(src, dst, weight):
+    """Pack SoA arrays into WEDGE_DTYPE records."""
+    e = np.zeros(len(src), dtype=WEDGE_DTYPE)
+    e["src"], e["dst"], e["weight"] = src, dst, weight
+    return e
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr() if hasattr(t, "data_ptr") else int(t))
+
+
+class Context:
+    """Owns a libkmst context (stream + cached device workspaces) on one GPU.
+
+    Device arrays are torch tensors (int32 / int64 storage reinterpreted as
+    uint32 / uint64) or raw device addresses."""
+
+    def __init__(self, device=0):
+        self._lib = load()
+        self._h = ctypes.c_void_p()
+        _check(self._lib.kmst_context_create(device, ctypes.byref(self._h)))
+        self.device = device
+
+    def close(self):
+        if self._h:
+            self._lib.kmst_context_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on=True):
+        self._lib.kmst_context_set_timing(self._h, 1 if on else 0)
+
+    def stats(self):
+        need = self._lib.kmst_context_stats_json(self._h, None, 0)
+        buf = ctypes.create_string_buffer(need + 1)
+        self._lib.kmst_context_stats_json(self._h, buf, need + 1)
+        return json.loads(buf.value.decode())
+
+    def solve_device(self, src, dst, weight, m, n, out_src, out_dst, out_weight, capacity,
+                     algorithm=BORUVKA, config=None):
+        """Device-resident solve; returns (number of MST edges, u64 weight)."""
+        cfg = (config or AlgorithmConfig(device=self.device))._c(algorithm)
+        e = _DevEdges(_ptr(src).value, _ptr(dst).value, _ptr(weight).value, m, n)
+        o = _DevMst(_ptr(out_src).value, _ptr(out_dst).value, _ptr(out_weight).value, capacity)
+        n_out, total = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(self._lib.kmst_solve_device(self._h, ctypes.byref(e), ctypes.byref(cfg),
+                                           ctypes.byref(o), ctypes.byref(n_out),
+                                           ctypes.byref(total)))
+        return n_out.value, total.value
+
+    def gen_gnm_device(self, n, m, max_weight, seed, src, dst, weight):
+        _check(self._lib.kmst_gen_gnm_device(self._h, n, m, max_weight, seed, _ptr(src),
+                                             _ptr(dst), _ptr(weight)))
+
+    def segmented_argmin(self, src, dst, val, m, n, best_val, best_dst):
+        _check(self._lib.kmst_k_segmented_argmin(self._h, _ptr(src), _ptr(dst), _ptr(val), m, n,
+                                                 _ptr(best_val), _ptr(best_dst)))
+
+    def sort_pairs(self, src, dst, val, m, bits):
+        _check(self._lib.kmst_k_sort_pairs(self._h, _ptr(src), _ptr(dst), _ptr(val), m, bits))
+
+    def pointer_jump(self, parent, n):
+        _check(self._lib.kmst_k_pointer_jump(self._h, _ptr(parent), n))

@@ -1,0 +1,154 @@
+// kmst_common.hpp -- shared types, error handling and wave/block primitives
+// for the libkmst HIP kernels (gfx950, 64-lane wavefronts).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/kmst.h"
+
+namespace kmst {
+
+using u8 = std::uint8_t;
+using u32 = std::uint32_t;
+using u64 = std::uint64_t;
+using i64 = std::int64_t;
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;            // 4 waves per workgroup
+constexpr int WAVES = BLOCK / WAVE;
+constexpr u32 NONE32 = 0xFFFFFFFFu;
+constexpr u64 VAL_NONE = ~u64{0};     // "no edge" / dead record key
+
+// Error bits a kernel can raise in the context's device error word.
+enum ErrBit : u32 {
+  ERR_UNSORTED = 1u,
+  ERR_INVALID = 2u,
+  ERR_ASYMMETRIC = 4u,
+  ERR_SPIN = 8u,      // a bounded wait or bounded loop hit its limit
+};
+
+struct Error : std::runtime_error {
+  kmst_status status;
+  Error(kmst_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
+#define KMST_HIP(expr)                                                              \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess)                                                           \
+      throw ::kmst::Error(KMST_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+inline u32 ceil_div_u32(u64 a, u64 b) { return static_cast<u32>((a + b - 1) / b); }
+inline u32 bits_for(u64 n) {  // bits needed to represent values in [0, n)
+  u32 b = 0;
+  while (b < 64 && (u64{1} << b) < n) ++b;
+  return b;
+}
+
+// ------------------------------------------------------- per-kernel timing
+// Every launch site brackets its kernel with a KTScope carrying the kernel's
+// algorithmic bytes; when a KernelTimer is installed (a context with timing
+// on) the scope records a HIP event pair on the launch stream.
+enum KernelKind : int {
+  K_FILL, K_SCAN_REDUCE, K_SCAN_DOWN, K_VALIDATE, K_OFFSETS, K_KEYS, K_ARGMIN, K_ARGMIN_FIXUP,
+  K_HOOK, K_JUMP, K_LABEL, K_GPAR, K_RELABEL_COUNT, K_RELABEL_SCATTER, K_SORT_HIST,
+  K_SORT_SCATTER, K_DEDUP, K_SAMPLE, K_SPLIT_COUNT, K_SPLIT_SCATTER, K_GATHER, K_GEN,
+  K_COUNT
+};
+const char* kernel_kind_name(int k);
+
+struct KernelTotals {
+  double launches = 0, ms = 0, bytes = 0;
+};
+
+struct KernelTimer {
+  struct Mark {
+    int kind;
+    double bytes;
+  };
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+  std::vector<Mark> marks;
+  size_t used = 0;
+  std::map<int, KernelTotals> totals;
+  void begin(hipStream_t st, int kind, double bytes);
+  void end(hipStream_t st) noexcept;
+  void collect();  // after the stream is synchronised
+  void reset() { totals.clear(); }
+  ~KernelTimer();
+};
+
+extern thread_local KernelTimer* g_ktimer;
+
+struct KTScope {
+  hipStream_t st;
+  bool on;
+  KTScope(hipStream_t s, int kind, double bytes) : st(s), on(g_ktimer != nullptr) {
+    if (on) g_ktimer->begin(s, kind, bytes);
+  }
+  ~KTScope() {
+    if (on) g_ktimer->end(st);
+  }
+  KTScope(const KTScope&) = delete;
+  KTScope& operator=(const KTScope&) = delete;
+};
+
+// ---------------------------------------------------------------- device side
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+__device__ __forceinline__ int wave_id() { return threadIdx.x / WAVE; }
+
+// popcount(mask & lanes-below-me)
+__device__ __forceinline__ u32 mbcnt(u64 mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<u32>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<u32>(mask), 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    T y = __shfl_up(x, d, WAVE);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// Exclusive prefix sum over the workgroup; `lds` holds >= WAVES entries.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_sum(T x, T& total, T* lds) {
+  const int lane = lane_id(), wid = wave_id();
+  const T incl = wave_inclusive_sum(x);
+  if (lane == WAVE - 1) lds[wid] = incl;
+  __syncthreads();
+  T off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < WAVES; ++w) {
+    const T v = lds[w];
+    off += (w < wid) ? v : T(0);
+    tot += v;
+  }
+  __syncthreads();
+  total = tot;
+  return off + incl - x;
+}
+
+// Wave-aggregated append: returns the slot for lanes with pred set.
+__device__ __forceinline__ u32 wave_append(bool pred, u32* counter) {
+  const u64 mask = __ballot(pred);
+  if (mask == 0) return 0;
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  u32 base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, static_cast<u32>(__popcll(mask)));
+  base = __shfl(base, leader, WAVE);
+  return base + mbcnt(mask);
+}
+
+}  // namespace kmst

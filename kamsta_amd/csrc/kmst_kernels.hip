@@ -1,0 +1,927 @@
+// kmst_kernels.hip -- hand-written CDNA4 (gfx950) kernels of the Boruvka /
+// filter-Boruvka MST path.  All work is integer and HBM-bound; no MFMA.
+//
+// Record layout (structure of arrays, 16 bytes per directed record):
+//   src[i], dst[i]  u32 endpoint ids in the current (compacted) numbering
+//   val[i]          u64 strict key (weight << 32) | uid, uid = input position
+//                   of the (min, max) twin; VAL_NONE marks a dead record.
+#include "kmst_kernels.hpp"
+
+namespace kmst {
+
+// ============================================================== fills
+__global__ __launch_bounds__(BLOCK) void fill_u64_kernel(u64* p, u64 n, u64 value) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += u64(gridDim.x) * BLOCK) p[i] = value;
+}
+__global__ __launch_bounds__(BLOCK) void fill_u32_kernel(u32* p, u64 n, u32 value) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += u64(gridDim.x) * BLOCK) p[i] = value;
+}
+
+static u32 grid_for(u64 n, u32 cap = 65536) {
+  const u64 g = (n + BLOCK - 1) / BLOCK;
+  return static_cast<u32>(g == 0 ? 1 : (g > cap ? cap : g));
+}
+
+void fill_u64(hipStream_t st, u64* p, u64 n, u64 value) {
+  if (n == 0) return;
+  KTScope t(st, K_FILL, 8.0 * n);
+  fill_u64_kernel<<<grid_for(n), BLOCK, 0, st>>>(p, n, value);
+  KMST_HIP(hipGetLastError());
+}
+void fill_u32(hipStream_t st, u32* p, u64 n, u32 value) {
+  if (n == 0) return;
+  KTScope t(st, K_FILL, 4.0 * n);
+  fill_u32_kernel<<<grid_for(n), BLOCK, 0, st>>>(p, n, value);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== scan
+// Three-phase exclusive scan: per-tile reduce, scan of tile sums (recursive),
+// per-tile downsweep.  Each thread owns SCAN_IPT consecutive elements.
+__global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(const u32* __restrict__ in, u64 n,
+                                                            u32* __restrict__ partial) {
+  __shared__ u32 lds[WAVES];
+  const u64 base = u64(blockIdx.x) * SCAN_TILE + u64(threadIdx.x) * SCAN_IPT;
+  u32 sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; ++k) {
+    const u64 i = base + k;
+    sum += (i < n) ? in[i] : 0u;
+  }
+  u32 total;
+  block_exclusive_sum(sum, total, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(BLOCK) void scan_down_kernel(const u32* in, u32* out, u64 n,
+                                                          const u32* __restrict__ partial,
+                                                          u32* total_dev) {
+  __shared__ u32 lds[WAVES];
+  const u64 base = u64(blockIdx.x) * SCAN_TILE + u64(threadIdx.x) * SCAN_IPT;
+  u32 v[SCAN_IPT];
+  u32 sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; ++k) {
+    const u64 i = base + k;
+    v[k] = (i < n) ? in[i] : 0u;
+    sum += v[k];
+  }
+  u32 total;
+  u32 run = block_exclusive_sum(sum, total, lds);
+  const u32 tile_off = partial ? partial[blockIdx.x] : 0u;
+  run += tile_off;
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; ++k) {
+    const u64 i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+  if (total_dev && partial == nullptr && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1)
+    *total_dev = tile_off + total;
+}
+
+u64 scan_scratch_elems(u64 n) {
+  u64 s = 0;
+  while (n > SCAN_TILE) {
+    n = (n + SCAN_TILE - 1) / SCAN_TILE;
+    s += n;
+  }
+  return s + 1;
+}
+
+void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scratch,
+                        u32* total_dev) {
+  if (n == 0) {
+    if (total_dev) fill_u32(st, total_dev, 1, 0);
+    return;
+  }
+  const u32 tiles = ceil_div_u32(n, SCAN_TILE);
+  if (tiles == 1) {
+    KTScope t(st, K_SCAN_DOWN, 8.0 * n);
+    scan_down_kernel<<<1, BLOCK, 0, st>>>(in, out, n, nullptr, total_dev);
+    KMST_HIP(hipGetLastError());
+    return;
+  }
+  u32* partial = scratch;
+  {
+    KTScope t(st, K_SCAN_REDUCE, 4.0 * n);
+    scan_reduce_kernel<<<tiles, BLOCK, 0, st>>>(in, n, partial);
+    KMST_HIP(hipGetLastError());
+  }
+  scan_exclusive_u32(st, partial, partial, tiles, scratch + tiles, total_dev);
+  KTScope t(st, K_SCAN_DOWN, 8.0 * n);
+  scan_down_kernel<<<tiles, BLOCK, 0, st>>>(in, out, n, partial, nullptr);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== setup
+// Contract check of the reference's assumed input: sorted by (src, dst), no
+// self-loops, ids < n (include/util/utils.hpp:391-395).  Counts min<max
+// records so the host can check #forward == #reverse.
+__global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__ src,
+                                                         const u32* __restrict__ dst, u64 m,
+                                                         u64 n, u32* err, u32* fwd_count) {
+  u32 bad = 0;
+  u32 fwd = 0;
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
+    const u32 s = src[i], d = dst[i];
+    if (s >= n || d >= n || s == d) bad |= ERR_INVALID;
+    if (i > 0) {
+      const u32 ps = src[i - 1], pd = dst[i - 1];
+      if (ps > s || (ps == s && pd > d)) bad |= ERR_UNSORTED;
+    }
+    fwd += (s < d) ? 1u : 0u;
+  }
+  // wave-reduce the forward count
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) fwd += __shfl_xor(fwd, o, WAVE);
+  if (lane_id() == 0 && fwd) atomicAdd(fwd_count, fwd);
+  if (bad) atomicOr(err, bad);
+}
+
+void validate_records(hipStream_t st, const u32* src, const u32* dst, u64 m, u64 n, u32* err,
+                      u32* fwd_count) {
+  if (m == 0) return;
+  KTScope t(st, K_VALIDATE, 8.0 * m);
+  validate_kernel<<<grid_for(m, 16384), BLOCK, 0, st>>>(src, dst, m, n, err, fwd_count);
+  KMST_HIP(hipGetLastError());
+}
+
+// off[v] = first record of vertex v, off[n] = m (input already validated).
+__global__ __launch_bounds__(BLOCK) void offsets_kernel(const u32* __restrict__ src, u64 m, u64 n,
+                                                        u32* __restrict__ off) {
+  const u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i >= m) return;
+  const u64 s = src[i];
+  const u64 prev = (i == 0) ? 0 : u64(src[i - 1]) + 1;  // first vertex this record may open
+  for (u64 v = prev; v <= s && v <= n; ++v) off[v] = static_cast<u32>(i);
+  if (i == m - 1)
+    for (u64 v = s + 1; v <= n; ++v) off[v] = static_cast<u32>(m);
+}
+
+void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off) {
+  if (m == 0) {
+    fill_u32(st, off, n + 1, 0);
+    return;
+  }
+  KTScope t(st, K_OFFSETS, 4.0 * m + 4.0 * n);
+  offsets_kernel<<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, m, n, off);
+  KMST_HIP(hipGetLastError());
+}
+
+// Strict key of every record: val = (w << 32) | uid, uid = position of the
+// (min, max) twin.  The reverse record (u, v) with u > v finds its twin in
+// v's segment by binary search; within runs of equal (src, dst) (multigraph
+// inputs) the k-th record of weight w pairs with the k-th twin of weight w.
+__global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src,
+                                                     const u32* __restrict__ dst,
+                                                     const u32* __restrict__ w, u64 m,
+                                                     const u32* __restrict__ off,
+                                                     u64* __restrict__ val, u32* err) {
+  const u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i >= m) return;
+  const u32 s = src[i], d = dst[i], wi = w[i];
+  if (s < d) {
+    val[i] = (u64(wi) << 32) | i;
+    return;
+  }
+  u32 k = 0;  // equal-weight records of the own run before i
+  for (u64 j = i; j > 0; --j) {
+    if (src[j - 1] != s || dst[j - 1] != d) break;
+    k += (w[j - 1] == wi) ? 1u : 0u;
+  }
+  u64 lo = off[d], hi = off[d + 1];
+  while (lo < hi) {
+    const u64 mid = (lo + hi) >> 1;
+    if (dst[mid] < s) lo = mid + 1; else hi = mid;
+  }
+  const u64 end = off[d + 1];
+  for (u64 j = lo; j < end && dst[j] == s; ++j) {
+    if (w[j] == wi) {
+      if (k == 0) {
+        val[i] = (u64(wi) << 32) | j;
+        return;
+      }
+      --k;
+    }
+  }
+  val[i] = VAL_NONE;
+  atomicOr(err, ERR_ASYMMETRIC);
+}
+
+void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
+                const u32* off, u64* val, u32* err) {
+  if (m == 0) return;
+  KTScope t(st, K_KEYS, 20.0 * m);
+  keys_kernel<<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, dst, w, m, off, val, err);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== K1 argmin
+// Segmented argmin over records grouped by src, record-centric so that hub
+// vertices cost no more than their records.  A workgroup owns ARG_TILE
+// consecutive records, ARG_IPT per thread.  Runs inside one thread are
+// finished locally; runs spanning threads are combined by a segmented min
+// scan over the threads' tail pieces; runs spanning tiles go through a u64
+// atomicMin plus a fix-up pass that writes best_dst.
+struct Piece {
+  u64 val;
+  u32 idx;
+  u32 flag;  // segment starts here
+};
+
+__device__ __forceinline__ void seg_combine(Piece& cur, const Piece& prev) {
+  // cur <- prev (+) cur
+  if (!cur.flag) {
+    if (prev.val < cur.val) {
+      cur.val = prev.val;
+      cur.idx = prev.idx;
+    }
+    cur.flag = prev.flag;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ src,
+                                                       const u32* __restrict__ dst,
+                                                       const u64* __restrict__ val, u64 m,
+                                                       u64* __restrict__ best_val,
+                                                       u32* __restrict__ best_dst,
+                                                       ArgPiece* __restrict__ pieces,
+                                                       u32* piece_count) {
+  __shared__ u32 s_tail_src[BLOCK];
+  __shared__ u32 s_first_src[BLOCK];
+  __shared__ u64 s_scan_val[BLOCK];
+  __shared__ u32 s_scan_idx[BLOCK];
+  __shared__ u64 s_wave_val[WAVES];
+  __shared__ u32 s_wave_idx[WAVES];
+  __shared__ u32 s_wave_flag[WAVES];
+
+  const int t = threadIdx.x;
+  const u64 tile_base = u64(blockIdx.x) * ARG_TILE;
+  const u64 tile_end = (tile_base + ARG_TILE < m) ? tile_base + ARG_TILE : m;
+  const u64 my_base = tile_base + u64(t) * ARG_IPT;
+
+  // tile context: does the first run continue from the previous tile, does the
+  // last run continue into the next one?
+  const u32 tile_first_src = src[tile_base];
+  const u32 tile_last_src = src[tile_end - 1];
+  const bool cross_prev = tile_base > 0 && src[tile_base - 1] == tile_first_src;
+  const bool cross_next = tile_end < m && src[tile_end] == tile_last_src;
+
+  bool has_items = my_base < tile_end;
+  u32 head_src = NONE32, cur_src = NONE32;
+  u64 head_val = VAL_NONE, cur_val = VAL_NONE;
+  u32 head_idx = 0, cur_idx = 0;
+  bool head_done = false;  // first run ended inside this thread
+  if (has_items) {
+    const u64 my_end = (my_base + ARG_IPT < tile_end) ? my_base + ARG_IPT : tile_end;
+    cur_src = src[my_base];
+    cur_val = val[my_base];
+    cur_idx = static_cast<u32>(my_base);
+    for (u64 i = my_base + 1; i < my_end; ++i) {
+      const u32 s = src[i];
+      const u64 v = val[i];
+      if (s != cur_src) {
+        if (!head_done) {
+          head_src = cur_src;
+          head_val = cur_val;
+          head_idx = cur_idx;
+          head_done = true;
+        } else {  // interior run: complete inside this thread
+          best_val[cur_src] = cur_val;
+          best_dst[cur_src] = dst[cur_idx];
+        }
+        cur_src = s;
+        cur_val = v;
+        cur_idx = static_cast<u32>(i);
+      } else if (v < cur_val) {
+        cur_val = v;
+        cur_idx = static_cast<u32>(i);
+      }
+    }
+  }
+  const u32 first_src = head_done ? head_src : cur_src;
+  s_tail_src[t] = cur_src;
+  s_first_src[t] = first_src;
+  __syncthreads();
+
+  // segmented inclusive min-scan over tail pieces
+  const u32 prev_tail = (t > 0) ? s_tail_src[t - 1] : NONE32;
+  Piece p;
+  p.val = cur_val;
+  p.idx = cur_idx;
+  p.flag = (!has_items || head_done || t == 0 || prev_tail != cur_src) ? 1u : 0u;
+  const int lane = lane_id(), wid = wave_id();
+#pragma unroll
+  for (int dd = 1; dd < WAVE; dd <<= 1) {
+    Piece q;
+    q.val = __shfl_up(p.val, dd, WAVE);
+    q.idx = __shfl_up(p.idx, dd, WAVE);
+    q.flag = __shfl_up(p.flag, dd, WAVE);
+    if (lane >= dd) seg_combine(p, q);
+  }
+  if (lane == WAVE - 1) {
+    s_wave_val[wid] = p.val;
+    s_wave_idx[wid] = p.idx;
+    s_wave_flag[wid] = p.flag;
+  }
+  __syncthreads();
+  if (wid > 0 && !p.flag) {
+    Piece carry{s_wave_val[0], s_wave_idx[0], s_wave_flag[0]};
+    for (int w = 1; w < wid; ++w) {
+      Piece nx{s_wave_val[w], s_wave_idx[w], s_wave_flag[w]};
+      seg_combine(nx, carry);
+      carry = nx;
+    }
+    seg_combine(p, carry);
+  }
+  s_scan_val[t] = p.val;
+  s_scan_idx[t] = p.idx;
+  __syncthreads();
+
+  if (!has_items) return;
+
+  // emit one finished run: direct store, or atomic + fix-up when it crosses
+  // a tile boundary
+  auto emit = [&](u32 s, u64 v, u32 idx) {
+    const bool crosses = (cross_prev && s == tile_first_src) || (cross_next && s == tile_last_src);
+    if (!crosses) {
+      best_val[s] = v;
+      best_dst[s] = dst[idx];
+    } else {
+      atomicMin(reinterpret_cast<unsigned long long*>(best_val + s),
+                static_cast<unsigned long long>(v));
+      const u32 slot = atomicAdd(piece_count, 1u);
+      pieces[slot] = ArgPiece{v, s, idx};
+    }
+  };
+
+  // head run (thread has more than one run)
+  if (head_done) {
+    if (t > 0 && prev_tail == head_src) {
+      // run started in an earlier thread and ends here
+      const u64 pv = s_scan_val[t - 1];
+      if (pv < head_val) emit(head_src, pv, s_scan_idx[t - 1]);
+      else emit(head_src, head_val, head_idx);
+    } else {
+      emit(head_src, head_val, head_idx);
+    }
+  }
+  // tail run: this thread finishes it when the next piece has another src
+  const bool last_thread = (t == BLOCK - 1) || (my_base + ARG_IPT >= tile_end);
+  const u32 next_first = last_thread ? NONE32 : s_first_src[t + 1];
+  if (last_thread || next_first != cur_src) emit(cur_src, p.val, p.idx);
+}
+
+__global__ __launch_bounds__(BLOCK) void argmin_fixup_kernel(const ArgPiece* __restrict__ pieces,
+                                                             const u32* piece_count,
+                                                             const u32* __restrict__ dst,
+                                                             const u64* __restrict__ best_val,
+                                                             u32* __restrict__ best_dst) {
+  const u32 cnt = *piece_count;
+  for (u32 i = blockIdx.x * BLOCK + threadIdx.x; i < cnt; i += gridDim.x * BLOCK) {
+    const ArgPiece pc = pieces[i];
+    if (best_val[pc.src] == pc.val) best_dst[pc.src] = dst[pc.idx];
+  }
+}
+
+void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const u64* val, u64 m,
+                      u64 n, u64* best_val, u32* best_dst, ArgPiece* pieces, u32* piece_count) {
+  fill_u64(st, best_val, n, VAL_NONE);
+  fill_u32(st, piece_count, 1, 0);
+  if (m == 0) return;
+  const u32 tiles = ceil_div_u32(m, ARG_TILE);
+  {
+    KTScope t(st, K_ARGMIN, 12.0 * m + 16.0 * n);
+    argmin_kernel<<<tiles, BLOCK, 0, st>>>(src, dst, val, m, best_val, best_dst, pieces,
+                                           piece_count);
+    KMST_HIP(hipGetLastError());
+  }
+  KTScope t(st, K_ARGMIN_FIXUP, 32.0 * tiles);
+  argmin_fixup_kernel<<<grid_for(2ull * tiles, 1024), BLOCK, 0, st>>>(pieces, piece_count, dst,
+                                                                      best_val, best_dst);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== K2 hook
+// Each vertex hooks onto the other endpoint of its lightest edge; of a mutual
+// pair the lower endpoint stays root (gbbs_reimplementation.hpp:117-127).
+// Non-roots append their edge's uid to the MST.
+__global__ __launch_bounds__(BLOCK) void hook_kernel(u64 n, const u64* __restrict__ best_val,
+                                                     const u32* __restrict__ best_dst,
+                                                     u32* __restrict__ parent,
+                                                     u32* __restrict__ root_flag,
+                                                     u32* __restrict__ mst, u32* mst_count) {
+  const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  bool emit = false;
+  u32 uid = 0;
+  if (v < n) {
+    const u64 bv = best_val[v];
+    if (bv == VAL_NONE) {
+      parent[v] = static_cast<u32>(v);
+      root_flag[v] = 0;
+    } else {
+      const u32 u = best_dst[v];
+      const bool mutual = best_val[u] == bv;
+      if (mutual && v < u) {
+        parent[v] = static_cast<u32>(v);
+        root_flag[v] = 1;
+      } else {
+        parent[v] = u;
+        root_flag[v] = 0;
+        emit = true;
+        uid = static_cast<u32>(bv);
+      }
+    }
+  }
+  const u32 slot = wave_append(emit, mst_count);
+  if (emit) mst[slot] = uid;
+}
+
+void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* parent,
+          u32* root_flag, u32* mst, u32* mst_count) {
+  if (n == 0) return;
+  KTScope t(st, K_HOOK, 28.0 * n);
+  hook_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, best_val, best_dst, parent, root_flag,
+                                                         mst, mst_count);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== K3 jump
+// Label doubling to the fixpoint in one launch: every vertex keeps replacing
+// its parent by its grandparent until it points at a root; the shortcuts it
+// writes are ancestors, so concurrent chases only speed each other up.  The
+// hook graph is a forest (the only 2-cycles are broken by the root rule), so
+// every chase ends; a loop bound turns a corrupted forest into an error.
+__global__ __launch_bounds__(BLOCK) void jump_kernel(u64 n, u32* parent, u32* err) {
+  const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  if (v >= n) return;
+  u32 x = parent[v];
+  u32 y = parent[x];
+  u32 guard = 0;
+  while (x != y) {
+    parent[v] = y;
+    x = y;
+    y = parent[x];
+    if (++guard > (1u << 26)) {
+      atomicOr(err, ERR_SPIN);
+      return;
+    }
+  }
+  parent[v] = x;
+}
+
+void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err) {
+  if (n == 0) return;
+  KTScope t(st, K_JUMP, 12.0 * n);
+  jump_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, parent, err);
+  KMST_HIP(hipGetLastError());
+}
+
+// label[v] = new consecutive id of v's root, NONE32 for finished vertices.
+__global__ __launch_bounds__(BLOCK) void label_kernel(u64 n, const u32* __restrict__ parent,
+                                                      const u32* __restrict__ root_flag,
+                                                      const u32* __restrict__ new_id,
+                                                      u32* __restrict__ label) {
+  const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  if (v >= n) return;
+  const u32 r = parent[v];
+  label[v] = root_flag[r] ? new_id[r] : NONE32;
+}
+
+void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_flag, const u32* new_id,
+                 u32* label) {
+  if (n == 0) return;
+  KTScope t(st, K_LABEL, 16.0 * n);
+  label_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, parent, root_flag, new_id, label);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== K4+K5
+// Relabel both endpoints, drop self-loops and dead records, keep order.
+__device__ __forceinline__ bool keep_relabel(u32 s, u32 d, u64 v, const u32* label, u32& ls,
+                                             u32& ld) {
+  if (v == VAL_NONE) return false;
+  ls = label[s];
+  ld = label[d];
+  return ls != ld;
+}
+
+__global__ __launch_bounds__(BLOCK) void relabel_count_kernel(const u32* __restrict__ s_in,
+                                                              const u32* __restrict__ d_in,
+                                                              const u64* __restrict__ v_in, u64 m,
+                                                              const u32* __restrict__ label,
+                                                              u32* __restrict__ tile_counts) {
+  __shared__ u32 lds[WAVES];
+  const u64 base = u64(blockIdx.x) * CMP_TILE + u64(threadIdx.x) * CMP_IPT;
+  u32 c = 0;
+#pragma unroll 4
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u64 i = base + k;
+    if (i < m) {
+      u32 ls, ld;
+      c += keep_relabel(s_in[i], d_in[i], v_in[i], label, ls, ld) ? 1u : 0u;
+    }
+  }
+  u32 total;
+  block_exclusive_sum(c, total, lds);
+  if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(BLOCK) void relabel_scatter_kernel(
+    const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in, u64 m,
+    const u32* __restrict__ label, const u32* __restrict__ tile_off, u32* __restrict__ s_out,
+    u32* __restrict__ d_out, u64* __restrict__ v_out) {
+  __shared__ u32 lds[WAVES];
+  const u64 base = u64(blockIdx.x) * CMP_TILE + u64(threadIdx.x) * CMP_IPT;
+  u32 ls[CMP_IPT], ld[CMP_IPT];
+  u64 vv[CMP_IPT];
+  u32 keepmask = 0, c = 0;
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u64 i = base + k;
+    vv[k] = VAL_NONE;
+    ls[k] = ld[k] = 0;
+    if (i < m) {
+      vv[k] = v_in[i];
+      if (keep_relabel(s_in[i], d_in[i], vv[k], label, ls[k], ld[k])) {
+        keepmask |= 1u << k;
+        ++c;
+      }
+    }
+  }
+  u32 total;
+  u32 pos = block_exclusive_sum(c, total, lds) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    if (keepmask & (1u << k)) {
+      s_out[pos] = ls[k];
+      d_out[pos] = ld[k];
+      v_out[pos] = vv[k];
+      ++pos;
+    }
+  }
+}
+
+void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
+                     const u32* label, u32* s_out, u32* d_out, u64* v_out, u32* tile_counts,
+                     u32* scan_scratch, u32* count_dev) {
+  if (m == 0) {
+    fill_u32(st, count_dev, 1, 0);
+    return;
+  }
+  const u32 tiles = ceil_div_u32(m, CMP_TILE);
+  {
+    KTScope t(st, K_RELABEL_COUNT, 24.0 * m);
+    relabel_count_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, label, tile_counts);
+    KMST_HIP(hipGetLastError());
+  }
+  scan_exclusive_u32(st, tile_counts, tile_counts, tiles, scan_scratch, count_dev);
+  KTScope t(st, K_RELABEL_SCATTER, 24.0 * m + 16.0 * m);
+  relabel_scatter_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, label, tile_counts, s_out,
+                                                   d_out, v_out);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== K6 sort
+// Classic LSD pass: per-tile digit histogram, exclusive scan over
+// (digit, tile), stable scatter.  Each wave owns a contiguous sub-tile and
+// ranks its items in index order with 8-ballot digit matching, so the pass is
+// stable without block-wide synchronisation per item.
+template <bool HAS_D>
+__device__ __forceinline__ u64 sort_key(const u32* s, const u32* d, u64 i, u32 dbits) {
+  return HAS_D ? ((u64(s[i]) << dbits) | d[i]) : u64(s[i]);
+}
+
+template <bool HAS_D>
+__global__ __launch_bounds__(BLOCK) void radix_hist_kernel(const u32* __restrict__ s,
+                                                           const u32* __restrict__ d, u64 m,
+                                                           u32 dbits, u32 shift,
+                                                           u32* __restrict__ hist, u32 tiles) {
+  __shared__ u32 h[WAVES][RADIX];
+  for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const int w = wave_id();
+  const u64 base = u64(blockIdx.x) * SORT_TILE;
+#pragma unroll 4
+  for (int k = 0; k < SORT_IPT; ++k) {
+    const u64 e = base + u64(k) * BLOCK + threadIdx.x;
+    if (e < m) {
+      const u32 digit = static_cast<u32>((sort_key<HAS_D>(s, d, e, dbits) >> shift) & (RADIX - 1));
+      atomicAdd(&h[w][digit], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < RADIX; i += BLOCK) {
+    u32 c = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) c += h[ww][i];
+    hist[u64(i) * tiles + blockIdx.x] = c;
+  }
+}
+
+template <bool HAS_D, bool HAS_V>
+__global__ __launch_bounds__(BLOCK) void radix_scatter_kernel(
+    const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in,
+    u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out, u64 m, u32 dbits,
+    u32 shift, const u32* __restrict__ hist_scan, u32 tiles) {
+  __shared__ u32 wcnt[WAVES][RADIX];
+  for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const int lane = lane_id(), w = wave_id();
+  const u64 base = u64(blockIdx.x) * SORT_TILE + u64(w) * (WAVE * SORT_IPT);
+  u32 ks[SORT_IPT], kd[SORT_IPT], dig[SORT_IPT], rank[SORT_IPT];
+  u64 kv[SORT_IPT];
+#pragma unroll
+  for (int k = 0; k < SORT_IPT; ++k) {
+    const u64 e = base + u64(k) * WAVE + lane;
+    const bool valid = e < m;
+    ks[k] = valid ? s_in[e] : 0u;
+    kd[k] = (HAS_D && valid) ? d_in[e] : 0u;
+    kv[k] = (HAS_V && valid) ? v_in[e] : 0ull;
+    const u64 key = HAS_D ? ((u64(ks[k]) << dbits) | kd[k]) : u64(ks[k]);
+    const u32 digit = valid ? static_cast<u32>((key >> shift) & (RADIX - 1)) : u32(RADIX);
+    u64 peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RADIX_BITS; ++b) {
+      const bool bit = (digit >> b) & 1u;
+      const u64 bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    dig[k] = digit;
+    rank[k] = 0;
+    if (valid) {
+      const u32 old = wcnt[w][digit];
+      rank[k] = old + mbcnt(peers);
+      if (lane == __ffsll(static_cast<long long>(peers)) - 1)
+        wcnt[w][digit] = old + static_cast<u32>(__popcll(peers));
+    }
+  }
+  __syncthreads();
+  for (int dg = threadIdx.x; dg < RADIX; dg += BLOCK) {
+    u32 run = hist_scan[u64(dg) * tiles + blockIdx.x];
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) {
+      const u32 c = wcnt[ww][dg];
+      wcnt[ww][dg] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SORT_IPT; ++k) {
+    if (dig[k] < RADIX) {
+      const u32 pos = wcnt[w][dig[k]] + rank[k];
+      s_out[pos] = ks[k];
+      if (HAS_D) d_out[pos] = kd[k];
+      if (HAS_V) v_out[pos] = kv[k];
+    }
+  }
+}
+
+u64 sort_hist_elems(u64 m) { return u64(RADIX) * ceil_div_u32(m ? m : 1, SORT_TILE); }
+
+bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, u64* v_alt, u64 m,
+                u32 dbits, u32 total_bits, u32* hist, u32* scan_scratch) {
+  if (m <= 1 || total_bits == 0) return false;
+  const u32 tiles = ceil_div_u32(m, SORT_TILE);
+  const u32 passes = (total_bits + RADIX_BITS - 1) / RADIX_BITS;
+  bool alt = false;
+  for (u32 p = 0; p < passes; ++p) {
+    const u32 shift = p * RADIX_BITS;
+    u32* si = alt ? s_alt : s;
+    u32* di = alt ? d_alt : d;
+    u64* vi = alt ? v_alt : v;
+    u32* so = alt ? s : s_alt;
+    u32* dout = alt ? d : d_alt;
+    u64* vo = alt ? v : v_alt;
+    {
+      KTScope t(st, K_SORT_HIST, (d ? 8.0 : 4.0) * m);
+      if (d) radix_hist_kernel<true><<<tiles, BLOCK, 0, st>>>(si, di, m, dbits, shift, hist, tiles);
+      else radix_hist_kernel<false><<<tiles, BLOCK, 0, st>>>(si, di, m, dbits, shift, hist, tiles);
+      KMST_HIP(hipGetLastError());
+    }
+    scan_exclusive_u32(st, hist, hist, u64(RADIX) * tiles, scan_scratch, nullptr);
+    KTScope t(st, K_SORT_SCATTER, 2.0 * m * (4.0 + (d ? 4.0 : 0.0) + (v ? 8.0 : 0.0)));
+    if (d && v)
+      radix_scatter_kernel<true, true><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m, dbits,
+                                                                shift, hist, tiles);
+    else if (d)
+      radix_scatter_kernel<true, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m, dbits,
+                                                                 shift, hist, tiles);
+    else if (v)
+      radix_scatter_kernel<false, true><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m, dbits,
+                                                                 shift, hist, tiles);
+    else
+      radix_scatter_kernel<false, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
+                                                                  dbits, shift, hist, tiles);
+    KMST_HIP(hipGetLastError());
+    alt = !alt;
+  }
+  return alt;
+}
+
+// ============================================================== K7 dedup
+// After the (src, dst) sort, keep the lightest record of each run of equal
+// (src, dst) (EdgeProcessor::remove_duplicates, duplicate_detection.hpp:26-64)
+// and mark the rest dead.  Runs are cut every 64 records so a head thread
+// scans a bounded range; a long run keeps one survivor per 64 records, which
+// later rounds remove.
+__global__ __launch_bounds__(BLOCK) void dedup_kernel(const u32* __restrict__ s,
+                                                      const u32* __restrict__ d,
+                                                      u64* __restrict__ v, u64 m) {
+  const u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i >= m) return;
+  const u32 si = s[i], di = d[i];
+  const bool head = (i % WAVE == 0) || s[i - 1] != si || d[i - 1] != di;
+  if (!head) return;
+  u64 j = i + 1;
+  while (j < m && (j % WAVE) != 0 && s[j] == si && d[j] == di) ++j;
+  if (j == i + 1) return;
+  u64 mn = v[i];
+  for (u64 k = i + 1; k < j; ++k) {
+    const u64 x = v[k];
+    mn = x < mn ? x : mn;
+  }
+  v[i] = mn;
+  for (u64 k = i + 1; k < j; ++k) v[k] = VAL_NONE;
+}
+
+void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m) {
+  if (m == 0) return;
+  KTScope t(st, K_DEDUP, 16.0 * m);
+  dedup_kernel<<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(s, d, v, m);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== K9 gather
+__global__ __launch_bounds__(BLOCK) void gather_kernel(const u32* __restrict__ uids, u64 count,
+                                                       const u32* __restrict__ in_src,
+                                                       const u32* __restrict__ in_dst,
+                                                       const u32* __restrict__ in_w,
+                                                       u32* __restrict__ out_s,
+                                                       u32* __restrict__ out_d,
+                                                       u32* __restrict__ out_w, u64* total) {
+  u64 sum = 0;
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < count; i += u64(gridDim.x) * BLOCK) {
+    const u32 uid = uids[i];
+    const u32 a = in_src[uid], b = in_dst[uid], w = in_w[uid];
+    out_s[i] = a < b ? a : b;
+    out_d[i] = a < b ? b : a;
+    out_w[i] = w;
+    sum += w;
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, WAVE);
+  if (lane_id() == 0 && sum) atomicAdd(reinterpret_cast<unsigned long long*>(total), sum);
+}
+
+void gather_mst(hipStream_t st, const u32* uids, u64 count, const u32* in_src, const u32* in_dst,
+                const u32* in_w, u32* out_s, u32* out_d, u32* out_w, u64* total_dev) {
+  if (count == 0) return;
+  KTScope t(st, K_GATHER, 28.0 * count);
+  gather_kernel<<<grid_for(count, 4096), BLOCK, 0, st>>>(uids, count, in_src, in_dst, in_w, out_s,
+                                                         out_d, out_w, total_dev);
+  KMST_HIP(hipGetLastError());
+}
+
+// ============================================================== filter
+__global__ __launch_bounds__(BLOCK) void iota_kernel(u32* p, u64 n) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += u64(gridDim.x) * BLOCK)
+    p[i] = static_cast<u32>(i);
+}
+
+void iota_u32(hipStream_t st, u32* p, u64 n) {
+  if (n == 0) return;
+  KTScope t(st, K_FILL, 4.0 * n);
+  iota_kernel<<<grid_for(n), BLOCK, 0, st>>>(p, n);
+  KMST_HIP(hipGetLastError());
+}
+
+// Parent-array update of one round (UpdateParentArray in filter_boruvka_step,
+// include/algorithms/hybrid_boruvka_edgefilter.hpp:91-95): every live vertex c
+// of this round, named rep[c] in the original numbering (identity in the first
+// round), points at its root's name; every surviving root passes its name to
+// its new id.
+__global__ __launch_bounds__(BLOCK) void gpar_kernel(u64 n, const u32* __restrict__ rep,
+                                                     const u32* __restrict__ parent,
+                                                     const u32* __restrict__ root_flag,
+                                                     const u32* __restrict__ new_id,
+                                                     const u64* __restrict__ best_val,
+                                                     u32* __restrict__ gpar,
+                                                     u32* __restrict__ rep_next) {
+  const u64 c = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  if (c >= n) return;
+  const u32 name = rep ? rep[c] : static_cast<u32>(c);
+  if (best_val[c] != VAL_NONE) {
+    const u32 p = parent[c];
+    gpar[name] = rep ? rep[p] : p;
+  }
+  if (root_flag[c]) rep_next[new_id[c]] = name;
+}
+
+void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* parent,
+                         const u32* root_flag, const u32* new_id, const u64* best_val, u32* gpar,
+                         u32* rep_next) {
+  if (n == 0) return;
+  KTScope t(st, K_GPAR, 32.0 * n);
+  gpar_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, rep, parent, root_flag, new_id,
+                                                         best_val, gpar, rep_next);
+  KMST_HIP(hipGetLastError());
+}
+
+// k evenly spaced weight samples (NONE32 for dead records); the pivot only
+// changes speed, never the result.
+__global__ __launch_bounds__(BLOCK) void sample_kernel(const u64* __restrict__ v, u64 m, u32 k,
+                                                       u32* __restrict__ out) {
+  const u32 j = blockIdx.x * BLOCK + threadIdx.x;
+  if (j >= k) return;
+  const u64 i = (u64(j) * m) / k + m / (2ull * k);
+  const u64 x = v[i < m ? i : m - 1];
+  out[j] = (x == VAL_NONE) ? NONE32 : static_cast<u32>(x >> 32);
+}
+
+void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32* out) {
+  if (k == 0 || m == 0) return;
+  KTScope t(st, K_SAMPLE, 12.0 * k);
+  sample_kernel<<<ceil_div_u32(k, BLOCK), BLOCK, 0, st>>>(v, m, k, out);
+  KMST_HIP(hipGetLastError());
+}
+
+// Stable two-way split by weight (partition_edges, filter_helpers.hpp:101-139):
+// light (w <= pivot) records to out[0, L), heavy ones to out[L, m), both in
+// input order.
+__global__ __launch_bounds__(BLOCK) void split_count_kernel(const u64* __restrict__ v, u64 m,
+                                                            u32 pivot, u32* __restrict__ counts) {
+  __shared__ u32 lds[WAVES];
+  const u64 base = u64(blockIdx.x) * CMP_TILE + u64(threadIdx.x) * CMP_IPT;
+  u32 c = 0;
+#pragma unroll 4
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u64 i = base + k;
+    if (i < m) c += (static_cast<u32>(v[i] >> 32) <= pivot) ? 1u : 0u;
+  }
+  u32 total;
+  block_exclusive_sum(c, total, lds);
+  if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(BLOCK) void split_scatter_kernel(
+    const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in, u64 m,
+    u32 pivot, const u32* __restrict__ light_off, const u32* light_total, u32* __restrict__ s_out,
+    u32* __restrict__ d_out, u64* __restrict__ v_out) {
+  __shared__ u32 lds[WAVES];
+  const u64 tile_base = u64(blockIdx.x) * CMP_TILE;
+  const u64 base = tile_base + u64(threadIdx.x) * CMP_IPT;
+  u32 lightmask = 0, cl = 0, valid = 0;
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u64 i = base + k;
+    if (i < m) {
+      ++valid;
+      if (static_cast<u32>(v_in[i] >> 32) <= pivot) {
+        lightmask |= 1u << k;
+        ++cl;
+      }
+    }
+  }
+  u32 tot_l, tot_v;
+  const u32 ex_l = block_exclusive_sum(cl, tot_l, lds);
+  const u32 ex_v = block_exclusive_sum(valid, tot_v, lds);
+  const u32 L = *light_total;
+  const u32 tile_l = light_off[blockIdx.x];
+  u32 pl = tile_l + ex_l;
+  u32 ph = L + static_cast<u32>(tile_base - tile_l) + (ex_v - ex_l);
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u64 i = base + k;
+    if (i < m) {
+      const u32 pos = (lightmask & (1u << k)) ? pl++ : ph++;
+      s_out[pos] = s_in[i];
+      d_out[pos] = d_in[i];
+      v_out[pos] = v_in[i];
+    }
+  }
+}
+
+void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
+                       u32 pivot, u32* s_out, u32* d_out, u64* v_out, u32* tile_counts,
+                       u32* scan_scratch, u32* light_count_dev) {
+  if (m == 0) {
+    fill_u32(st, light_count_dev, 1, 0);
+    return;
+  }
+  const u32 tiles = ceil_div_u32(m, CMP_TILE);
+  {
+    KTScope t(st, K_SPLIT_COUNT, 8.0 * m);
+    split_count_kernel<<<tiles, BLOCK, 0, st>>>(v_in, m, pivot, tile_counts);
+    KMST_HIP(hipGetLastError());
+  }
+  scan_exclusive_u32(st, tile_counts, tile_counts, tiles, scan_scratch, light_count_dev);
+  KTScope t(st, K_SPLIT_SCATTER, 32.0 * m);
+  split_scatter_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, pivot, tile_counts,
+                                                 light_count_dev, s_out, d_out, v_out);
+  KMST_HIP(hipGetLastError());
+}
+
+}  // namespace kmst

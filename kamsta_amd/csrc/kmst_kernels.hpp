@@ -1,0 +1,88 @@
+// kmst_kernels.hpp -- host-side launchers of the libkmst HIP kernels.
+// Every launcher enqueues work on `st` and returns immediately.
+#pragma once
+
+#include "kmst_common.hpp"
+
+namespace kmst {
+
+// Tile shapes (elements per workgroup).
+constexpr int SCAN_IPT = 16;
+constexpr int SCAN_TILE = BLOCK * SCAN_IPT;      // 4096
+constexpr int SORT_IPT = 16;
+constexpr int SORT_TILE = BLOCK * SORT_IPT;      // 4096
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int CMP_IPT = 16;
+constexpr int CMP_TILE = BLOCK * CMP_IPT;        // 4096
+constexpr int ARG_IPT = 8;
+constexpr int ARG_TILE = BLOCK * ARG_IPT;        // 2048
+
+// Boundary piece of the segmented argmin: a vertex whose records cross a tile
+// boundary; the per-tile partial minimum is resolved in a fix-up pass.
+struct ArgPiece {
+  u64 val;
+  u32 src;
+  u32 idx;
+};
+
+void fill_u64(hipStream_t st, u64* p, u64 n, u64 value);
+void fill_u32(hipStream_t st, u32* p, u64 n, u32 value);
+
+// Exclusive prefix sum of in[0,n) into out (may alias in); *total_dev gets the
+// sum when non-null.  `scratch` needs scan_scratch_elems(n) entries.
+u64 scan_scratch_elems(u64 n);
+void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scratch,
+                        u32* total_dev);
+
+// Setup: contract checks, CSR offsets, strict keys.
+void validate_records(hipStream_t st, const u32* src, const u32* dst, u64 m, u64 n,
+                      u32* err, u32* fwd_count);
+void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off);
+void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
+                const u32* off, u64* val, u32* err);
+
+// K1 segmented argmin over records grouped by src.
+void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const u64* val,
+                      u64 m, u64 n, u64* best_val, u32* best_dst, ArgPiece* pieces,
+                      u32* piece_count);
+
+// K2 hook; K3 pointer jumping; vertex renumbering label pass.
+void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* parent,
+          u32* root_flag, u32* mst, u32* mst_count);
+void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err);
+void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_flag,
+                 const u32* new_id, u32* label);
+
+// K4+K5 relabel, drop self-loops and dead records, order-preserving compaction.
+// tile_counts needs ceil(m / CMP_TILE) entries; *count_dev gets the kept count.
+void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in,
+                     u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
+                     u32* tile_counts, u32* scan_scratch, u32* count_dev);
+
+// K6 stable LSD radix sort by key (s << dbits) | d over total_bits bits.
+// Returns true when the sorted data ended in the *_alt arrays.  d / v (and
+// their alt) may be null.  hist needs sort_hist_elems(m) entries.
+u64 sort_hist_elems(u64 m);
+bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, u64* v_alt,
+                u64 m, u32 dbits, u32 total_bits, u32* hist, u32* scan_scratch);
+
+// K7 keep the lightest record of each (s, d) run (others marked dead).
+void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m);
+
+// K9 canonical MST gather + u64 weight.
+void gather_mst(hipStream_t st, const u32* uids, u64 count, const u32* in_src,
+                const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d, u32* out_w,
+                u64* total_dev);
+
+// Filter-Boruvka helpers.
+void iota_u32(hipStream_t st, u32* p, u64 n);
+void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* parent,
+                         const u32* root_flag, const u32* new_id, const u64* best_val, u32* gpar,
+                         u32* rep_next);
+void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32* out);
+void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
+                       u32 pivot, u32* s_out, u32* d_out, u64* v_out, u32* tile_counts,
+                       u32* scan_scratch, u32* light_count_dev);
+
+}  // namespace kmst

@@ -1,0 +1,715 @@
+// kmst_solver.hip -- host orchestration of the Boruvka / filter-Boruvka round
+// loop on one MI355X and the C ABI declared in include/kmst.h.
+//
+// Plain Boruvka (hybridMST::boruvka, include/algorithms/hybrid_boruvka.hpp:263-301):
+//   setup    validate contract, CSR offsets, strict keys   (UncompressedGraph ids)
+//   round    K1 segmented argmin       (MinimumEdgeOpenMP, compute_min_edges)
+//            K2 hook + MST append      (IsRepresentative / determine_mst_edges)
+//            K3 label doubling         (ComputeRepresentative / update_parents)
+//            renumber roots            (MakeVerticesConsecutive)
+//            K4+K5 relabel + compact   (EdgeRenamer + remove_self_loops)
+//            K6 radix sort (src, dst)  (RedistributeViaSelectedSorter)
+//            K7 keep lightest copy     (EdgeProcessor::remove_duplicates)
+//   finish   K9 gather canonical edges (GetMstEdge)
+// Filter-Boruvka (hybridMST::filter_boruvka,
+// include/algorithms/hybrid_boruvka_edgefilter.hpp:227-330) wraps the same
+// rounds in the pivot recursion: light/heavy stream split, recurse on light,
+// relabel heavy by the parent array, drop same-component edges, recurse.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <vector>
+
+#include "kmst_kernels.hpp"
+#include "kmst_solver.hpp"
+
+namespace kmst {
+
+void gen_gnm_device(kmst_context& C, u64 n, u64 m, u32 max_w, u64 seed, u32* out_s, u32* out_d,
+                    u32* out_w);
+void gen_gnm_host(u64 n, u64 m, u32 max_w, u64 seed, u32* out_s, u32* out_d, u32* out_w);
+
+thread_local std::string g_last_error;
+thread_local KernelTimer* g_ktimer = nullptr;
+
+// ------------------------------------------------------------ kernel timer
+void KernelTimer::begin(hipStream_t st, int kind, double bytes) {
+  if (used == pool.size()) {
+    hipEvent_t a, b;
+    KMST_HIP(hipEventCreate(&a));
+    KMST_HIP(hipEventCreate(&b));
+    pool.push_back({a, b});
+  }
+  KMST_HIP(hipEventRecord(pool[used].first, st));
+  marks.push_back({kind, bytes});
+}
+void KernelTimer::end(hipStream_t st) noexcept {
+  (void)hipEventRecord(pool[used].second, st);
+  ++used;
+}
+void KernelTimer::collect() {
+  for (size_t i = 0; i < used; ++i) {
+    float ms = 0.f;
+    KMST_HIP(hipEventElapsedTime(&ms, pool[i].first, pool[i].second));
+    auto& acc = totals[marks[i].kind];
+    acc.launches += 1;
+    acc.ms += ms;
+    acc.bytes += marks[i].bytes;
+  }
+  used = 0;
+  marks.clear();
+}
+KernelTimer::~KernelTimer() {
+  for (auto& e : pool) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+}
+
+const char* kernel_kind_name(int k) {
+  static const char* names[] = {"fill", "scan_reduce", "scan_down", "validate", "offsets",
+                                "keys", "argmin", "argmin_fixup", "hook", "jump", "label",
+                                "gpar", "relabel_count", "relabel_scatter", "sort_hist",
+                                "sort_scatter", "dedup", "sample", "split_count",
+                                "split_scatter", "gather", "gen"};
+  return (k >= 0 && k < K_COUNT) ? names[k] : "?";
+}
+
+// ------------------------------------------------------------ buffers
+void* DevBuf::get(size_t need) {
+  if (need == 0) need = 1;
+  if (need > bytes) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, need) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      throw Error(KMST_ENOMEM, "hipMalloc of " + std::to_string(need) + " bytes failed");
+    }
+    bytes = need;
+  }
+  return p;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+}  // namespace kmst
+
+using namespace kmst;
+
+// ------------------------------------------------------------ context
+kmst_context::kmst_context(int dev) : device(dev) {
+  KMST_HIP(hipSetDevice(device));
+  KMST_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_counters), 256, hipHostMallocDefault));
+}
+kmst_context::~kmst_context() {
+  (void)hipSetDevice(device);
+  (void)hipStreamSynchronize(stream);
+  for (DevBuf* b : all_bufs()) b->release();
+  if (host_counters) (void)hipHostFree(host_counters);
+  (void)hipStreamDestroy(stream);
+}
+std::vector<DevBuf*> kmst_context::all_bufs() {
+  return {&xs, &xd, &xv, &ys, &yd, &yv, &val0, &off, &best_val, &best_dst, &parent, &flag,
+          &newid, &label, &rep, &rep_next, &gpar, &mst, &pieces, &tiles, &scan_scratch, &hist,
+          &counters, &in_s, &in_d, &in_w, &out_s, &out_d, &out_w, &samples, &gen_a, &gen_b,
+          &gen_c, &gen_d};
+}
+
+namespace kmst {
+
+// Device counter slots (u32 words; the u64 total lives at word 16).
+enum Ctr : int { C_ERR = 0, C_FWD = 1, C_MST = 2, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6 };
+
+struct RecView {
+  u32* s = nullptr;
+  u32* d = nullptr;
+  u64* v = nullptr;
+  u64 m = 0;
+  RecView at(u64 off, u64 cnt) const { return RecView{s + off, d + off, v + off, cnt}; }
+};
+
+class Solver {
+ public:
+  Solver(kmst_context& c, const u32* is, const u32* id, const u32* iw, u64 m, u64 n)
+      : C(c), st(c.stream), in_s(is), in_d(id), in_w(iw), m0(m), n0(static_cast<u32>(n)) {}
+
+  void run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64 out_cap, u64* out_n,
+           u64* total);
+
+ private:
+  kmst_context& C;
+  hipStream_t st;
+  const u32 *in_s, *in_d, *in_w;
+  u64 m0;
+  u32 n0;
+  u32* ctr = nullptr;
+  u64 filter_threshold = 4;
+  RecView X, Y;
+  u64* val0 = nullptr;
+  u64* best_val = nullptr;
+  u32 *best_dst = nullptr, *parent = nullptr, *flag = nullptr, *newid = nullptr, *label = nullptr;
+  u32 *rep = nullptr, *rep_next = nullptr, *gpar = nullptr, *mst = nullptr;
+  u32 *tiles = nullptr, *scan_scratch = nullptr, *hist = nullptr;
+  ArgPiece* pieces = nullptr;
+  u32* samples = nullptr;
+
+  void allocate();
+  void setup();
+  void check_errors(const char* where);
+  void read_counters();
+  // rounds of plain Boruvka on records sorted by (src, dst) with ids < n0;
+  // A is writable scratch disjoint from recs, B writable and may alias recs.
+  void rounds(RecView recs, RecView A, RecView B, bool track_gpar);
+  void filter_rec(RecView P, RecView Q, bool stop, int depth);
+  u32 pick_pivot(RecView r);
+  u64 partition(RecView in, RecView out, u32 pivot);
+  RecView sort_dedup(RecView cur, RecView alt, u32 bits);
+};
+
+void Solver::allocate() {
+  const u64 m = m0 ? m0 : 1;
+  const u64 n = u64(n0) + 1;
+  X = RecView{static_cast<u32*>(C.xs.get(m * 4)), static_cast<u32*>(C.xd.get(m * 4)),
+              static_cast<u64*>(C.xv.get(m * 8)), m};
+  Y = RecView{static_cast<u32*>(C.ys.get(m * 4)), static_cast<u32*>(C.yd.get(m * 4)),
+              static_cast<u64*>(C.yv.get(m * 8)), m};
+  val0 = static_cast<u64*>(C.val0.get(m * 8));
+  best_val = static_cast<u64*>(C.best_val.get(n * 8));
+  best_dst = static_cast<u32*>(C.best_dst.get(n * 4));
+  parent = static_cast<u32*>(C.parent.get(n * 4));
+  flag = static_cast<u32*>(C.flag.get(n * 4));
+  newid = static_cast<u32*>(C.newid.get(n * 4));
+  label = static_cast<u32*>(C.label.get(n * 4));
+  rep = static_cast<u32*>(C.rep.get(n * 4));
+  rep_next = static_cast<u32*>(C.rep_next.get(n * 4));
+  gpar = static_cast<u32*>(C.gpar.get(n * 4));
+  mst = static_cast<u32*>(C.mst.get(n * 4));
+  const u64 arg_tiles = (m + ARG_TILE - 1) / ARG_TILE;
+  pieces = static_cast<ArgPiece*>(C.pieces.get(2 * arg_tiles * sizeof(ArgPiece) + 64));
+  tiles = static_cast<u32*>(C.tiles.get(((m + CMP_TILE - 1) / CMP_TILE + 1) * 4));
+  const u64 hist_elems = sort_hist_elems(std::max(m, n));
+  hist = static_cast<u32*>(C.hist.get(hist_elems * 4));
+  const u64 scan_max = std::max<u64>({hist_elems, n, (m + CMP_TILE - 1) / CMP_TILE + 1});
+  scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 4 + 64));
+  ctr = static_cast<u32*>(C.counters.get(256));
+  samples = static_cast<u32*>(C.samples.get(4096 * 4));
+}
+
+void Solver::read_counters() {
+  KMST_HIP(hipMemcpyAsync(C.host_counters, ctr, 128, hipMemcpyDeviceToHost, st));
+  KMST_HIP(hipStreamSynchronize(st));
+}
+
+void Solver::check_errors(const char* where) {
+  read_counters();
+  const u32 e = C.host_counters[C_ERR];
+  if (e & ERR_SPIN) throw Error(KMST_EDEVICE, std::string(where) + ": bounded wait expired");
+  if (e & ERR_INVALID)
+    throw Error(KMST_EINVAL, std::string(where) + ": self-loop or vertex id out of range");
+  if (e & ERR_UNSORTED) throw Error(KMST_EUNSORTED, std::string(where) + ": records not sorted by (src, dst)");
+  if (e & ERR_ASYMMETRIC)
+    throw Error(KMST_EASYMMETRIC, std::string(where) + ": a record has no reverse twin");
+}
+
+void Solver::setup() {
+  fill_u32(st, ctr, 64, 0);
+  {
+    validate_records(st, in_s, in_d, m0, n0, ctr + C_ERR, ctr + C_FWD);
+  }
+  check_errors("input");
+  if (u64(C.host_counters[C_FWD]) * 2 != m0)
+    throw Error(KMST_EASYMMETRIC, "input: #(src<dst) records != #(src>dst) records");
+  {
+    build_offsets(st, in_s, m0, n0, label /* reused as offsets, n0+1 */);
+  }
+  {
+    build_keys(st, in_s, in_d, in_w, m0, label, val0, ctr + C_ERR);
+  }
+  check_errors("input");
+}
+
+RecView Solver::sort_dedup(RecView cur, RecView alt, u32 bits) {
+  if (cur.m > 1 && bits > 0) {
+    if (radix_sort(st, cur.s, cur.d, cur.v, alt.s, alt.d, alt.v, cur.m, bits, 2 * bits, hist,
+                   scan_scratch))
+      cur = RecView{alt.s, alt.d, alt.v, cur.m};
+  }
+  {
+    dedup_mark(st, cur.s, cur.d, cur.v, cur.m);
+  }
+  return cur;
+}
+
+void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar) {
+  RecView cur = recs;
+  u64 m = recs.m;
+  u64 n = n0;  // the first round runs in the original id space
+  bool first = true;
+  while (m > 0 && n > 1) {
+    RoundStat rs;
+    rs.n = n;
+    rs.m = m;
+    {
+      segmented_argmin(st, cur.s, cur.d, cur.v, m, n, best_val, best_dst, pieces, ctr + C_PIECES);
+    }
+    {
+      hook(st, n, best_val, best_dst, parent, flag, mst, ctr + C_MST);
+    }
+    {
+      pointer_jump(st, n, parent, ctr + C_ERR);
+    }
+    {
+      scan_exclusive_u32(st, flag, newid, n, scan_scratch, ctr + C_NNEXT);
+    }
+    {
+      make_labels(st, n, parent, flag, newid, label);
+    }
+    if (track_gpar) {
+      update_parent_array(st, n, first ? nullptr : rep, parent, flag, newid, best_val, gpar,
+                          rep_next);
+      std::swap(rep, rep_next);
+    }
+    RecView out = (cur.s == A.s) ? B : A;
+    {
+      relabel_compact(st, cur.s, cur.d, cur.v, m, label, out.s, out.d, out.v, tiles, scan_scratch,
+                      ctr + C_MNEXT);
+    }
+    check_errors("round");
+    const u64 n_next = C.host_counters[C_NNEXT];
+    const u64 m_next = C.host_counters[C_MNEXT];
+    rs.n_next = n_next;
+    rs.m_next = m_next;
+    C.stats.rounds.push_back(rs);
+    m = m_next;
+    n = n_next;
+    cur = RecView{out.s, out.d, out.v, m};
+    if (m == 0 || n <= 1) break;
+    RecView alt = (cur.s == A.s) ? B : A;
+    cur = sort_dedup(cur, alt, bits_for(n));
+    first = false;
+  }
+  if (track_gpar) {
+    pointer_jump(st, n0, gpar, ctr + C_ERR);
+  }
+}
+
+u32 Solver::pick_pivot(RecView r) {
+  const u32 k = static_cast<u32>(std::min<u64>(1023, r.m));
+  sample_weights(st, r.v, r.m, k, samples);
+  std::vector<u32> h(k);
+  KMST_HIP(hipMemcpyAsync(h.data(), samples, k * 4, hipMemcpyDeviceToHost, st));
+  KMST_HIP(hipStreamSynchronize(st));
+  h.erase(std::remove(h.begin(), h.end(), NONE32), h.end());
+  if (h.empty()) return NONE32;
+  std::sort(h.begin(), h.end());
+  return h[h.size() / 2];  // median sampled weight (select_pivot, distributed_partitioning.hpp:92-120)
+}
+
+u64 Solver::partition(RecView in, RecView out, u32 pivot) {
+  split_light_heavy(st, in.s, in.d, in.v, in.m, pivot, out.s, out.d, out.v, tiles, scan_scratch,
+                    ctr + C_LIGHT);
+  read_counters();
+  return C.host_counters[C_LIGHT];
+}
+
+// boruvka_recursive (include/algorithms/hybrid_boruvka_edgefilter.hpp:227-330)
+// on records P (sorted by (src, dst), ids are parent-array representatives);
+// Q is scratch of the same extent.
+void Solver::filter_rec(RecView P, RecView Q, bool stop, int depth) {
+  if (P.m == 0) return;
+  const double avg_degree = double(P.m) / double(n0);
+  if (avg_degree <= double(filter_threshold) || stop || P.m <= 1000 || depth > 40) {
+    FilterStat fs{depth, P.m, 0, 0, true};
+    C.stats.filter.push_back(fs);
+    rounds(P, Q, P, /*track_gpar=*/true);
+    return;
+  }
+  const u32 pivot = pick_pivot(P);
+  const u64 L = partition(P, Q, pivot);
+  const bool stop_rec = double(L) / double(P.m) > 0.85;  // filter_helpers.hpp:130-132
+  FilterStat fs{depth, P.m, L, pivot, false};
+  C.stats.filter.push_back(fs);
+  filter_rec(Q.at(0, L), P.at(0, L), stop_rec, depth + 1);
+  // heavy half: relabel by the parent array, drop same-component edges
+  // (boruvka_recursive_setup, hybrid_boruvka_edgefilter.hpp:173-225)
+  const u64 H = P.m - L;
+  if (H == 0) return;
+  RecView heavy = Q.at(L, H), scratch = P.at(L, H);
+  {
+    relabel_compact(st, heavy.s, heavy.d, heavy.v, H, gpar, scratch.s, scratch.d, scratch.v, tiles,
+                    scan_scratch, ctr + C_MNEXT);
+  }
+  check_errors("filter");
+  const u64 Hk = C.host_counters[C_MNEXT];
+  C.stats.filter.back().heavy_kept = Hk;
+  if (Hk == 0) return;
+  RecView kept = scratch.at(0, Hk);
+  RecView sorted = sort_dedup(kept, heavy.at(0, Hk), bits_for(n0));
+  RecView other = (sorted.s == kept.s) ? heavy.at(0, Hk) : kept;
+  filter_rec(sorted, other, false, depth + 1);
+}
+
+void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64 out_cap,
+                 u64* out_n, u64* total) {
+  C.stats = SolveStats{};
+  C.stats.n = n0;
+  C.stats.m = m0;
+  C.stats.algorithm = cfg.algorithm;
+  filter_threshold = cfg.filter_threshold;
+  allocate();
+  setup();
+  const RecView input{const_cast<u32*>(in_s), const_cast<u32*>(in_d), val0, m0};
+  if (cfg.algorithm == KMST_ALGO_FILTER_BORUVKA) {
+    {
+      iota_u32(st, gpar, n0);
+    }
+    // the top level reads the (read-only) input and splits into Y, so the
+    // first level cannot run its base case in place; filter_rec handles the
+    // base case itself when the split is not taken.
+    const double avg_degree = double(m0) / double(std::max<u32>(n0, 1));
+    if (avg_degree <= double(filter_threshold) || m0 <= 1000) {
+      C.stats.filter.push_back(FilterStat{0, m0, 0, 0, true});
+      rounds(input, X, Y, true);
+    } else {
+      const u32 pivot = pick_pivot(input);
+      const u64 L = partition(input, Y, pivot);
+      const bool stop_rec = double(L) / double(m0) > 0.85;
+      C.stats.filter.push_back(FilterStat{0, m0, L, pivot, false});
+      filter_rec(Y.at(0, L), X.at(0, L), stop_rec, 1);
+      const u64 H = m0 - L;
+      if (H > 0) {
+        RecView heavy = Y.at(L, H), scratch = X.at(L, H);
+        {
+          relabel_compact(st, heavy.s, heavy.d, heavy.v, H, gpar, scratch.s, scratch.d, scratch.v,
+                          tiles, scan_scratch, ctr + C_MNEXT);
+        }
+        check_errors("filter");
+        const u64 Hk = C.host_counters[C_MNEXT];
+        C.stats.filter.back().heavy_kept = Hk;
+        if (Hk > 0) {
+          RecView kept = scratch.at(0, Hk);
+          RecView sorted = sort_dedup(kept, heavy.at(0, Hk), bits_for(n0));
+          RecView other = (sorted.s == kept.s) ? heavy.at(0, Hk) : kept;
+          filter_rec(sorted, other, false, 1);
+        }
+      }
+    }
+  } else {
+    rounds(input, X, Y, false);
+  }
+  // finish: sort MST uids (uid order == canonical (min, max) order), gather
+  read_counters();
+  const u64 cnt = C.host_counters[C_MST];
+  if (cnt > out_cap) throw Error(KMST_EINVAL, "output capacity too small");
+  if (cnt > 1) {
+    if (radix_sort(st, mst, nullptr, nullptr, rep_next, nullptr, nullptr, cnt, 0, 32, hist,
+                   scan_scratch))
+      KMST_HIP(hipMemcpyAsync(mst, rep_next, cnt * 4, hipMemcpyDeviceToDevice, st));
+  }
+  u64* total_dev = reinterpret_cast<u64*>(ctr + 16);
+  fill_u64(st, total_dev, 1, 0);
+  {
+    gather_mst(st, mst, cnt, in_s, in_d, in_w, out_s, out_d, out_w, total_dev);
+  }
+  read_counters();
+  *out_n = cnt;
+  std::memcpy(total, C.host_counters + 16, 8);
+  C.stats.mst_edges = cnt;
+  C.stats.total_weight = *total;
+}
+
+}  // namespace kmst
+
+// =================================================================== C ABI
+namespace {
+
+std::mutex g_default_mu;
+std::map<int, std::unique_ptr<kmst_context>> g_default;
+
+kmst_status fail(const kmst::Error& e) {
+  kmst::g_last_error = e.what();
+  return e.status;
+}
+
+template <typename F>
+kmst_status guarded(F&& f) {
+  try {
+    f();
+    kmst::g_last_error.clear();
+    return KMST_OK;
+  } catch (const kmst::Error& e) {
+    return fail(e);
+  } catch (const std::bad_alloc&) {
+    kmst::g_last_error = "host allocation failed";
+    return KMST_ENOMEM;
+  } catch (const std::exception& e) {
+    kmst::g_last_error = e.what();
+    return KMST_EDEVICE;
+  }
+}
+
+kmst_context* default_context(int device) {
+  auto& slot = g_default[device];
+  if (!slot) slot.reset(new kmst_context(device));
+  return slot.get();
+}
+
+void check_config(const kmst_config* cfg) {
+  if (!cfg) throw kmst::Error(KMST_EINVAL, "null config");
+  if (cfg->algorithm != KMST_ALGO_BORUVKA && cfg->algorithm != KMST_ALGO_FILTER_BORUVKA)
+    throw kmst::Error(KMST_EINVAL, "unknown algorithm");
+  if (cfg->compression != 0)
+    throw kmst::Error(KMST_EINVAL, "only NO_COMPRESSION is supported");
+}
+
+void solve_device_impl(kmst_context* ctx, const kmst_dev_edges* in, const kmst_config* cfg,
+                       kmst_dev_mst* out, uint64_t* out_n, uint64_t* total_weight) {
+  check_config(cfg);
+  if (!ctx || !in || !out || !out_n || !total_weight)
+    throw kmst::Error(KMST_EINVAL, "null argument");
+  if (in->m_directed >= (uint64_t{1} << 32) - 1)
+    throw kmst::Error(KMST_EINVAL, "more than 2^32-2 records");
+  if (in->n >= (uint64_t{1} << 32) - 1) throw kmst::Error(KMST_EINVAL, "n >= 2^32-1");
+  if (in->m_directed && (!in->src || !in->dst || !in->weight))
+    throw kmst::Error(KMST_EINVAL, "null input array");
+  KMST_HIP(hipSetDevice(ctx->device));
+  kmst::KernelTimer* prev = kmst::g_ktimer;
+  kmst::g_ktimer = ctx->timing ? &ctx->timer : nullptr;
+  try {
+    if (in->m_directed == 0 || in->n <= 1) {
+      if (in->m_directed != 0) throw kmst::Error(KMST_EINVAL, "records with n <= 1");
+      *out_n = 0;
+      *total_weight = 0;
+    } else {
+      kmst::Solver s(*ctx, in->src, in->dst, in->weight, in->m_directed, in->n);
+      s.run(*cfg, out->src, out->dst, out->weight, out->capacity, out_n, total_weight);
+    }
+    if (ctx->timing) ctx->timer.collect();
+  } catch (...) {
+    kmst::g_ktimer = prev;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->timing) ctx->timer.collect();
+    throw;
+  }
+  kmst::g_ktimer = prev;
+}
+
+kmst_status host_solve(const kmst_wedge* in, uint64_t m, const kmst_config* cfg,
+                       int32_t algorithm, kmst_wedge* out, uint64_t* out_n,
+                       uint64_t* total_weight) {
+  return guarded([&] {
+    check_config(cfg);
+    if ((!in && m) || !out_n || !total_weight) throw kmst::Error(KMST_EINVAL, "null argument");
+    if (m >= (uint64_t{1} << 32) - 1) throw kmst::Error(KMST_EINVAL, "more than 2^32-2 records");
+    kmst_config c = *cfg;
+    c.algorithm = algorithm;
+    std::vector<uint32_t> s(m), d(m), w(m);
+    uint64_t vmax = 0;
+    for (uint64_t i = 0; i < m; ++i) {
+      if (in[i].src >= 0xFFFFFFFFull || in[i].dst >= 0xFFFFFFFFull)
+        throw kmst::Error(KMST_EINVAL, "vertex id >= 2^32-1");
+      s[i] = static_cast<uint32_t>(in[i].src);
+      d[i] = static_cast<uint32_t>(in[i].dst);
+      w[i] = in[i].weight;
+      vmax = std::max<uint64_t>(vmax, std::max(in[i].src, in[i].dst));
+    }
+    const uint64_t n = m ? vmax + 1 : 0;
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    kmst_context* ctx = default_context(c.device);
+    KMST_HIP(hipSetDevice(ctx->device));
+    const uint64_t mm = m ? m : 1;
+    uint32_t* ds = static_cast<uint32_t*>(ctx->in_s.get(mm * 4));
+    uint32_t* dd = static_cast<uint32_t*>(ctx->in_d.get(mm * 4));
+    uint32_t* dw = static_cast<uint32_t*>(ctx->in_w.get(mm * 4));
+    const uint64_t cap = n ? n : 1;
+    kmst_dev_mst o{static_cast<uint32_t*>(ctx->out_s.get(cap * 4)),
+                   static_cast<uint32_t*>(ctx->out_d.get(cap * 4)),
+                   static_cast<uint32_t*>(ctx->out_w.get(cap * 4)), cap};
+    if (m) {
+      KMST_HIP(hipMemcpyAsync(ds, s.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
+      KMST_HIP(hipMemcpyAsync(dd, d.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
+      KMST_HIP(hipMemcpyAsync(dw, w.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    kmst_dev_edges e{ds, dd, dw, m, n};
+    solve_device_impl(ctx, &e, &c, &o, out_n, total_weight);
+    const uint64_t k = *out_n;
+    std::vector<uint32_t> hs(k), hd(k), hw(k);
+    if (k) {
+      KMST_HIP(hipMemcpyAsync(hs.data(), o.src, k * 4, hipMemcpyDeviceToHost, ctx->stream));
+      KMST_HIP(hipMemcpyAsync(hd.data(), o.dst, k * 4, hipMemcpyDeviceToHost, ctx->stream));
+      KMST_HIP(hipMemcpyAsync(hw.data(), o.weight, k * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    KMST_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t i = 0; i < k; ++i) out[i] = kmst_wedge{hs[i], hd[i], hw[i], 0};
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+void kmst_config_default(kmst_config* cfg) {
+  if (!cfg) return;
+  cfg->algorithm = KMST_ALGO_BORUVKA;
+  cfg->local_preprocessing = 0;
+  cfg->compression = 0;
+  cfg->device = 0;
+  cfg->filter_threshold = 4;
+}
+
+kmst_status kmst_context_create(int device, kmst_context** out) {
+  return guarded([&] {
+    if (!out) throw kmst::Error(KMST_EINVAL, "null out");
+    *out = new kmst_context(device);
+  });
+}
+
+void kmst_context_destroy(kmst_context* ctx) { delete ctx; }
+
+void kmst_context_trim(kmst_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (kmst::DevBuf* b : ctx->all_bufs())
+    if (b != &ctx->counters) b->release();
+}
+
+kmst_status kmst_boruvka(const kmst_wedge* in, uint64_t m, const kmst_config* cfg, kmst_wedge* out,
+                         uint64_t* out_n, uint64_t* total_weight) {
+  return host_solve(in, m, cfg, KMST_ALGO_BORUVKA, out, out_n, total_weight);
+}
+
+kmst_status kmst_filter_boruvka(const kmst_wedge* in, uint64_t m, const kmst_config* cfg,
+                                kmst_wedge* out, uint64_t* out_n, uint64_t* total_weight) {
+  return host_solve(in, m, cfg, KMST_ALGO_FILTER_BORUVKA, out, out_n, total_weight);
+}
+
+kmst_status kmst_solve_device(kmst_context* ctx, const kmst_dev_edges* in, const kmst_config* cfg,
+                              kmst_dev_mst* out, uint64_t* out_n, uint64_t* total_weight) {
+  return guarded([&] { solve_device_impl(ctx, in, cfg, out, out_n, total_weight); });
+}
+
+const char* kmst_last_error(void) { return kmst::g_last_error.c_str(); }
+
+void kmst_context_set_timing(kmst_context* ctx, int on) {
+  if (!ctx) return;
+  ctx->timing = on != 0;
+  ctx->timer.reset();
+}
+
+kmst_status kmst_gen_gnm_device(kmst_context* ctx, uint64_t n, uint64_t m, uint32_t max_weight,
+                                uint64_t seed, uint32_t* src, uint32_t* dst, uint32_t* weight) {
+  return guarded([&] {
+    if (!ctx || (m && (!src || !dst || !weight))) throw kmst::Error(KMST_EINVAL, "null argument");
+    KMST_HIP(hipSetDevice(ctx->device));
+    kmst::gen_gnm_device(*ctx, n, m, max_weight, seed, src, dst, weight);
+  });
+}
+
+kmst_status kmst_gen_gnm_host(uint64_t n, uint64_t m, uint32_t max_weight, uint64_t seed,
+                              uint32_t* src, uint32_t* dst, uint32_t* weight) {
+  return guarded([&] {
+    if (m && (!src || !dst || !weight)) throw kmst::Error(KMST_EINVAL, "null argument");
+    kmst::gen_gnm_host(n, m, max_weight, seed, src, dst, weight);
+  });
+}
+
+kmst_status kmst_k_segmented_argmin(kmst_context* ctx, const uint32_t* src, const uint32_t* dst,
+                                    const uint64_t* val, uint64_t m, uint64_t n,
+                                    uint64_t* best_val, uint32_t* best_dst) {
+  return guarded([&] {
+    if (!ctx) throw kmst::Error(KMST_EINVAL, "null context");
+    KMST_HIP(hipSetDevice(ctx->device));
+    const uint64_t tiles = (m + kmst::ARG_TILE - 1) / kmst::ARG_TILE;
+    auto* pieces = static_cast<kmst::ArgPiece*>(
+        ctx->pieces.get(2 * tiles * sizeof(kmst::ArgPiece) + 64));
+    auto* ctr = static_cast<uint32_t*>(ctx->counters.get(256));
+    kmst::segmented_argmin(ctx->stream, src, dst, val, m, n, best_val, best_dst, pieces,
+                           ctr + kmst::C_PIECES);
+    KMST_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+kmst_status kmst_k_sort_pairs(kmst_context* ctx, uint32_t* src, uint32_t* dst, uint64_t* val,
+                              uint64_t m, uint32_t bits) {
+  return guarded([&] {
+    if (!ctx) throw kmst::Error(KMST_EINVAL, "null context");
+    if (bits > 32) throw kmst::Error(KMST_EINVAL, "bits > 32");
+    KMST_HIP(hipSetDevice(ctx->device));
+    const uint64_t mm = m ? m : 1;
+    auto* as = static_cast<uint32_t*>(ctx->xs.get(mm * 4));
+    auto* ad = static_cast<uint32_t*>(ctx->xd.get(mm * 4));
+    auto* av = static_cast<uint64_t*>(ctx->xv.get(mm * 8));
+    const uint64_t he = kmst::sort_hist_elems(mm);
+    auto* hist = static_cast<uint32_t*>(ctx->hist.get(he * 4));
+    auto* sc = static_cast<uint32_t*>(ctx->scan_scratch.get(kmst::scan_scratch_elems(he) * 4 + 64));
+    if (kmst::radix_sort(ctx->stream, src, dst, val, as, ad, av, m, bits, 2 * bits, hist, sc)) {
+      KMST_HIP(hipMemcpyAsync(src, as, m * 4, hipMemcpyDeviceToDevice, ctx->stream));
+      KMST_HIP(hipMemcpyAsync(dst, ad, m * 4, hipMemcpyDeviceToDevice, ctx->stream));
+      KMST_HIP(hipMemcpyAsync(val, av, m * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    KMST_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+kmst_status kmst_k_pointer_jump(kmst_context* ctx, uint32_t* parent, uint64_t n) {
+  return guarded([&] {
+    if (!ctx) throw kmst::Error(KMST_EINVAL, "null context");
+    KMST_HIP(hipSetDevice(ctx->device));
+    auto* ctr = static_cast<uint32_t*>(ctx->counters.get(256));
+    kmst::fill_u32(ctx->stream, ctr, 64, 0);
+    kmst::pointer_jump(ctx->stream, n, parent, ctr + kmst::C_ERR);
+    uint32_t err = 0;
+    KMST_HIP(hipMemcpyAsync(&err, ctr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    KMST_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) throw kmst::Error(KMST_EDEVICE, "pointer jumping did not converge");
+  });
+}
+
+size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
+  if (!ctx) return 0;
+  std::ostringstream o;
+  const kmst::SolveStats& s = ctx->stats;
+  o << "{\"n\":" << s.n << ",\"m\":" << s.m << ",\"algorithm\":" << s.algorithm
+    << ",\"mst_edges\":" << s.mst_edges << ",\"total_weight\":" << s.total_weight
+    << ",\"rounds\":[";
+  for (size_t i = 0; i < s.rounds.size(); ++i) {
+    const auto& r = s.rounds[i];
+    o << (i ? "," : "") << "{\"n\":" << r.n << ",\"m\":" << r.m << ",\"n_next\":" << r.n_next
+      << ",\"m_next\":" << r.m_next << "}";
+  }
+  o << "],\"filter\":[";
+  for (size_t i = 0; i < s.filter.size(); ++i) {
+    const auto& f = s.filter[i];
+    o << (i ? "," : "") << "{\"depth\":" << f.depth << ",\"m\":" << f.m << ",\"light\":" << f.light
+      << ",\"pivot\":" << f.pivot << ",\"base\":" << (f.base ? "true" : "false")
+      << ",\"heavy_kept\":" << f.heavy_kept << "}";
+  }
+  o << "],\"kernels\":{";
+  bool firstk = true;
+  for (const auto& kv : ctx->timer.totals) {
+    o << (firstk ? "" : ",") << "\"" << kmst::kernel_kind_name(kv.first)
+      << "\":{\"launches\":" << kv.second.launches << ",\"ms\":" << kv.second.ms
+      << ",\"bytes\":" << kv.second.bytes << "}";
+    firstk = false;
+  }
+  o << "}}";
+  const std::string str = o.str();
+  if (buf && cap > str.size()) {
+    std::memcpy(buf, str.c_str(), str.size() + 1);
+    return str.size();
+  }
+  return str.size() + 1;
+}
+
+}  // extern "C"

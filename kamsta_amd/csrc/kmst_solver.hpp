@@ -1,0 +1,67 @@
+// kmst_solver.hpp -- the context behind the C ABI handle: stream, cached
+// device workspaces, per-solve statistics and the kernel timer.
+#pragma once
+
+#include <vector>
+
+#include "kmst_common.hpp"
+
+namespace kmst {
+
+// A device allocation that only grows; reused across solves so steady-state
+// calls do not allocate.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need);
+  void release();
+};
+
+struct RoundStat {
+  u64 n = 0, m = 0, n_next = 0, m_next = 0;
+};
+struct FilterStat {
+  int depth = 0;
+  u64 m = 0;
+  u64 light = 0;
+  u32 pivot = 0;
+  bool base = false;
+  u64 heavy_kept = 0;
+};
+struct SolveStats {
+  u64 n = 0, m = 0, mst_edges = 0, total_weight = 0;
+  int algorithm = 0;
+  std::vector<RoundStat> rounds;
+  std::vector<FilterStat> filter;
+};
+
+extern thread_local std::string g_last_error;
+
+}  // namespace kmst
+
+struct kmst_context {
+  explicit kmst_context(int device);
+  ~kmst_context();
+  kmst_context(const kmst_context&) = delete;
+  kmst_context& operator=(const kmst_context&) = delete;
+
+  int device = 0;
+  hipStream_t stream = nullptr;
+  kmst::u32* host_counters = nullptr;  // pinned, 256 bytes
+  bool timing = false;
+  kmst::KernelTimer timer;
+  kmst::SolveStats stats;
+
+  // record buffers (X, Y ping-pong), strict keys of the input
+  kmst::DevBuf xs, xd, xv, ys, yd, yv, val0;
+  // vertex arrays
+  kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, mst;
+  // kernel scratch
+  kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples;
+  // host-API staging
+  kmst::DevBuf in_s, in_d, in_w, out_s, out_d, out_w;
+  // generator scratch
+  kmst::DevBuf gen_a, gen_b, gen_c, gen_d;
+
+  std::vector<kmst::DevBuf*> all_bufs();
+};

@@ -1,0 +1,785 @@
+// kmst_oracle.cpp -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// A CPU restatement of KaMSTa's Boruvka / filter-Boruvka MST path on one
+// process (MPI collectives reduce to identities), used only as the checker by
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The
+// product (kamsta_amd/libkmst.so) never links, loads or calls this file.
+//
+// Every comparator of the reference is refined to the strict key
+// (weight, uid), where uid is the input position of the (min, max) record of
+// an undirected edge.  Because the input is sorted by (src, dst), uid order
+// equals (min, max) order, so the key order is exactly the reference check's
+// Kruskal order WeightSrcDstOrder restricted to the smaller copy of each edge
+// (include/util/utils.hpp:81-86, include/algorithms/base_case_mst_algos.hpp:92-113)
+// and exactly the local-contraction tie-break (weight, local edge index)
+// (include/algorithms/local_contraction/local_boruvka/mst_utils_common.hpp:76-81).
+// Under a strict key the minimum spanning forest is unique, so every function
+// below must return the same canonical edge set.
+//
+// Pinning (what checks each function; see tests/test_oracle.py):
+//   orc_kruskal            brute-force enumeration of spanning forests (n <= 7),
+//                          closed forms (path, star, cycle, equal weights),
+//                          committed golden fixtures.
+//   orc_boruvka (level 1)  equality with orc_kruskal on random/hand graphs.
+//   orc_boruvka (level 0)  equality with orc_kruskal on random/hand graphs.
+//   orc_filter_boruvka     equality with orc_kruskal on random/hand graphs.
+//   orc_pair_records       symmetric-pairing invariants (bijection, equal
+//                          weights) and planted asymmetric/unsorted inputs.
+// The reference itself stores no golden vectors; its only runtime check
+// compares the total weight against Kruskal (benchmarks/mst_benchmarks.hpp:287-318).
+
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include <omp.h>
+
+namespace {
+
+using u32 = std::uint32_t;
+using u64 = std::uint64_t;
+using i64 = std::int64_t;
+
+// Same 24-byte layout as hybridMST::WEdge (include/definitions.hpp:66-84).
+struct WEdge {
+  u64 src;
+  u64 dst;
+  u32 weight;
+  u32 pad;
+};
+static_assert(sizeof(WEdge) == 24, "WEdge layout");
+
+enum Status : int { OK = 0, EINVAL_ = 1, EUNSORTED = 2, EASYMMETRIC = 3 };
+
+constexpr u64 KEY_NONE = ~u64{0};
+constexpr u32 U32_NONE = ~u32{0};
+
+inline u64 make_key(u32 w, u32 uid) { return (u64(w) << 32) | uid; }
+
+// Lock-free write-min, the role of hybridMST::write_min_gbbs
+// (include/util/atomic_ops.hpp:11-38), on the packed strict key.
+inline void write_min(std::atomic<u64>& slot, u64 key) {
+  u64 cur = slot.load(std::memory_order_relaxed);
+  while (key < cur &&
+         !slot.compare_exchange_weak(cur, key, std::memory_order_relaxed)) {
+  }
+}
+
+// Order-preserving pack (parlay::pack_out / filter) parallelised with a
+// blocked prefix sum.
+template <typename T, typename Pred>
+std::size_t pack(const T* in, std::size_t n, T* out, Pred&& keep) {
+  const int nt = omp_get_max_threads();
+  std::vector<std::size_t> counts(nt + 1, 0);
+#pragma omp parallel num_threads(nt)
+  {
+    const int t = omp_get_thread_num();
+    const std::size_t b = n * t / nt, e = n * (t + 1) / nt;
+    std::size_t c = 0;
+    for (std::size_t i = b; i < e; ++i) c += keep(i) ? 1 : 0;
+    counts[t + 1] = c;
+#pragma omp barrier
+#pragma omp single
+    for (int k = 0; k < nt; ++k) counts[k + 1] += counts[k];
+    std::size_t o = counts[t];
+    for (std::size_t i = b; i < e; ++i)
+      if (keep(i)) out[o++] = in[i];
+  }
+  return counts[nt];
+}
+
+struct UnionFind {  // include/datastructures/union_find.hpp:9-32
+  std::vector<u32> parent;
+  std::vector<unsigned char> rank;
+  explicit UnionFind(std::size_t n) : parent(n), rank(n, 0) {
+    std::iota(parent.begin(), parent.end(), 0u);
+  }
+  u32 find(u32 v) {
+    while (parent[v] != v) {
+      parent[v] = parent[parent[v]];
+      v = parent[v];
+    }
+    return v;
+  }
+  bool unify(u32 a, u32 b) {
+    a = find(a);
+    b = find(b);
+    if (a == b) return false;
+    if (rank[a] < rank[b]) std::swap(a, b);
+    parent[b] = a;
+    if (rank[a] == rank[b]) ++rank[a];
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Input contract and the edge-id assignment.
+//
+// The reference assumes (does not check) that the input is sorted by
+// (src, dst), symmetric and free of self-loops (include/util/utils.hpp:391-395,
+// include/algorithms/hybrid_boruvka_substeps/get_ghost_representatives.hpp:74-77)
+// and names each record by its position (UncompressedGraph::get_WEdgeIds,
+// include/datastructures/compression/uncompressed_graph.hpp:38-50).  Here each
+// record gets uid = position of its (min, max) twin; the k-th (v, u, w) record
+// of a run of equal (v, u) pairs with the k-th (u, v, w) record of the twin
+// run that has the same weight, so multigraphs pair as a bijection.
+int pair_records(const WEdge* in, u64 N, std::vector<u32>& uid, u64& n_out) {
+  if (N >= (u64{1} << 32)) return EINVAL_;
+  uid.assign(N, 0);
+  u64 vmax = 0;
+  int status = OK;
+  u64 n_fwd = 0, n_rev = 0;
+#pragma omp parallel for reduction(max : vmax) reduction(+ : n_fwd, n_rev)
+  for (i64 i = 0; i < i64(N); ++i) {
+    const WEdge& e = in[i];
+    if (e.src >= U32_NONE || e.dst >= U32_NONE) {
+#pragma omp atomic write
+      status = EINVAL_;
+      continue;
+    }
+    if (e.src == e.dst) {
+#pragma omp atomic write
+      status = EINVAL_;
+      continue;
+    }
+    if (i > 0) {
+      const WEdge& p = in[i - 1];
+      if (p.src > e.src || (p.src == e.src && p.dst > e.dst)) {
+#pragma omp atomic write
+        status = EUNSORTED;
+        continue;
+      }
+    }
+    vmax = std::max<u64>(vmax, std::max(e.src, e.dst));
+    if (e.src < e.dst) ++n_fwd; else ++n_rev;
+  }
+  if (status != OK) return status;
+  if (n_fwd != n_rev) return EASYMMETRIC;
+  auto lower = [&](u64 s, u64 d) {
+    u64 lo = 0, hi = N;
+    while (lo < hi) {
+      const u64 mid = (lo + hi) / 2;
+      if (in[mid].src < s || (in[mid].src == s && in[mid].dst < d)) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+#pragma omp parallel for
+  for (i64 i = 0; i < i64(N); ++i) {
+    const WEdge& e = in[i];
+    if (e.src < e.dst) {
+      uid[i] = u32(i);
+      continue;
+    }
+    // rank k among equal-weight records of the own run before i
+    u64 k = 0;
+    for (i64 j = i - 1; j >= 0 && in[j].src == e.src && in[j].dst == e.dst; --j)
+      k += (in[j].weight == e.weight);
+    u64 j = lower(e.dst, e.src);
+    u64 found = KEY_NONE;
+    for (; j < N && in[j].src == e.dst && in[j].dst == e.src; ++j) {
+      if (in[j].weight == e.weight) {
+        if (k == 0) { found = j; break; }
+        --k;
+      }
+    }
+    if (found == KEY_NONE) {
+#pragma omp atomic write
+      status = EASYMMETRIC;
+    } else {
+      uid[i] = u32(found);
+    }
+  }
+  n_out = N ? vmax + 1 : 0;
+  return status;
+}
+
+// Canonical output: (min, max, w) sorted by (min, max), from MST uids.
+void emit(const WEdge* in, std::vector<u32>& mst_uids, WEdge* out, u64* out_n,
+          u64* total) {
+  std::sort(mst_uids.begin(), mst_uids.end());
+  u64 t = 0;
+  for (std::size_t i = 0; i < mst_uids.size(); ++i) {
+    const WEdge& e = in[mst_uids[i]];
+    if (out) out[i] = WEdge{std::min(e.src, e.dst), std::max(e.src, e.dst), e.weight, 0};
+    t += e.weight;
+  }
+  *out_n = mst_uids.size();
+  *total = t;
+}
+
+// ---------------------------------------------------------------------------
+// Path (a): local contraction "MODIFIED_BORUVKA" on the u<v records, i.e.
+// boruvka_filter / boruvka_filter_rec / boruvka3
+// (include/algorithms/local_contraction/local_boruvka/local_mst_respecting_cut_edges.hpp:101-302).
+// One GPU has no PE boundary, so there are no cut edges: every
+// min_cut_weight is WEIGHT_MAX and every cut-edge guard of
+// determine_mst_edges_exhausted (mst_utils_cut_edges.hpp:106-168) passes.
+
+struct LEdge {  // working record of the local contraction
+  u32 src, dst, w, uid;
+};
+
+struct LocalState {
+  u32 n_initial = 0;
+  std::vector<u32> parents;           // indexed by original vertex id
+  std::vector<std::atomic<u64>> min_edges;
+  std::vector<u32> vertices, vertices_tmp;  // SwapStorage<VId> vertices_
+  std::vector<u32> mst;               // uids
+};
+
+struct BoruvkaResult {  // mst_utils_common.hpp:11-14
+  std::size_t num_mst_edges_found;
+  std::size_t num_active_vertices;
+};
+
+// boruvka3: local_mst_respecting_cut_edges.hpp:101-207
+BoruvkaResult boruvka3(LocalState& st, LEdge* edges, std::size_t m_in,
+                       std::size_t n_active) {
+  std::size_t m = m_in, n = n_active;
+  std::vector<u32> edge_ids(m), edge_ids_tmp(m);
+#pragma omp parallel for
+  for (i64 i = 0; i < i64(m); ++i) edge_ids[i] = u32(i);
+  std::vector<u32> exhausted;
+  std::vector<u32> new_mst(n);
+  std::vector<unsigned char> is_active(n), can_be_reactivated(n);
+  std::size_t nb_mst = 0;
+  constexpr u32 MARK = u32{1} << 31;  // MSD<LocalEdgeId>
+  while (n > 0 && m > 0) {
+    // compute_min_edges (mst_utils_common.hpp:87-105): both endpoints.
+#pragma omp parallel for
+    for (i64 i = 0; i < i64(n); ++i)
+      st.min_edges[st.vertices[i]].store(KEY_NONE, std::memory_order_relaxed);
+#pragma omp parallel for
+    for (i64 i = 0; i < i64(m); ++i) {
+      const LEdge& e = edges[edge_ids[i]];
+      const u64 key = make_key(e.w, e.uid);
+      write_min(st.min_edges[e.src], key);
+      write_min(st.min_edges[e.dst], key);
+    }
+    // The reference keeps the local edge index in min_edges and reads the
+    // partner endpoint from the edge; here the partner is written by the one
+    // live edge whose key won (keys are unique per live edge).
+    std::vector<u32>& partner = st.vertices_tmp;  // scratch indexed by vertex
+#pragma omp parallel for
+    for (i64 i = 0; i < i64(m); ++i) {
+      const LEdge& e = edges[edge_ids[i]];
+      const u64 key = make_key(e.w, e.uid);
+      if (st.min_edges[e.src].load(std::memory_order_relaxed) == key) partner[e.src] = e.dst;
+      if (st.min_edges[e.dst].load(std::memory_order_relaxed) == key) partner[e.dst] = e.src;
+    }
+    // determine_mst_edges_exhausted (mst_utils_cut_edges.hpp:106-168) without
+    // cut edges: the lower endpoint of a mutual pair is the root.
+#pragma omp parallel for
+    for (i64 i = 0; i < i64(n); ++i) {
+      const u32 v = st.vertices[i];
+      const u64 key = st.min_edges[v].load(std::memory_order_relaxed);
+      if (key == KEY_NONE) {  // no more local edges incident to v
+        is_active[i] = 0;
+        can_be_reactivated[i] = 1;
+        new_mst[i] = U32_NONE;
+        continue;
+      }
+      can_be_reactivated[i] = 0;
+      const u32 u = partner[v];
+      if (u > v && st.min_edges[u].load(std::memory_order_relaxed) == key) {
+        st.parents[v] = v;
+        is_active[i] = 1;
+        new_mst[i] = U32_NONE;
+      } else {
+        st.parents[v] = u;
+        is_active[i] = 0;
+        new_mst[i] = u32(key);  // uid of the min edge
+      }
+    }
+    // add_new_mst_edges_to_mst_array (gbbs_reimplementation.hpp:138-159)
+    {
+      std::vector<u32> added(n);
+      const std::size_t c = pack(new_mst.data(), n, added.data(),
+                                 [&](std::size_t i) { return new_mst[i] != U32_NONE; });
+      st.mst.insert(st.mst.end(), added.begin(), added.begin() + c);
+      nb_mst += c;
+    }
+    // update_parents_min_cut_weights (mst_utils_cut_edges.hpp:170-197):
+    // pointer jumping to the fixpoint.
+#pragma omp parallel for
+    for (i64 i = 0; i < i64(n); ++i) {
+      const u32 v = st.vertices[i];
+      while (st.parents[v] != st.parents[st.parents[v]])
+        st.parents[v] = st.parents[st.parents[v]];
+    }
+    // identify_exhausted_vertices (mst_utils_common.hpp:106-120)
+    {
+      std::vector<u32> ex(n);
+      const std::size_t c = pack(st.vertices.data(), n, ex.data(),
+                                 [&](std::size_t i) { return can_be_reactivated[i] != 0; });
+      exhausted.insert(exhausted.end(), ex.begin(), ex.begin() + c);
+    }
+    // compactify_vertices (mst_utils_common.hpp:122-132)
+    {
+      std::vector<u32> nv(n);
+      n = pack(st.vertices.data(), n, nv.data(),
+               [&](std::size_t i) { return is_active[i] != 0; });
+      std::copy(nv.begin(), nv.begin() + n, st.vertices.begin());
+    }
+    // relabel_edges (mst_utils_cut_edges.hpp:12-39), no cut-edge guard
+#pragma omp parallel for
+    for (i64 i = 0; i < i64(m); ++i) {
+      LEdge& e = edges[edge_ids[i]];
+      const u32 pu = st.parents[e.src], pv = st.parents[e.dst];
+      e.src = pu;
+      e.dst = pv;
+      if (pu == pv) edge_ids[i] |= MARK;
+    }
+    // filter_out_self_loops (mst_utils_common.hpp:157-175)
+    m = pack(edge_ids.data(), m, edge_ids_tmp.data(),
+             [&](std::size_t i) { return (edge_ids[i] & MARK) == 0; });
+    std::swap(edge_ids, edge_ids_tmp);
+  }
+  // re-append exhausted vertices (local_mst_respecting_cut_edges.hpp:195-198)
+  for (std::size_t i = 0; i < exhausted.size(); ++i) st.vertices[n + i] = exhausted[i];
+  n += exhausted.size();
+  // final shortcut of every parent (199-203)
+#pragma omp parallel for
+  for (i64 v = 0; v < i64(st.n_initial); ++v) {
+    while (st.parents[v] != st.parents[st.parents[v]])
+      st.parents[v] = st.parents[st.parents[v]];
+  }
+  return BoruvkaResult{nb_mst, n};
+}
+
+// localMST::get_pivot (mst_utils_filter.hpp:13-30): 25th percentile of 50
+// mt19937(0) samples.  The pivot only changes speed, never the result.
+u32 local_pivot(const LEdge* edges, std::size_t m, double balance_ratio = 0.25) {
+  if (m == 0) return 1;
+  if (m <= 2) return edges[0].w;
+  std::mt19937 gen(0);
+  std::uniform_int_distribution<> distrib(0, int(m - 1));
+  const std::size_t num_samples = 50;
+  std::vector<u32> samples(num_samples);
+  for (std::size_t i = 0; i < num_samples; ++i) samples[i] = edges[distrib(gen)].w;
+  std::sort(samples.begin(), samples.end());
+  return samples[std::size_t(num_samples * balance_ratio)];
+}
+
+bool is_unbalanced(std::size_t num_edges, std::size_t num_light) {  // mst_utils_filter.hpp:32-37
+  const std::size_t diff = std::size_t(std::abs(i64(num_edges) - i64(num_light)));
+  const double ratio = double(diff) / double(num_edges);
+  return std::abs(0.5 - ratio) > 0.4;
+}
+
+// boruvka_filter_rec (local_mst_respecting_cut_edges.hpp:209-260)
+BoruvkaResult boruvka_filter_rec(LocalState& st, LEdge* edges, std::size_t m,
+                                 std::size_t n_active) {
+  if (n_active <= 1) return BoruvkaResult{0, n_active};
+  if (m <= 3 * n_active) return boruvka3(st, edges, m, n_active);
+  const u32 pivot = local_pivot(edges, m);
+  std::size_t num_light = 0;
+#pragma omp parallel for reduction(+ : num_light)
+  for (i64 i = 0; i < i64(m); ++i) num_light += (edges[i].w <= pivot);
+  if (is_unbalanced(m, num_light)) return boruvka3(st, edges, m, n_active);
+  std::vector<LEdge> light(num_light);
+  pack(edges, m, light.data(), [&](std::size_t i) { return edges[i].w <= pivot; });
+  const BoruvkaResult lr = boruvka_filter_rec(st, light.data(), num_light, n_active);
+  light.clear();
+  light.shrink_to_fit();
+  // localMST::filter (mst_utils_filter.hpp:103-126): light edges become
+  // (0,0) self-loops, heavy edges are relabelled by their parents, then
+  // self-loops are dropped.
+#pragma omp parallel for
+  for (i64 i = 0; i < i64(m); ++i) {
+    LEdge& e = edges[i];
+    const bool small = e.w <= pivot;
+    e.src = small ? 0 : st.parents[e.src];
+    e.dst = small ? 0 : st.parents[e.dst];
+  }
+  std::vector<LEdge> heavy(m);
+  const std::size_t mh = pack(edges, m, heavy.data(),
+                              [&](std::size_t i) { return edges[i].src != edges[i].dst; });
+  std::copy(heavy.begin(), heavy.begin() + mh, edges);
+  BoruvkaResult res = boruvka3(st, edges, mh, lr.num_active_vertices);
+  res.num_mst_edges_found += lr.num_mst_edges_found;
+  return res;
+}
+
+// ---------------------------------------------------------------------------
+// Paths (b) and (c): the distributed rounds on directed records, one process.
+
+struct DEdge {  // directed working record (WEdgeId_4_1_7 role)
+  u32 src, dst;
+  u64 key;      // (w << 32) | uid
+};
+
+// Relabel + self-loop removal + (src,dst) sort + keep lightest copy:
+// EdgeRenamer::rename_edges (edge_renaming.hpp:11-38),
+// EdgeProcessor::remove_self_loops / remove_duplicates
+// (duplicate_detection.hpp:11-64), RedistributeViaSelectedSorter (misc.hpp:121-143).
+void contract(std::vector<DEdge>& edges, const std::vector<u32>& label, bool dedup) {
+  std::vector<DEdge> out(edges.size());
+#pragma omp parallel for
+  for (i64 i = 0; i < i64(edges.size()); ++i) {
+    edges[i].src = label[edges[i].src];
+    edges[i].dst = label[edges[i].dst];
+  }
+  const std::size_t m = pack(edges.data(), edges.size(), out.data(),
+                             [&](std::size_t i) { return edges[i].src != edges[i].dst; });
+  out.resize(m);
+  if (!dedup) {
+    edges.swap(out);
+    return;
+  }
+  std::sort(out.begin(), out.end(), [](const DEdge& a, const DEdge& b) {
+    return std::tie(a.src, a.dst, a.key) < std::tie(b.src, b.dst, b.key);
+  });
+  std::size_t k = 0;
+  for (std::size_t i = 0; i < out.size(); ++i)
+    if (i == 0 || out[i].src != out[i - 1].src || out[i].dst != out[i - 1].dst) out[k++] = out[i];
+  out.resize(k);
+  edges.swap(out);
+}
+
+// One Boruvka round over directed records whose endpoints are in [0, n):
+// MinimumEdgeOpenMP (minimum_edge_computation.hpp:48-73, key (w, dst) refined
+// to the strict key), IsRepresentative_Push with the parity root rule
+// (is_representative_computation.hpp:69-75), AddMstEdgesSeq
+// (edge_renaming.hpp:56-66), ComputeRepresentative pointer jumping
+// (representative_computation.hpp:357-421) and MakeVerticesConsecutive
+// renumbering (edge_consecutive.hpp:16-60).  Returns the new vertex count
+// and writes label[v] in [0, n') for every v; parent[] (over [0,n)) gets the
+// root in the old numbering.
+u32 round_step(std::vector<DEdge>& edges, u32 n, std::vector<u32>& mst,
+               std::vector<u32>& label, std::vector<u32>& parent, bool parity_rule) {
+  std::vector<std::atomic<u64>> best(n);
+  std::vector<u32> best_dst(n, U32_NONE);
+#pragma omp parallel for
+  for (i64 v = 0; v < i64(n); ++v) best[v].store(KEY_NONE, std::memory_order_relaxed);
+#pragma omp parallel for
+  for (i64 i = 0; i < i64(edges.size()); ++i) write_min(best[edges[i].src], edges[i].key);
+#pragma omp parallel for
+  for (i64 i = 0; i < i64(edges.size()); ++i)
+    if (best[edges[i].src].load(std::memory_order_relaxed) == edges[i].key)
+      best_dst[edges[i].src] = edges[i].dst;
+  parent.assign(n, 0);
+  std::vector<u32> add(n, U32_NONE);
+  std::vector<unsigned char> has_edge(n, 0);
+#pragma omp parallel for
+  for (i64 vi = 0; vi < i64(n); ++vi) {
+    const u32 v = u32(vi);
+    const u64 k = best[v].load(std::memory_order_relaxed);
+    if (k == KEY_NONE) {
+      parent[v] = v;
+      continue;
+    }
+    has_edge[v] = 1;
+    const u32 u = best_dst[v];
+    const bool mutual = best[u].load(std::memory_order_relaxed) == k;
+    bool v_root = false;
+    if (mutual) {
+      if (parity_rule) {
+        const bool even = ((u64(v) + u) % 2) == 0;
+        v_root = (even == (v < u));
+      } else {
+        v_root = v < u;
+      }
+    }
+    if (v_root) {
+      parent[v] = v;
+    } else {
+      parent[v] = u;
+      add[v] = u32(k);
+    }
+  }
+  for (u32 v = 0; v < n; ++v)
+    if (add[v] != U32_NONE) mst.push_back(add[v]);
+#pragma omp parallel for
+  for (i64 v = 0; v < i64(n); ++v) {
+    u32 p = parent[v];
+    while (parent[p] != p) p = parent[p];
+    parent[v] = p;
+  }
+  // consecutive renumbering of the roots that still have edges
+  std::vector<u32> root_id(n, U32_NONE);
+  u32 nn = 0;
+  for (u32 v = 0; v < n; ++v)
+    if (parent[v] == v && has_edge[v]) root_id[v] = nn++;
+  label.assign(n, U32_NONE);
+#pragma omp parallel for
+  for (i64 v = 0; v < i64(n); ++v) label[v] = root_id[parent[v]];
+  return nn;
+}
+
+// Plain distributed rounds until fewer than 35000 vertices remain
+// (stop_boruvka, misc.hpp:27-48), then the dense base case
+// (dense_boruvka_allreduce / gbbs_boruvka_distributed::Boruvka,
+// boruvka_allreduce.hpp:313-427: lower endpoint is the root, no dedup).
+// comp[] (optional) maps each vertex of the starting numbering to its final
+// root in that numbering.
+void boruvka_rounds(std::vector<DEdge> edges, u32 n, std::vector<u32>& mst,
+                    std::vector<u32>* comp) {
+  std::vector<u32> label, parent;
+  std::vector<u32> to_start;  // current vertex -> a start vertex (its root)
+  std::vector<u32> comp_local(n);
+  std::iota(comp_local.begin(), comp_local.end(), 0u);
+  // cur_of_start[v]: current id of the component of start vertex v (or NONE).
+  std::vector<u32> cur_of_start(n);
+  std::iota(cur_of_start.begin(), cur_of_start.end(), 0u);
+  std::vector<u32> start_of_cur(n);
+  std::iota(start_of_cur.begin(), start_of_cur.end(), 0u);
+  u32 cur_n = n;
+  bool base = false;
+  while (cur_n > 1 && !edges.empty()) {
+    if (!base && cur_n < 35000) base = true;  // switch to the dense base case
+    const u32 nn = round_step(edges, cur_n, mst, label, parent, /*parity=*/!base);
+    // start-vertex bookkeeping: component root of each start vertex.
+    std::vector<u32> new_start_of_cur(nn, U32_NONE);
+    for (u32 v = 0; v < cur_n; ++v)
+      if (parent[v] == v && label[v] != U32_NONE) new_start_of_cur[label[v]] = start_of_cur[v];
+#pragma omp parallel for
+    for (i64 s = 0; s < i64(n); ++s) {
+      const u32 c = cur_of_start[s];
+      if (c == U32_NONE) continue;
+      comp_local[s] = start_of_cur[parent[c]];
+      cur_of_start[s] = label[c];
+    }
+    start_of_cur.swap(new_start_of_cur);
+    contract(edges, label, /*dedup=*/!base);
+    cur_n = nn;
+  }
+  if (comp) {
+    // comp_local[s] is a start vertex; chase to the final representative.
+#pragma omp parallel for
+    for (i64 s = 0; s < i64(n); ++s) {
+      u32 r = comp_local[s];
+      while (comp_local[r] != r) r = comp_local[r];
+      (*comp)[s] = r;
+    }
+  }
+}
+
+// Build the directed working records (both copies, strict key).
+std::vector<DEdge> directed_records(const WEdge* in, u64 N, const std::vector<u32>& uid) {
+  std::vector<DEdge> e(N);
+#pragma omp parallel for
+  for (i64 i = 0; i < i64(N); ++i)
+    e[i] = DEdge{u32(in[i].src), u32(in[i].dst), make_key(in[i].weight, uid[i])};
+  return e;
+}
+
+// Path (c): filter-Boruvka recursion boruvka_recursive
+// (include/algorithms/hybrid_boruvka_edgefilter.hpp:227-330) with
+// partition_edges (filter_helpers.hpp:101-139, pivot = median weight of 8
+// samples, distributed_partitioning.hpp:20-33, 92-120) and
+// boruvka_recursive_setup (hybrid_boruvka_edgefilter.hpp:173-225).  The
+// parent array (ParentArray, distributed_parent_array.hpp:16-81) is `parent`
+// over the original vertex ids.
+struct FilterCtx {
+  u32 n0;
+  u64 threshold;
+  std::vector<u32> parent;      // ParentArray over original ids
+  std::vector<unsigned char> has_edges;
+  std::vector<u32> mst;
+  int leftover_counter = 0;     // LeftoverManager (hybrid_boruvka_edgefilter.hpp:32-37)
+  std::vector<DEdge> leftovers;
+};
+
+u64 global_num_vertices(const FilterCtx& c) {  // distributed_parent_array.hpp:62-72
+  u64 roots = 0;
+#pragma omp parallel for reduction(+ : roots)
+  for (i64 v = 0; v < i64(c.n0); ++v) roots += (c.parent[v] == u32(v) && c.has_edges[v]);
+  return roots;
+}
+
+// filter_boruvka_basecase (hybrid_boruvka_edgefilter.hpp:130-171): compact the
+// endpoints, run the rounds, write the new roots into the parent array and
+// shortcut it (255).
+void filter_basecase(FilterCtx& c, std::vector<DEdge>& edges) {
+  std::vector<u32> present(c.n0, 0);
+  for (const DEdge& e : edges) present[e.src] = 1;
+  std::vector<u32> orig;  // compact id -> original id
+  std::vector<u32> cid(c.n0, U32_NONE);
+  for (u32 v = 0; v < c.n0; ++v)
+    if (present[v]) {
+      cid[v] = u32(orig.size());
+      orig.push_back(v);
+    }
+  std::sort(edges.begin(), edges.end(), [](const DEdge& a, const DEdge& b) {
+    return std::tie(a.src, a.dst, a.key) < std::tie(b.src, b.dst, b.key);
+  });
+  for (DEdge& e : edges) {
+    e.src = cid[e.src];
+    e.dst = cid[e.dst];
+  }
+  const u32 nl = u32(orig.size());
+  std::vector<u32> comp(nl);
+  boruvka_rounds(std::move(edges), nl, c.mst, &comp);
+  for (u32 i = 0; i < nl; ++i) c.parent[orig[i]] = orig[comp[i]];
+  for (u32 v = 0; v < c.n0; ++v) {  // shortcut
+    u32 r = c.parent[v];
+    while (c.parent[r] != r) r = c.parent[r];
+    c.parent[v] = r;
+  }
+}
+
+void filter_recursive(FilterCtx& c, std::vector<DEdge>& edges, u64 global_n,
+                      u64 global_m, u32 threshold_weight, bool stop_recursion) {
+  if (global_m == 0 || global_n <= 1) return;
+  const double avg_degree = double(global_m) / double(c.n0);
+  const bool is_graph_small = global_m <= 1000;
+  if (avg_degree <= double(c.threshold) || stop_recursion || is_graph_small) {
+    filter_basecase(c, edges);
+    return;
+  }
+  // partition_edges: 8 samples, mt19937(rank + size % 1000), median weight.
+  std::mt19937 gen(unsigned(0 + edges.size() % 1000));
+  std::uniform_int_distribution<std::size_t> idx(0, edges.size() - 1);
+  std::vector<u32> samples(8);
+  for (auto& s : samples) s = u32(edges[idx(gen)].key >> 32);
+  std::sort(samples.begin(), samples.end());
+  const u32 pivot = samples[samples.size() / 2];
+  std::vector<DEdge> light, heavy;
+  light.reserve(edges.size());
+  heavy.reserve(edges.size());
+  for (const DEdge& e : edges) (u32(e.key >> 32) <= pivot ? light : heavy).push_back(e);
+  const bool stop = double(light.size()) / double(edges.size()) > 0.85;
+  edges.swap(heavy);
+  heavy.clear();
+  c.leftover_counter++;
+  filter_recursive(c, light, global_n, light.size(), threshold_weight, stop);
+  c.leftover_counter--;
+  const u64 n_filtered = global_num_vertices(c);
+  if (n_filtered <= 1) return;
+  // boruvka_recursive_setup: relabel by the parent array, drop self-loops,
+  // merge leftovers, sort, keep the lightest copy per (src, dst).
+  for (DEdge& e : edges) {
+    e.src = c.parent[e.src];
+    e.dst = c.parent[e.dst];
+  }
+  std::vector<DEdge> kept;
+  kept.reserve(edges.size() + c.leftovers.size());
+  for (const DEdge& e : edges)
+    if (e.src != e.dst) kept.push_back(e);
+  for (const DEdge& e : c.leftovers) kept.push_back(e);
+  c.leftovers.clear();
+  std::sort(kept.begin(), kept.end(), [](const DEdge& a, const DEdge& b) {
+    return std::tie(a.src, a.dst, a.key) < std::tie(b.src, b.dst, b.key);
+  });
+  std::size_t k = 0;
+  for (std::size_t i = 0; i < kept.size(); ++i)
+    if (i == 0 || kept[i].src != kept[i - 1].src || kept[i].dst != kept[i - 1].dst) kept[k++] = kept[i];
+  kept.resize(k);
+  edges.swap(kept);
+  // do_perform_right_recursion (hybrid_boruvka_edgefilter.hpp:39-44)
+  const bool perform = c.leftover_counter <= 0 || edges.size() >= 10000;
+  if (perform) {
+    filter_recursive(c, edges, n_filtered, edges.size(), pivot, false);
+  } else {
+    c.leftovers.swap(edges);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Pair every record with its (min, max) twin; uid[i] receives the position of
+// the twin (i itself for a min<max record).  Returns 0 or an error status.
+int orc_pair_records(const WEdge* in, u64 N, u32* uid_out, u64* n_out) {
+  std::vector<u32> uid;
+  u64 n = 0;
+  const int st = pair_records(in, N, uid, n);
+  if (st != OK) return st;
+  if (uid_out) std::memcpy(uid_out, uid.data(), N * sizeof(u32));
+  if (n_out) *n_out = n;
+  return OK;
+}
+
+// Kruskal of the reference check: sort directed records by (w, src, dst),
+// union-find (include/algorithms/base_case_mst_algos.hpp:92-113,
+// benchmarks/mst_benchmarks.hpp:287-318).  Output canonical (min, max, w)
+// sorted by (min, max).
+int orc_kruskal(const WEdge* in, u64 N, WEdge* out, u64* out_n, u64* total) {
+  std::vector<u32> uid;
+  u64 n = 0;
+  const int st = pair_records(in, N, uid, n);
+  if (st != OK) return st;
+  std::vector<u32> order(N);
+  std::iota(order.begin(), order.end(), 0u);
+  std::sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+    return std::make_tuple(in[a].weight, in[a].src, in[a].dst) <
+           std::make_tuple(in[b].weight, in[b].src, in[b].dst);
+  });
+  UnionFind uf(n);
+  std::vector<u32> mst;
+  for (u32 i : order) {
+    if (uf.unify(u32(in[i].src), u32(in[i].dst))) mst.push_back(uid[i]);
+  }
+  emit(in, mst, out, out_n, total);
+  return OK;
+}
+
+// hybridMST::boruvka on one process (include/algorithms/hybrid_boruvka.hpp:263-301).
+// local_preprocessing = 1 (MODIFIED_BORUVKA, the driver default,
+// benchmarks/mst_benchmarks.hpp:84): local filter-Boruvka contracts the whole
+// graph.  local_preprocessing = 0: boruvka_core rounds + dense base case.
+int orc_boruvka(const WEdge* in, u64 N, int local_preprocessing, int threads,
+                WEdge* out, u64* out_n, u64* total) {
+  if (threads > 0) omp_set_num_threads(threads);
+  std::vector<u32> uid;
+  u64 n = 0;
+  const int st = pair_records(in, N, uid, n);
+  if (st != OK) return st;
+  std::vector<u32> mst;
+  if (local_preprocessing != 0) {
+    std::vector<LEdge> local;
+    local.reserve(N / 2);
+    for (u64 i = 0; i < N; ++i)
+      if (in[i].src < in[i].dst) local.push_back(LEdge{u32(in[i].src), u32(in[i].dst), in[i].weight, u32(i)});
+    LocalState s;
+    s.n_initial = u32(n);
+    s.parents.resize(n);
+    std::iota(s.parents.begin(), s.parents.end(), 0u);
+    s.min_edges = std::vector<std::atomic<u64>>(n);
+    s.vertices.resize(n);
+    s.vertices_tmp.resize(n);
+    std::iota(s.vertices.begin(), s.vertices.end(), 0u);
+    boruvka_filter_rec(s, local.data(), local.size(), n);
+    mst.swap(s.mst);
+  } else {
+    boruvka_rounds(directed_records(in, N, uid), u32(n), mst, nullptr);
+  }
+  emit(in, mst, out, out_n, total);
+  return OK;
+}
+
+// hybridMST::filter_boruvka on one process
+// (include/algorithms/hybrid_boruvka_edgefilter.hpp:420-462).
+int orc_filter_boruvka(const WEdge* in, u64 N, int local_preprocessing,
+                       u64 filter_threshold, int threads, WEdge* out, u64* out_n,
+                       u64* total) {
+  if (local_preprocessing != 0)  // local contraction leaves no edges on one process
+    return orc_boruvka(in, N, 1, threads, out, out_n, total);
+  if (threads > 0) omp_set_num_threads(threads);
+  std::vector<u32> uid;
+  u64 n = 0;
+  const int st = pair_records(in, N, uid, n);
+  if (st != OK) return st;
+  FilterCtx c;
+  c.n0 = u32(n);
+  c.threshold = filter_threshold;
+  c.parent.resize(n);
+  std::iota(c.parent.begin(), c.parent.end(), 0u);
+  c.has_edges.assign(n, 0);
+  for (u64 i = 0; i < N; ++i) c.has_edges[in[i].src] = 1;  // init_parent_array
+  std::vector<DEdge> edges = directed_records(in, N, uid);
+  u64 nv = 0;
+  for (u32 v = 0; v < n; ++v) nv += c.has_edges[v];
+  filter_recursive(c, edges, n, edges.size(), 0, false);
+  emit(in, c.mst, out, out_n, total);
+  return OK;
+}
+
+}  // extern "C"

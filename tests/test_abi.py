@@ -1,0 +1,82 @@
+"""C-ABI and host-side logic, CPU only (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import kamsta_amd as k
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "kmst.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kmst_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(k.LIB_PATH)
+    declared = header_symbols()
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(k.EXPORTED_SYMBOLS) == declared
+
+
+def test_config_default_matches_reference():
+    cfg = k._Config()
+    k.load().kmst_config_default(ctypes.byref(cfg))
+    # AlgorithmConfig defaults: NO_PREPROCESSING, NO_COMPRESSION, threshold 4
+    assert (cfg.algorithm, cfg.local_preprocessing, cfg.compression, cfg.filter_threshold) == (0, 0, 0, 4)
+
+
+def test_host_api_rejects_wide_ids_without_gpu():
+    e = np.zeros(2, dtype=k.WEDGE_DTYPE)
+    e["src"] = [0, 1 << 33]
+    e["dst"] = [1 << 33, 0]
+    e["weight"] = 1
+    with pytest.raises(k.KmstError) as err:
+        k.boruvka(e)
+    assert err.value.status == k.KMST_EINVAL
+    with pytest.raises(k.KmstError) as err:
+        k.boruvka(e, k.AlgorithmConfig(compression=1))
+    assert err.value.status == k.KMST_EINVAL
+
+
+@pytest.mark.parametrize("n,m,mw,seed", [(2, 1, 254, 0), (16, 120, 3, 1), (300, 2000, 254, 2),
+                                         (1 << 14, 1 << 16, 254, 0x4B414D5354410001),
+                                         (256, 4000, 254, 9)])
+def test_gnm_host_contract(n, m, mw, seed):
+    s, d, w = k.gen_gnm_host(n, m, mw, seed)
+    assert len(s) == 2 * m
+    key = s.astype(np.uint64) << 32 | d
+    assert np.all(np.diff(key.astype(np.int64)) > 0), "sorted, no duplicate records"
+    assert np.all(s != d) and s.max() < n and d.max() < n
+    assert np.all((w >= 1) & (w <= mw))
+    fwd = s < d
+    assert fwd.sum() == m
+    # symmetric with equal weights
+    rev = {(int(b), int(a)): int(x) for a, b, x in zip(s[~fwd], d[~fwd], w[~fwd])}
+    for a, b, x in zip(s[fwd], d[fwd], w[fwd]):
+        assert rev[(int(a), int(b))] == int(x)
+    s2, d2, w2 = k.gen_gnm_host(n, m, mw, seed)
+    assert np.array_equal(s, s2) and np.array_equal(d, d2) and np.array_equal(w, w2)
+
+
+def test_gnm_prefix_property():
+    # the first m distinct pairs of the counter sequence: a smaller m is a subset
+    s1, d1, _ = k.gen_gnm_host(500, 800, 254, 3)
+    s2, d2, _ = k.gen_gnm_host(500, 1600, 254, 3)
+    a = set(zip(s1.tolist(), d1.tolist()))
+    b = set(zip(s2.tolist(), d2.tolist()))
+    assert a < b
+
+
+def test_gnm_rejects_bad_args():
+    with pytest.raises(k.KmstError):
+        k.gen_gnm_host(4, 7, 254, 1)  # more than n(n-1)/2 pairs
+    with pytest.raises(k.KmstError):
+        k.gen_gnm_host(4, 2, 0, 1)

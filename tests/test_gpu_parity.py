@@ -1,0 +1,233 @@
+"""GPU parity: libkmst (through the C ABI) against the CPU oracle and numpy
+references, bit-exact (integer work)."""
+import numpy as np
+import pytest
+
+import graphs
+import kamsta_amd as k
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(torch, arr, dtype):
+    a = np.ascontiguousarray(arr)
+    if dtype == np.uint32:
+        return torch.from_numpy(a.astype(np.uint32).view(np.int32)).cuda()
+    return torch.from_numpy(a.astype(np.uint64).view(np.int64)).cuda()
+
+
+def host(t, dtype):
+    return t.cpu().numpy().view(dtype)
+
+
+def strip(rec):
+    return rec[["src", "dst", "weight"]]
+
+
+def check_equal(rec, threshold=4):
+    exp, exp_w = oracle.kruskal(rec)
+    for fn in (k.boruvka, k.filter_boruvka):
+        got, got_w = fn(rec, k.AlgorithmConfig(filter_threshold=threshold))
+        assert got_w == exp_w, (fn.__name__, got_w, exp_w)
+        assert np.array_equal(strip(got), strip(exp)), fn.__name__
+    return exp, exp_w
+
+
+# ------------------------------------------------------------------ kernels
+@pytest.mark.parametrize("case", ["uniform", "hubs", "single", "sparse_ids"])
+def test_segmented_argmin_kernel(gpu, ctx, case):
+    torch = gpu
+    rng = np.random.default_rng(11)
+    if case == "uniform":
+        n, deg = 5000, rng.integers(1, 40, 5000)
+    elif case == "hubs":
+        n = 300
+        deg = rng.integers(1, 5, n)
+        deg[[3, 150, 299]] = [70000, 5000, 9000]  # cross many 2048-record tiles
+    elif case == "single":
+        n, deg = 1, np.array([100000])
+    else:
+        n = 20000
+        deg = np.where(rng.random(n) < 0.3, rng.integers(1, 9, n), 0)
+    src = np.repeat(np.arange(n, dtype=np.uint32), deg)
+    m = len(src)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    val = rng.permutation(m).astype(np.uint64) * 7 + 3  # unique keys
+    val[rng.random(m) < 0.05] = np.uint64(2**64 - 1)  # some dead records
+    bv = dev(torch, np.zeros(n, np.uint64), np.uint64)
+    bd = dev(torch, np.zeros(n, np.uint32), np.uint32)
+    ctx.segmented_argmin(dev(torch, src, np.uint32), dev(torch, dst, np.uint32),
+                         dev(torch, val, np.uint64), m, n, bv, bd)
+    torch.cuda.synchronize()
+    gv, gd = host(bv, np.uint64), host(bd, np.uint32)
+    # numpy reference
+    exp_v = np.full(n, np.uint64(2**64 - 1))
+    exp_d = np.zeros(n, np.uint32)
+    order = np.lexsort((val, src))
+    first = np.ones(m, bool)
+    first[1:] = src[order][1:] != src[order][:-1]
+    idx = order[first]
+    exp_v[src[idx]] = val[idx]
+    exp_d[src[idx]] = dst[idx]
+    assert np.array_equal(gv, exp_v)
+    live = exp_v != np.uint64(2**64 - 1)
+    assert np.array_equal(gd[live], exp_d[live])
+
+
+@pytest.mark.parametrize("m,bits", [(1, 3), (5000, 4), (100003, 17), (300000, 24), (70000, 32)])
+def test_radix_sort_kernel_stable(gpu, ctx, m, bits):
+    torch = gpu
+    rng = np.random.default_rng(m)
+    hi = (1 << bits) if bits < 32 else (1 << 32)
+    s = rng.integers(0, min(hi, 50), m).astype(np.uint32)  # many equal keys: stability matters
+    d = rng.integers(0, hi, m).astype(np.uint32) if bits > 8 else rng.integers(0, hi, m).astype(np.uint32)
+    v = np.arange(m, dtype=np.uint64)
+    ts, td, tv = dev(torch, s, np.uint32), dev(torch, d, np.uint32), dev(torch, v, np.uint64)
+    ctx.sort_pairs(ts, td, tv, m, bits)
+    torch.cuda.synchronize()
+    order = np.lexsort((v, d, s))
+    assert np.array_equal(host(ts, np.uint32), s[order])
+    assert np.array_equal(host(td, np.uint32), d[order])
+    assert np.array_equal(host(tv, np.uint64), v[order])
+
+
+def test_pointer_jump_kernel(gpu, ctx):
+    torch = gpu
+    rng = np.random.default_rng(3)
+    n = 200000
+    # random forest: parent[v] < v or v (roots), plus long chains
+    parent = np.array([v if (v == 0 or rng.random() < 0.01) else rng.integers(max(0, v - 50), v)
+                       for v in range(n)], dtype=np.uint32)
+    chain = np.arange(50000, 60000, dtype=np.uint32)
+    parent[chain[1:]] = chain[:-1]
+    p = dev(torch, parent, np.uint32)
+    ctx.pointer_jump(p, n)
+    got = host(p, np.uint32)
+    exp = parent.copy()
+    for v in range(n):
+        r = exp[v]
+        while exp[r] != r:
+            r = exp[r]
+        exp[v] = r
+    assert np.array_equal(got, exp)
+
+
+# ------------------------------------------------------------------ end to end
+@pytest.mark.parametrize("seed", range(40))
+def test_small_random_graphs(gpu, seed):
+    n = 2 + seed % 9
+    m = min(n * (n - 1) // 2, 1 + seed % 12)
+    und = graphs.random_undirected(n, m, 1 + seed % 3, seed)
+    rec = graphs.records(und)
+    if len(rec) == 0:
+        return
+    exp, _ = check_equal(rec, threshold=seed % 3)
+    assert graphs.canonical(exp) == graphs.brute_force_msf(n, und)
+
+
+def test_hand_graphs(gpu):
+    cases = [
+        [(i, i + 1, 1 + (7 * i) % 5) for i in range(300)],                 # path
+        [(0, i, 3) for i in range(1, 400)],                                # star (hub)
+        [(i, (i + 1) % 500, 9) for i in range(500)],                       # cycle, ties
+        [(u, v, 1) for u in range(60) for v in range(u + 1, 60)],          # K60 equal weights
+        [(0, 1, 5), (1, 2, 5), (0, 2, 5), (10, 11, 1), (200, 300, 7)],     # forest, isolated ids
+    ]
+    for und in cases:
+        check_equal(graphs.records(und), threshold=2)
+
+
+def test_multigraph(gpu):
+    for seed in range(10):
+        und = graphs.random_undirected(40, 300, 4, 500 + seed, allow_multi=True)
+        check_equal(graphs.records(und), threshold=1)
+
+
+@pytest.mark.parametrize("n,m,mw,seed", [(1 << 14, 1 << 18, 254, 0x4B414D5354410001),
+                                         (1 << 16, 1 << 20, 254, 5), (100000, 150000, 254, 6),
+                                         (1 << 12, 1 << 20, 254, 7), (70000, 900000, (1 << 31) - 1, 8)])
+def test_gnm_vs_oracle(gpu, n, m, mw, seed):
+    s, d, w = k.gen_gnm_host(n, m, mw, seed)
+    rec = k.to_wedges(s, d, w)
+    for thr in (4, 16):
+        check_equal(rec, threshold=thr)
+
+
+def test_golden_fixtures_gpu(gpu):
+    import os
+    gdir = os.path.join(os.path.dirname(__file__), "golden")
+    for f in sorted(os.listdir(gdir)):
+        if not f.endswith(".npz"):
+            continue
+        z = np.load(os.path.join(gdir, f))
+        rec = np.zeros(len(z["src"]), dtype=k.WEDGE_DTYPE)
+        rec["src"], rec["dst"], rec["weight"] = z["src"], z["dst"], z["weight"]
+        for fn in (k.boruvka, k.filter_boruvka):
+            got, tw = fn(rec, k.AlgorithmConfig(filter_threshold=1))
+            assert tw == int(z["total_weight"]), f
+            assert np.array_equal(got["src"], z["mst_src"]) and np.array_equal(got["dst"], z["mst_dst"])
+            assert np.array_equal(got["weight"], z["mst_weight"])
+
+
+def test_contract_errors(gpu):
+    good = graphs.records([(0, 1, 3), (1, 2, 4), (0, 2, 9)])
+    bad = good.copy()
+    bad[[0, 1]] = bad[[1, 0]]
+    with pytest.raises(k.KmstError) as e:
+        k.boruvka(bad)
+    assert e.value.status == k.KMST_EUNSORTED
+    asym = good.copy()
+    asym["weight"][0] = 99
+    with pytest.raises(k.KmstError) as e:
+        k.filter_boruvka(asym)
+    assert e.value.status == k.KMST_EASYMMETRIC
+    loop = good.copy()
+    loop["dst"][0] = 0
+    with pytest.raises(k.KmstError) as e:
+        k.boruvka(loop)
+    assert e.value.status in (k.KMST_EINVAL, k.KMST_EUNSORTED)
+    odd = good[:-1].copy()
+    with pytest.raises(k.KmstError) as e:
+        k.boruvka(odd)
+    assert e.value.status == k.KMST_EASYMMETRIC
+    # the library still works after errors
+    check_equal(good)
+    # empty input
+    got, tw = k.boruvka(np.zeros(0, dtype=k.WEDGE_DTYPE))
+    assert len(got) == 0 and tw == 0
+
+
+def test_device_generator_matches_host(gpu, ctx):
+    torch = gpu
+    for n, m, mw, seed in [(1 << 14, 1 << 18, 254, 1), (256, 4000, 254, 2), (1 << 20, 1 << 22, 254, 3)]:
+        s = torch.empty(2 * m, dtype=torch.int32, device="cuda")
+        d = torch.empty_like(s)
+        w = torch.empty_like(s)
+        ctx.gen_gnm_device(n, m, mw, seed, s, d, w)
+        hs, hd, hw = k.gen_gnm_host(n, m, mw, seed)
+        assert np.array_equal(host(s, np.uint32), hs)
+        assert np.array_equal(host(d, np.uint32), hd)
+        assert np.array_equal(host(w, np.uint32), hw)
+
+
+def test_device_solve_and_stats(gpu, ctx):
+    torch = gpu
+    n, m = 1 << 16, 1 << 20
+    s = torch.empty(2 * m, dtype=torch.int32, device="cuda")
+    d, w = torch.empty_like(s), torch.empty_like(s)
+    ctx.gen_gnm_device(n, m, 254, 42, s, d, w)
+    os_, od, ow = (torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3))
+    rec = k.to_wedges(host(s, np.uint32), host(d, np.uint32), host(w, np.uint32))
+    exp, exp_w = oracle.kruskal(rec)
+    ctx.set_timing(True)
+    for algo in (k.BORUVKA, k.FILTER_BORUVKA):
+        cnt, tw = ctx.solve_device(s, d, w, 2 * m, n, os_, od, ow, n, algorithm=algo)
+        assert tw == exp_w and cnt == len(exp)
+        assert np.array_equal(host(os_, np.uint32)[:cnt], exp["src"])
+        assert np.array_equal(host(od, np.uint32)[:cnt], exp["dst"])
+        assert np.array_equal(host(ow, np.uint32)[:cnt], exp["weight"])
+        st = ctx.stats()
+        assert st["mst_edges"] == cnt and st["rounds"]
+        assert st["kernels"]["argmin"]["launches"] >= 1
+    ctx.set_timing(False)
