@@ -104,6 +104,13 @@ def main():
         ms, st = timed(algos[name], args.steps, args.warmup, True)
         results[name] = (ms, st)
         log(f"[bench] {name}: {ms:.2f} ms/step, {m / (ms / 1000.0) / 1e9:.3f} G input edges/s")
+        kt = st["kernels"]
+        log(f"[bench] {name} kernels: " + json.dumps(
+            {kk: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches": v["launches"] / args.steps,
+                  "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)}
+             for kk, v in sorted(kt.items(), key=lambda x: -x[1]["ms"])}))
+        log(f"[bench] {name} rounds: " + json.dumps(st["rounds"][:40]))
+        log(f"[bench] {name} filter: " + json.dumps(st["filter"]))
     best = min(results, key=lambda a: results[a][0])
     ms, st = results[best]
 
@@ -126,9 +133,6 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic,
                 "end_to_end_frac": round(total_bytes / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}
-    log("[bench] kernels: " + json.dumps({kk: {"ms_per_step": round(v["ms"] / args.steps, 3),
-                                              "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)}
-                                         for kk, v in sorted(kern.items(), key=lambda x: -x[1]["ms"])}))
 
     cpu = None
     if not args.no_cpu:
@@ -139,7 +143,8 @@ def main():
         rec = k.to_wedges(hs.cpu().numpy().view(np.uint32), hd.cpu().numpy().view(np.uint32),
                           hw.cpu().numpy().view(np.uint32))
         cores = min(os.cpu_count() or 1, 16)
-        oracle.boruvka(rec[: 2 * 1024], 1, cores)  # load + warm
+        ws, wd, ww = k.gen_gnm_host(1 << 10, 1 << 13, 254, 1)
+        oracle.boruvka(k.to_wedges(ws, wd, ww), 1, cores)  # load + warm
         t0 = time.perf_counter()
         cmst, cw = oracle.boruvka(rec, 1, cores)
         ct = time.perf_counter() - t0

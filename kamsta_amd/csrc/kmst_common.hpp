@@ -140,6 +140,67 @@ __device__ __forceinline__ T block_exclusive_sum(T x, T& total, T* lds) {
   return off + incl - x;
 }
 
+// ------------------------------------------------ decoupled look-back
+// Tile status word: bits 63..62 flag (0 none, 1 aggregate, 2 inclusive
+// prefix), bits 61..0 value.  Tiles take dynamic ids in dispatch order, so a
+// tile only ever waits for tiles that are already running, and every wait is
+// bounded: on expiry the kernel raises ERR_SPIN and the tile writes nothing,
+// which the host turns into an error.
+constexpr u64 LB_AGG = u64{1} << 62;
+constexpr u64 LB_INC = u64{2} << 62;
+constexpr u64 LB_VAL = (u64{1} << 62) - 1;
+constexpr u32 LB_SPIN_LIMIT = 1u << 22;
+
+__device__ __forceinline__ u64 lb_load(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(u64* p, u64 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by all 64 lanes of one wave; returns the exclusive prefix of `tile`
+// (wave-uniform) and publishes the inclusive one.  `ok` turns false when the
+// bounded wait expired; the caller must then write nothing.
+__device__ __forceinline__ u64 lookback_exclusive(u64* status, u32 tile, u64 aggregate, u32* err,
+                                                  bool& ok) {
+  ok = true;
+  const int lane = threadIdx.x & (WAVE - 1);
+  if (tile == 0) {
+    if (lane == 0) lb_store(status, LB_INC | aggregate);
+    return 0;
+  }
+  if (lane == 0) lb_store(status + tile, LB_AGG | aggregate);
+  u64 excl = 0;
+  i64 end = static_cast<i64>(tile);  // window covers tiles [end-64, end)
+  u32 spins = 0;
+  while (true) {
+    const i64 idx = end - 1 - lane;
+    const u64 w = (idx >= 0) ? lb_load(status + idx) : LB_INC;
+    const u64 flag = w >> 62;
+    const u64 inc_mask = __ballot(flag == 2);
+    const u64 none_mask = __ballot(flag == 0);
+    const int first_inc = inc_mask ? (__ffsll(static_cast<long long>(inc_mask)) - 1) : WAVE;
+    const u64 upto = (first_inc >= WAVE - 1) ? ~u64{0} : ((u64{2} << first_inc) - 1);
+    if (none_mask & upto) {
+      if (++spins > LB_SPIN_LIMIT) {
+        if (lane == 0) atomicOr(err, ERR_SPIN);
+        ok = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    u64 v = (lane <= first_inc) ? (w & LB_VAL) : 0;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    excl += v;
+    if (first_inc < WAVE) break;
+    end -= WAVE;
+  }
+  if (lane == 0) lb_store(status + tile, LB_INC | ((excl + aggregate) & LB_VAL));
+  return excl;
+}
+
 // Wave-aggregated append: returns the slot for lanes with pred set.
 __device__ __forceinline__ u32 wave_append(bool pred, u32* counter) {
   const u64 mask = __ballot(pred);

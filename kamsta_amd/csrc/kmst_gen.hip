@@ -117,12 +117,17 @@ void gen_gnm_device(kmst_context& C, u64 n, u64 m, u32 max_w, u64 seed, u32* out
   const u32 pair_bits = bits_for(n);
   u32* ctr = static_cast<u32*>(C.counters.get(256));
   for (int attempt = 0; attempt < 40; ++attempt) {
-    u32* a = static_cast<u32*>(C.gen_a.get(k * 4));
-    u32* b = static_cast<u32*>(C.gen_b.get(k * 4));
-    u64* cv = static_cast<u64*>(C.gen_c.get(k * 8));
-    u32* a2 = static_cast<u32*>(C.xs.get(std::max(k, 2 * m) * 4));
-    u32* b2 = static_cast<u32*>(C.xd.get(std::max(k, 2 * m) * 4));
-    u64* c2 = static_cast<u64*>(C.xv.get(std::max(k, 2 * m) * 8));
+    // Each of the six arrays serves as candidate storage (k entries), as the
+    // emit target and as the alt buffer of the final record sort (2m
+    // entries), depending on the parity of the pass counts; size all of
+    // them for the larger role.
+    const u64 cap = std::max(k, 2 * m);
+    u32* a = static_cast<u32*>(C.gen_a.get(cap * 4));
+    u32* b = static_cast<u32*>(C.gen_b.get(cap * 4));
+    u64* cv = static_cast<u64*>(C.gen_c.get(cap * 8));
+    u32* a2 = static_cast<u32*>(C.xs.get(cap * 4));
+    u32* b2 = static_cast<u32*>(C.xd.get(cap * 4));
+    u64* c2 = static_cast<u64*>(C.xv.get(cap * 8));
     u32* flag = static_cast<u32*>(C.gen_d.get(2 * k * 4));
     u32* rank = flag + k;
     const u64 hist_elems = sort_hist_elems(std::max(k, 2 * m));

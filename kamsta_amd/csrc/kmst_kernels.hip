@@ -498,88 +498,90 @@ void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_flag,
 }
 
 // ============================================================== K4+K5
-// Relabel both endpoints, drop self-loops and dead records, keep order.
-__device__ __forceinline__ bool keep_relabel(u32 s, u32 d, u64 v, const u32* label, u32& ls,
-                                             u32& ld) {
-  if (v == VAL_NONE) return false;
-  ls = label[s];
-  ld = label[d];
-  return ls != ld;
+// Relabel both endpoints, drop self-loops and dead records, keep order; one
+// pass: each wave owns a contiguous sub-tile (coalesced rows of 64 records),
+// ballots give the in-wave order, a decoupled look-back gives the tile offset.
+__device__ __forceinline__ u32 lb_tile_id(u32* counter) {
+  __shared__ u32 s_tile;
+  if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  return s_tile;
 }
 
-__global__ __launch_bounds__(BLOCK) void relabel_count_kernel(const u32* __restrict__ s_in,
-                                                              const u32* __restrict__ d_in,
-                                                              const u64* __restrict__ v_in, u64 m,
-                                                              const u32* __restrict__ label,
-                                                              u32* __restrict__ tile_counts) {
-  __shared__ u32 lds[WAVES];
-  const u64 base = u64(blockIdx.x) * CMP_TILE + u64(threadIdx.x) * CMP_IPT;
-  u32 c = 0;
-#pragma unroll 4
-  for (int k = 0; k < CMP_IPT; ++k) {
-    const u64 i = base + k;
-    if (i < m) {
-      u32 ls, ld;
-      c += keep_relabel(s_in[i], d_in[i], v_in[i], label, ls, ld) ? 1u : 0u;
-    }
-  }
-  u32 total;
-  block_exclusive_sum(c, total, lds);
-  if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(BLOCK) void relabel_scatter_kernel(
+__global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in, u64 m,
-    const u32* __restrict__ label, const u32* __restrict__ tile_off, u32* __restrict__ s_out,
-    u32* __restrict__ d_out, u64* __restrict__ v_out) {
-  __shared__ u32 lds[WAVES];
-  const u64 base = u64(blockIdx.x) * CMP_TILE + u64(threadIdx.x) * CMP_IPT;
+    const u32* __restrict__ label, u32* __restrict__ s_out, u32* __restrict__ d_out,
+    u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles, u32* count_dev, u32* err) {
+  __shared__ u32 s_wave_cnt[WAVES];
+  __shared__ u64 s_excl;
+  __shared__ u32 s_ok;
+  const u32 tile = lb_tile_id(lb_counter);
+  const int lane = lane_id(), w = wave_id();
+  const u64 wbase = u64(tile) * CMP_TILE + u64(w) * (WAVE * CMP_IPT);
   u32 ls[CMP_IPT], ld[CMP_IPT];
   u64 vv[CMP_IPT];
-  u32 keepmask = 0, c = 0;
+  u64 bal[CMP_IPT];
+  u32 cnt = 0;
 #pragma unroll
   for (int k = 0; k < CMP_IPT; ++k) {
-    const u64 i = base + k;
-    vv[k] = VAL_NONE;
+    const u64 e = wbase + u64(k) * WAVE + lane;
+    bool keep = false;
     ls[k] = ld[k] = 0;
-    if (i < m) {
-      vv[k] = v_in[i];
-      if (keep_relabel(s_in[i], d_in[i], vv[k], label, ls[k], ld[k])) {
-        keepmask |= 1u << k;
-        ++c;
+    vv[k] = VAL_NONE;
+    if (e < m) {
+      vv[k] = v_in[e];
+      if (vv[k] != VAL_NONE) {
+        ls[k] = label[s_in[e]];
+        ld[k] = label[d_in[e]];
+        keep = ls[k] != ld[k];
       }
     }
+    bal[k] = __ballot(keep);
+    cnt += static_cast<u32>(__popcll(bal[k]));
   }
-  u32 total;
-  u32 pos = block_exclusive_sum(c, total, lds) + tile_off[blockIdx.x];
+  if (lane == 0) s_wave_cnt[w] = cnt;
+  __syncthreads();
+  if (w == 0) {
+    u64 agg = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) agg += s_wave_cnt[ww];
+    bool ok;
+    const u64 excl = lookback_exclusive(lb_status, tile, agg, err, ok);
+    if (lane == 0) {
+      s_excl = excl;
+      s_ok = ok ? 1u : 0u;
+      if (tile == tiles - 1) *count_dev = static_cast<u32>(excl + agg);
+    }
+  }
+  __syncthreads();
+  if (!s_ok) return;
+  u64 pos = s_excl;
+  for (int ww = 0; ww < w; ++ww) pos += s_wave_cnt[ww];
 #pragma unroll
   for (int k = 0; k < CMP_IPT; ++k) {
-    if (keepmask & (1u << k)) {
-      s_out[pos] = ls[k];
-      d_out[pos] = ld[k];
-      v_out[pos] = vv[k];
-      ++pos;
+    if ((bal[k] >> lane) & 1ull) {
+      const u64 p = pos + mbcnt(bal[k]);
+      s_out[p] = ls[k];
+      d_out[p] = ld[k];
+      v_out[p] = vv[k];
     }
+    pos += static_cast<u32>(__popcll(bal[k]));
   }
 }
 
 void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
-                     const u32* label, u32* s_out, u32* d_out, u64* v_out, u32* tile_counts,
-                     u32* scan_scratch, u32* count_dev) {
+                     const u32* label, u32* s_out, u32* d_out, u64* v_out, u64* lb, u32* count_dev,
+                     u32* err) {
   if (m == 0) {
     fill_u32(st, count_dev, 1, 0);
     return;
   }
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
-  {
-    KTScope t(st, K_RELABEL_COUNT, 24.0 * m);
-    relabel_count_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, label, tile_counts);
-    KMST_HIP(hipGetLastError());
-  }
-  scan_exclusive_u32(st, tile_counts, tile_counts, tiles, scan_scratch, count_dev);
+  fill_u64(st, lb, u64(tiles) + 1, 0);  // tile states + dynamic tile counter
   KTScope t(st, K_RELABEL_SCATTER, 24.0 * m + 16.0 * m);
-  relabel_scatter_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, label, tile_counts, s_out,
-                                                   d_out, v_out);
+  relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, label, s_out, d_out, v_out,
+                                                   lb, reinterpret_cast<u32*>(lb + tiles), tiles,
+                                                   count_dev, err);
   KMST_HIP(hipGetLastError());
 }
 
@@ -848,79 +850,90 @@ void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32* out) {
   KMST_HIP(hipGetLastError());
 }
 
-// Stable two-way split by weight (partition_edges, filter_helpers.hpp:101-139):
-// light (w <= pivot) records to out[0, L), heavy ones to out[L, m), both in
-// input order.
-__global__ __launch_bounds__(BLOCK) void split_count_kernel(const u64* __restrict__ v, u64 m,
-                                                            u32 pivot, u32* __restrict__ counts) {
-  __shared__ u32 lds[WAVES];
-  const u64 base = u64(blockIdx.x) * CMP_TILE + u64(threadIdx.x) * CMP_IPT;
-  u32 c = 0;
-#pragma unroll 4
-  for (int k = 0; k < CMP_IPT; ++k) {
-    const u64 i = base + k;
-    if (i < m) c += (static_cast<u32>(v[i] >> 32) <= pivot) ? 1u : 0u;
-  }
-  u32 total;
-  block_exclusive_sum(c, total, lds);
-  if (threadIdx.x == 0) counts[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(BLOCK) void split_scatter_kernel(
+// Two-way split by weight (partition_edges, filter_helpers.hpp:101-139):
+// light (w <= pivot) records to out[0, L) in input order (they stay sorted by
+// (src, dst)); heavy records to out[L, m) in reverse order (they are
+// relabelled and re-sorted before use).  One pass with a decoupled look-back
+// over the light counts; heavy offsets follow from them.
+__global__ __launch_bounds__(BLOCK) void split_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in, u64 m,
-    u32 pivot, const u32* __restrict__ light_off, const u32* light_total, u32* __restrict__ s_out,
-    u32* __restrict__ d_out, u64* __restrict__ v_out) {
-  __shared__ u32 lds[WAVES];
-  const u64 tile_base = u64(blockIdx.x) * CMP_TILE;
-  const u64 base = tile_base + u64(threadIdx.x) * CMP_IPT;
-  u32 lightmask = 0, cl = 0, valid = 0;
+    u32 pivot, u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out,
+    u64* lb_status, u32* lb_counter, u32 tiles, u32* light_dev, u32* err) {
+  __shared__ u32 s_wave_light[WAVES];
+  __shared__ u64 s_excl;
+  __shared__ u32 s_ok;
+  const u32 tile = lb_tile_id(lb_counter);
+  const int lane = lane_id(), w = wave_id();
+  const u64 tile_base = u64(tile) * CMP_TILE;
+  const u64 wbase = tile_base + u64(w) * (WAVE * CMP_IPT);
+  u32 ss[CMP_IPT], dd[CMP_IPT];
+  u64 vv[CMP_IPT];
+  u64 bal[CMP_IPT];
+  u32 cnt = 0;
 #pragma unroll
   for (int k = 0; k < CMP_IPT; ++k) {
-    const u64 i = base + k;
-    if (i < m) {
-      ++valid;
-      if (static_cast<u32>(v_in[i] >> 32) <= pivot) {
-        lightmask |= 1u << k;
-        ++cl;
-      }
+    const u64 e = wbase + u64(k) * WAVE + lane;
+    bool light = false;
+    if (e < m) {
+      ss[k] = s_in[e];
+      dd[k] = d_in[e];
+      vv[k] = v_in[e];
+      light = static_cast<u32>(vv[k] >> 32) <= pivot;
+    }
+    bal[k] = __ballot(light);
+    cnt += static_cast<u32>(__popcll(bal[k]));
+  }
+  if (lane == 0) s_wave_light[w] = cnt;
+  __syncthreads();
+  if (w == 0) {
+    u64 agg = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) agg += s_wave_light[ww];
+    bool ok;
+    const u64 excl = lookback_exclusive(lb_status, tile, agg, err, ok);
+    if (lane == 0) {
+      s_excl = excl;
+      s_ok = ok ? 1u : 0u;
+      if (tile == tiles - 1) *light_dev = static_cast<u32>(excl + agg);
     }
   }
-  u32 tot_l, tot_v;
-  const u32 ex_l = block_exclusive_sum(cl, tot_l, lds);
-  const u32 ex_v = block_exclusive_sum(valid, tot_v, lds);
-  const u32 L = *light_total;
-  const u32 tile_l = light_off[blockIdx.x];
-  u32 pl = tile_l + ex_l;
-  u32 ph = L + static_cast<u32>(tile_base - tile_l) + (ex_v - ex_l);
+  __syncthreads();
+  if (!s_ok) return;
+  u64 lpos = s_excl;
+  u64 before = u64(w) * (WAVE * CMP_IPT);  // records of this tile before this wave
+  for (int ww = 0; ww < w; ++ww) lpos += s_wave_light[ww];
+  // heavy rank = records before - light records before
+  u64 hrank = (tile_base - s_excl) + (before - (lpos - s_excl));
 #pragma unroll
   for (int k = 0; k < CMP_IPT; ++k) {
-    const u64 i = base + k;
-    if (i < m) {
-      const u32 pos = (lightmask & (1u << k)) ? pl++ : ph++;
-      s_out[pos] = s_in[i];
-      d_out[pos] = d_in[i];
-      v_out[pos] = v_in[i];
+    const u64 e = wbase + u64(k) * WAVE + lane;
+    const u64 valid = __ballot(e < m);
+    if (e < m) {
+      const bool light = (bal[k] >> lane) & 1ull;
+      const u64 p = light ? lpos + mbcnt(bal[k]) : (m - 1) - (hrank + mbcnt(valid & ~bal[k]));
+      s_out[p] = ss[k];
+      d_out[p] = dd[k];
+      v_out[p] = vv[k];
     }
+    const u32 nl = static_cast<u32>(__popcll(bal[k]));
+    lpos += nl;
+    hrank += static_cast<u32>(__popcll(valid)) - nl;
   }
 }
 
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
-                       u32 pivot, u32* s_out, u32* d_out, u64* v_out, u32* tile_counts,
-                       u32* scan_scratch, u32* light_count_dev) {
+                       u32 pivot, u32* s_out, u32* d_out, u64* v_out, u64* lb,
+                       u32* light_count_dev, u32* err) {
   if (m == 0) {
     fill_u32(st, light_count_dev, 1, 0);
     return;
   }
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
-  {
-    KTScope t(st, K_SPLIT_COUNT, 8.0 * m);
-    split_count_kernel<<<tiles, BLOCK, 0, st>>>(v_in, m, pivot, tile_counts);
-    KMST_HIP(hipGetLastError());
-  }
-  scan_exclusive_u32(st, tile_counts, tile_counts, tiles, scan_scratch, light_count_dev);
+  fill_u64(st, lb, u64(tiles) + 1, 0);
   KTScope t(st, K_SPLIT_SCATTER, 32.0 * m);
-  split_scatter_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, pivot, tile_counts,
-                                                 light_count_dev, s_out, d_out, v_out);
+  split_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, pivot, s_out, d_out, v_out, lb,
+                                         reinterpret_cast<u32*>(lb + tiles), tiles,
+                                         light_count_dev, err);
   KMST_HIP(hipGetLastError());
 }
 

@@ -54,11 +54,13 @@ void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err);
 void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_flag,
                  const u32* new_id, u32* label);
 
-// K4+K5 relabel, drop self-loops and dead records, order-preserving compaction.
-// tile_counts needs ceil(m / CMP_TILE) entries; *count_dev gets the kept count.
+// K4+K5 relabel, drop self-loops and dead records, order-preserving compaction
+// in one pass.  lb needs lookback_elems(m) u64 entries; *count_dev gets the
+// kept count.
+inline u64 lookback_elems(u64 m) { return (m + CMP_TILE - 1) / CMP_TILE + 1; }
 void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in,
                      u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
-                     u32* tile_counts, u32* scan_scratch, u32* count_dev);
+                     u64* lb, u32* count_dev, u32* err);
 
 // K6 stable LSD radix sort by key (s << dbits) | d over total_bits bits.
 // Returns true when the sorted data ended in the *_alt arrays.  d / v (and
@@ -82,7 +84,7 @@ void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* paren
                          u32* rep_next);
 void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32* out);
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
-                       u32 pivot, u32* s_out, u32* d_out, u64* v_out, u32* tile_counts,
-                       u32* scan_scratch, u32* light_count_dev);
+                       u32 pivot, u32* s_out, u32* d_out, u64* v_out, u64* lb,
+                       u32* light_count_dev, u32* err);
 
 }  // namespace kmst

@@ -121,7 +121,7 @@ kmst_context::~kmst_context() {
 std::vector<DevBuf*> kmst_context::all_bufs() {
   return {&xs, &xd, &xv, &ys, &yd, &yv, &val0, &off, &best_val, &best_dst, &parent, &flag,
           &newid, &label, &rep, &rep_next, &gpar, &mst, &pieces, &tiles, &scan_scratch, &hist,
-          &counters, &in_s, &in_d, &in_w, &out_s, &out_d, &out_w, &samples, &gen_a, &gen_b,
+          &counters, &lookback, &in_s, &in_d, &in_w, &out_s, &out_d, &out_w, &samples, &gen_a, &gen_b,
           &gen_c, &gen_d};
 }
 
@@ -160,6 +160,7 @@ class Solver {
   u32 *best_dst = nullptr, *parent = nullptr, *flag = nullptr, *newid = nullptr, *label = nullptr;
   u32 *rep = nullptr, *rep_next = nullptr, *gpar = nullptr, *mst = nullptr;
   u32 *tiles = nullptr, *scan_scratch = nullptr, *hist = nullptr;
+  u64* lb = nullptr;
   ArgPiece* pieces = nullptr;
   u32* samples = nullptr;
 
@@ -197,6 +198,7 @@ void Solver::allocate() {
   const u64 arg_tiles = (m + ARG_TILE - 1) / ARG_TILE;
   pieces = static_cast<ArgPiece*>(C.pieces.get(2 * arg_tiles * sizeof(ArgPiece) + 64));
   tiles = static_cast<u32*>(C.tiles.get(((m + CMP_TILE - 1) / CMP_TILE + 1) * 4));
+  lb = static_cast<u64*>(C.lookback.get(lookback_elems(m) * 8));
   const u64 hist_elems = sort_hist_elems(std::max(m, n));
   hist = static_cast<u32*>(C.hist.get(hist_elems * 4));
   const u64 scan_max = std::max<u64>({hist_elems, n, (m + CMP_TILE - 1) / CMP_TILE + 1});
@@ -281,8 +283,8 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar) {
     }
     RecView out = (cur.s == A.s) ? B : A;
     {
-      relabel_compact(st, cur.s, cur.d, cur.v, m, label, out.s, out.d, out.v, tiles, scan_scratch,
-                      ctr + C_MNEXT);
+      relabel_compact(st, cur.s, cur.d, cur.v, m, label, out.s, out.d, out.v, lb, ctr + C_MNEXT,
+                      ctr + C_ERR);
     }
     check_errors("round");
     const u64 n_next = C.host_counters[C_NNEXT];
@@ -316,9 +318,9 @@ u32 Solver::pick_pivot(RecView r) {
 }
 
 u64 Solver::partition(RecView in, RecView out, u32 pivot) {
-  split_light_heavy(st, in.s, in.d, in.v, in.m, pivot, out.s, out.d, out.v, tiles, scan_scratch,
-                    ctr + C_LIGHT);
-  read_counters();
+  split_light_heavy(st, in.s, in.d, in.v, in.m, pivot, out.s, out.d, out.v, lb, ctr + C_LIGHT,
+                    ctr + C_ERR);
+  check_errors("split");
   return C.host_counters[C_LIGHT];
 }
 
@@ -346,8 +348,8 @@ void Solver::filter_rec(RecView P, RecView Q, bool stop, int depth) {
   if (H == 0) return;
   RecView heavy = Q.at(L, H), scratch = P.at(L, H);
   {
-    relabel_compact(st, heavy.s, heavy.d, heavy.v, H, gpar, scratch.s, scratch.d, scratch.v, tiles,
-                    scan_scratch, ctr + C_MNEXT);
+    relabel_compact(st, heavy.s, heavy.d, heavy.v, H, gpar, scratch.s, scratch.d, scratch.v, lb,
+                    ctr + C_MNEXT, ctr + C_ERR);
   }
   check_errors("filter");
   const u64 Hk = C.host_counters[C_MNEXT];
@@ -391,7 +393,7 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
         RecView heavy = Y.at(L, H), scratch = X.at(L, H);
         {
           relabel_compact(st, heavy.s, heavy.d, heavy.v, H, gpar, scratch.s, scratch.d, scratch.v,
-                          tiles, scan_scratch, ctr + C_MNEXT);
+                          lb, ctr + C_MNEXT, ctr + C_ERR);
         }
         check_errors("filter");
         const u64 Hk = C.host_counters[C_MNEXT];

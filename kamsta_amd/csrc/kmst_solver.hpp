@@ -57,7 +57,7 @@ struct kmst_context {
   // vertex arrays
   kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, mst;
   // kernel scratch
-  kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples;
+  kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback;
   // host-API staging
   kmst::DevBuf in_s, in_d, in_w, out_s, out_d, out_w;
   // generator scratch
