@@ -34,8 +34,8 @@ def parse():
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--algorithm", choices=["boruvka", "filter_boruvka", "best"], default="best")
     p.add_argument("--filter-threshold", type=int, default=4)
-    p.add_argument("--cpu-log-n", type=int, default=20)
-    p.add_argument("--cpu-log-m", type=int, default=24)
+    p.add_argument("--cpu-log-n", type=int, default=23)
+    p.add_argument("--cpu-log-m", type=int, default=27)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--verify-full", action="store_true",
                    help="also run the oracle's Kruskal on the full input (minutes of CPU)")

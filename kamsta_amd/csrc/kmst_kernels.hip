@@ -171,8 +171,44 @@ void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off) {
 
 // Strict key of every record: val = (w << 32) | uid, uid = position of the
 // (min, max) twin.  The reverse record (u, v) with u > v finds its twin in
-// v's segment by binary search; within runs of equal (src, dst) (multigraph
-// inputs) the k-th record of weight w pairs with the k-th twin of weight w.
+// v's segment: segments of up to KEY_LINEAR records are read with
+// independent 16-byte loads (one memory round trip) and the lower bound is a
+// count of smaller entries; longer segments are narrowed by binary search
+// first.  Within runs of equal (src, dst) (multigraph inputs) the k-th record
+// of weight w pairs with the k-th twin of weight w.
+constexpr u32 KEY_LINEAR = 64;
+
+template <bool VEC>
+__device__ __forceinline__ u64 lower_bound_dst(const u32* __restrict__ dst, u64 m, u64 lo, u64 hi,
+                                               u32 x) {
+  while (hi - lo > KEY_LINEAR) {
+    const u64 mid = (lo + hi) >> 1;
+    if (dst[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  // count entries < x in [lo, hi); chunks of 4 aligned to the array start,
+  // vector loads only for chunks that lie wholly inside the array
+  const u64 a0 = lo & ~u64{3};
+  u32 cnt = 0;
+#pragma unroll
+  for (u32 c = 0; c < (KEY_LINEAR / 4) + 1; ++c) {
+    const u64 base = a0 + 4ull * c;
+    if (base < hi) {
+      u32 q[4];
+      if (VEC && base + 3 < m) {
+        const uint4 v = *reinterpret_cast<const uint4*>(dst + base);
+        q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) q[t] = (base + t < hi) ? dst[base + t] : 0u;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) cnt += (base + t >= lo && base + t < hi && q[t] < x) ? 1u : 0u;
+    }
+  }
+  return lo + cnt;
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src,
                                                      const u32* __restrict__ dst,
                                                      const u32* __restrict__ w, u64 m,
@@ -190,13 +226,8 @@ __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src
     if (src[j - 1] != s || dst[j - 1] != d) break;
     k += (w[j - 1] == wi) ? 1u : 0u;
   }
-  u64 lo = off[d], hi = off[d + 1];
-  while (lo < hi) {
-    const u64 mid = (lo + hi) >> 1;
-    if (dst[mid] < s) lo = mid + 1; else hi = mid;
-  }
   const u64 end = off[d + 1];
-  for (u64 j = lo; j < end && dst[j] == s; ++j) {
+  for (u64 j = lower_bound_dst<VEC>(dst, m, off[d], end, s); j < end && dst[j] == s; ++j) {
     if (w[j] == wi) {
       if (k == 0) {
         val[i] = (u64(wi) << 32) | j;
@@ -213,7 +244,10 @@ void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u6
                 const u32* off, u64* val, u32* err) {
   if (m == 0) return;
   KTScope t(st, K_KEYS, 20.0 * m);
-  keys_kernel<<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, dst, w, m, off, val, err);
+  if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0)
+    keys_kernel<true><<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, dst, w, m, off, val, err);
+  else
+    keys_kernel<false><<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, dst, w, m, off, val, err);
   KMST_HIP(hipGetLastError());
 }
 
@@ -622,16 +656,30 @@ __global__ __launch_bounds__(BLOCK) void radix_hist_kernel(const u32* __restrict
   }
 }
 
-template <bool HAS_D, bool HAS_V>
+// Scatter pass: rank every item inside its wave (8-ballot digit matching, so
+// the pass is stable), place the tile in LDS in (digit, input order), then
+// write it out row by row so that each digit's run leaves as consecutive
+// addresses.  In the last pass the LDS tile is sorted by the full key, and
+// with DEDUP the first record of every run of equal (src, dst) inside the
+// tile keeps the run's lightest key while the others are marked dead
+// (EdgeProcessor::remove_duplicates, duplicate_detection.hpp:26-64; runs
+// split across tiles keep one survivor per tile, which later rounds remove).
+template <bool HAS_D, bool HAS_V, bool DEDUP>
 __global__ __launch_bounds__(BLOCK) void radix_scatter_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in,
     u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out, u64 m, u32 dbits,
     u32 shift, const u32* __restrict__ hist_scan, u32 tiles) {
   __shared__ u32 wcnt[WAVES][RADIX];
+  __shared__ u32 delta[RADIX];  // global position - tile position, per digit
+  __shared__ u32 lds_s[SORT_TILE];
+  __shared__ u32 lds_d[HAS_D ? SORT_TILE : 1];
+  __shared__ u64 lds_v[HAS_V ? SORT_TILE : 1];
+  __shared__ u32 scan_tmp[WAVES];
   for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&wcnt[0][0])[i] = 0;
   __syncthreads();
   const int lane = lane_id(), w = wave_id();
-  const u64 base = u64(blockIdx.x) * SORT_TILE + u64(w) * (WAVE * SORT_IPT);
+  const u64 tile_base = u64(blockIdx.x) * SORT_TILE;
+  const u64 base = tile_base + u64(w) * (WAVE * SORT_IPT);
   u32 ks[SORT_IPT], kd[SORT_IPT], dig[SORT_IPT], rank[SORT_IPT];
   u64 kv[SORT_IPT];
 #pragma unroll
@@ -641,6 +689,11 @@ __global__ __launch_bounds__(BLOCK) void radix_scatter_kernel(
     ks[k] = valid ? s_in[e] : 0u;
     kd[k] = (HAS_D && valid) ? d_in[e] : 0u;
     kv[k] = (HAS_V && valid) ? v_in[e] : 0ull;
+  }
+#pragma unroll
+  for (int k = 0; k < SORT_IPT; ++k) {
+    const u64 e = base + u64(k) * WAVE + lane;
+    const bool valid = e < m;
     const u64 key = HAS_D ? ((u64(ks[k]) << dbits) | kd[k]) : u64(ks[k]);
     const u32 digit = valid ? static_cast<u32>((key >> shift) & (RADIX - 1)) : u32(RADIX);
     u64 peers = __ballot(valid);
@@ -660,23 +713,57 @@ __global__ __launch_bounds__(BLOCK) void radix_scatter_kernel(
     }
   }
   __syncthreads();
-  for (int dg = threadIdx.x; dg < RADIX; dg += BLOCK) {
-    u32 run = hist_scan[u64(dg) * tiles + blockIdx.x];
+  {
+    // thread t owns digit t: tile-local start of the digit and per-wave offsets
+    const int dg = threadIdx.x;
+    u32 tot = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) tot += wcnt[ww][dg];
+    u32 all;
+    const u32 start = block_exclusive_sum(tot, all, scan_tmp);
+    u32 run = start;
 #pragma unroll
     for (int ww = 0; ww < WAVES; ++ww) {
       const u32 c = wcnt[ww][dg];
       wcnt[ww][dg] = run;
       run += c;
     }
+    delta[dg] = hist_scan[u64(dg) * tiles + blockIdx.x] - start;
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < SORT_IPT; ++k) {
     if (dig[k] < RADIX) {
-      const u32 pos = wcnt[w][dig[k]] + rank[k];
-      s_out[pos] = ks[k];
-      if (HAS_D) d_out[pos] = kd[k];
-      if (HAS_V) v_out[pos] = kv[k];
+      const u32 lp = wcnt[w][dig[k]] + rank[k];
+      lds_s[lp] = ks[k];
+      if (HAS_D) lds_d[lp] = kd[k];
+      if (HAS_V) lds_v[lp] = kv[k];
+    }
+  }
+  __syncthreads();
+  const u32 count = static_cast<u32>((m - tile_base) < SORT_TILE ? (m - tile_base) : SORT_TILE);
+  for (u32 j = threadIdx.x; j < count; j += BLOCK) {
+    const u32 sj = lds_s[j];
+    const u32 dj = HAS_D ? lds_d[j] : 0u;
+    const u64 key = HAS_D ? ((u64(sj) << dbits) | dj) : u64(sj);
+    const u32 digit = static_cast<u32>((key >> shift) & (RADIX - 1));
+    const u32 pos = delta[digit] + j;
+    s_out[pos] = sj;
+    if (HAS_D) d_out[pos] = dj;
+    if (HAS_V) {
+      u64 vj = lds_v[j];
+      if (DEDUP) {
+        const bool head = (j == 0) || lds_s[j - 1] != sj || lds_d[j - 1] != dj;
+        if (head) {
+          for (u32 t = j + 1; t < count && lds_s[t] == sj && lds_d[t] == dj; ++t) {
+            const u64 x = lds_v[t];
+            vj = x < vj ? x : vj;
+          }
+        } else {
+          vj = VAL_NONE;
+        }
+      }
+      v_out[pos] = vj;
     }
   }
 }
@@ -684,7 +771,7 @@ __global__ __launch_bounds__(BLOCK) void radix_scatter_kernel(
 u64 sort_hist_elems(u64 m) { return u64(RADIX) * ceil_div_u32(m ? m : 1, SORT_TILE); }
 
 bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, u64* v_alt, u64 m,
-                u32 dbits, u32 total_bits, u32* hist, u32* scan_scratch) {
+                u32 dbits, u32 total_bits, u32* hist, u32* scan_scratch, bool dedup_last) {
   if (m <= 1 || total_bits == 0) return false;
   const u32 tiles = ceil_div_u32(m, SORT_TILE);
   const u32 passes = (total_bits + RADIX_BITS - 1) / RADIX_BITS;
@@ -705,18 +792,23 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
     }
     scan_exclusive_u32(st, hist, hist, u64(RADIX) * tiles, scan_scratch, nullptr);
     KTScope t(st, K_SORT_SCATTER, 2.0 * m * (4.0 + (d ? 4.0 : 0.0) + (v ? 8.0 : 0.0)));
-    if (d && v)
-      radix_scatter_kernel<true, true><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m, dbits,
-                                                                shift, hist, tiles);
+    const bool dd = dedup_last && (p + 1 == passes) && d && v;
+    if (dd)
+      radix_scatter_kernel<true, true, true><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
+                                                                      dbits, shift, hist, tiles);
+    else if (d && v)
+      radix_scatter_kernel<true, true, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
+                                                                       dbits, shift, hist, tiles);
     else if (d)
-      radix_scatter_kernel<true, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m, dbits,
-                                                                 shift, hist, tiles);
+      radix_scatter_kernel<true, false, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
+                                                                        dbits, shift, hist, tiles);
     else if (v)
-      radix_scatter_kernel<false, true><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m, dbits,
-                                                                 shift, hist, tiles);
+      radix_scatter_kernel<false, true, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
+                                                                        dbits, shift, hist, tiles);
     else
-      radix_scatter_kernel<false, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
-                                                                  dbits, shift, hist, tiles);
+      radix_scatter_kernel<false, false, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo,
+                                                                         m, dbits, shift, hist,
+                                                                         tiles);
     KMST_HIP(hipGetLastError());
     alt = !alt;
   }
@@ -934,6 +1026,37 @@ void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u
   split_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, pivot, s_out, d_out, v_out, lb,
                                          reinterpret_cast<u32*>(lb + tiles), tiles,
                                          light_count_dev, err);
+  KMST_HIP(hipGetLastError());
+}
+
+// Present-vertex compaction before a sparse base case: flag every src, scan,
+// and keep compact id -> original id.
+__global__ __launch_bounds__(BLOCK) void mark_present_kernel(const u32* __restrict__ s, u64 m,
+                                                             u32* __restrict__ flag) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
+    const u32 v = s[i];
+    if (i == 0 || s[i - 1] != v) flag[v] = 1u;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void compact_rep_kernel(const u32* __restrict__ flag,
+                                                            const u32* __restrict__ new_id, u64 n,
+                                                            u32* __restrict__ rep) {
+  for (u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x; v < n; v += u64(gridDim.x) * BLOCK)
+    if (flag[v]) rep[new_id[v]] = static_cast<u32>(v);
+}
+
+void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag) {
+  if (m == 0) return;
+  KTScope t(st, K_OTHER, 8.0 * m);
+  mark_present_kernel<<<grid_for(m), BLOCK, 0, st>>>(s, m, flag);
+  KMST_HIP(hipGetLastError());
+}
+
+void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32* rep) {
+  if (n == 0) return;
+  KTScope t(st, K_OTHER, 8.0 * n);
+  compact_rep_kernel<<<grid_for(n), BLOCK, 0, st>>>(flag, new_id, n, rep);
   KMST_HIP(hipGetLastError());
 }
 

@@ -66,8 +66,11 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const u64
 // Returns true when the sorted data ended in the *_alt arrays.  d / v (and
 // their alt) may be null.  hist needs sort_hist_elems(m) entries.
 u64 sort_hist_elems(u64 m);
+// With dedup_last (needs d and v) the last pass also keeps only the lightest
+// record of each run of equal (s, d) inside a tile; the rest are marked dead.
 bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, u64* v_alt,
-                u64 m, u32 dbits, u32 total_bits, u32* hist, u32* scan_scratch);
+                u64 m, u32 dbits, u32 total_bits, u32* hist, u32* scan_scratch,
+                bool dedup_last = false);
 
 // K7 keep the lightest record of each (s, d) run (others marked dead).
 void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m);
@@ -86,5 +89,9 @@ void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32* out);
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
                        u32 pivot, u32* s_out, u32* d_out, u64* v_out, u64* lb,
                        u32* light_count_dev, u32* err);
+
+// Present-vertex compaction helpers (sorted records; flag must be zeroed).
+void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag);
+void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32* rep);
 
 }  // namespace kmst

@@ -75,7 +75,7 @@ const char* kernel_kind_name(int k) {
                                 "keys", "argmin", "argmin_fixup", "hook", "jump", "label",
                                 "gpar", "relabel_count", "relabel_scatter", "sort_hist",
                                 "sort_scatter", "dedup", "sample", "split_count",
-                                "split_scatter", "gather", "gen"};
+                                "split_scatter", "gather", "gen", "other"};
   return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -170,7 +170,10 @@ class Solver {
   void read_counters();
   // rounds of plain Boruvka on records sorted by (src, dst) with ids < n0;
   // A is writable scratch disjoint from recs, B writable and may alias recs.
-  void rounds(RecView recs, RecView A, RecView B, bool track_gpar);
+  void rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_start = 0,
+              bool rep_given = false);
+  // base case: compact sparse vertex ids first, then run the rounds
+  void base_case(RecView P, RecView Q);
   void filter_rec(RecView P, RecView Q, bool stop, int depth);
   u32 pick_pivot(RecView r);
   u64 partition(RecView in, RecView out, u32 pivot);
@@ -242,21 +245,22 @@ void Solver::setup() {
 
 RecView Solver::sort_dedup(RecView cur, RecView alt, u32 bits) {
   if (cur.m > 1 && bits > 0) {
+    // the last scatter pass keeps the lightest copy of each (src, dst) run
     if (radix_sort(st, cur.s, cur.d, cur.v, alt.s, alt.d, alt.v, cur.m, bits, 2 * bits, hist,
-                   scan_scratch))
+                   scan_scratch, /*dedup_last=*/true))
       cur = RecView{alt.s, alt.d, alt.v, cur.m};
-  }
-  {
+  } else {
     dedup_mark(st, cur.s, cur.d, cur.v, cur.m);
   }
   return cur;
 }
 
-void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar) {
+void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_start,
+                    bool rep_given) {
   RecView cur = recs;
   u64 m = recs.m;
-  u64 n = n0;  // the first round runs in the original id space
-  bool first = true;
+  u64 n = n_start ? n_start : n0;  // by default the first round runs in the original id space
+  bool first = !rep_given;
   while (m > 0 && n > 1) {
     RoundStat rs;
     rs.n = n;
@@ -305,6 +309,25 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar) {
   }
 }
 
+void Solver::base_case(RecView P, RecView Q) {
+  // a base case with few records touches few vertices: renumber them to
+  // [0, n_c) (monotone, so the records stay sorted) instead of running the
+  // first round over all n0 vertex slots
+  if (P.m * 2 < u64(n0)) {
+    fill_u32(st, flag, n0, 0);
+    mark_present(st, P.s, P.m, flag);
+    scan_exclusive_u32(st, flag, newid, n0, scan_scratch, ctr + C_NNEXT);
+    compact_rep(st, flag, newid, n0, rep);
+    relabel_compact(st, P.s, P.d, P.v, P.m, newid, Q.s, Q.d, Q.v, lb, ctr + C_MNEXT, ctr + C_ERR);
+    check_errors("compact");
+    const u64 nc = C.host_counters[C_NNEXT];
+    const u64 mc = C.host_counters[C_MNEXT];
+    rounds(Q.at(0, mc), P.at(0, mc), Q.at(0, mc), true, nc, true);
+    return;
+  }
+  rounds(P, Q, P, true);
+}
+
 u32 Solver::pick_pivot(RecView r) {
   const u32 k = static_cast<u32>(std::min<u64>(1023, r.m));
   sample_weights(st, r.v, r.m, k, samples);
@@ -333,7 +356,7 @@ void Solver::filter_rec(RecView P, RecView Q, bool stop, int depth) {
   if (avg_degree <= double(filter_threshold) || stop || P.m <= 1000 || depth > 40) {
     FilterStat fs{depth, P.m, 0, 0, true};
     C.stats.filter.push_back(fs);
-    rounds(P, Q, P, /*track_gpar=*/true);
+    base_case(P, Q);
     return;
   }
   const u32 pivot = pick_pivot(P);

@@ -231,3 +231,28 @@ def test_device_solve_and_stats(gpu, ctx):
         assert st["mst_edges"] == cnt and st["rounds"]
         assert st["kernels"]["argmin"]["launches"] >= 1
     ctx.set_timing(False)
+
+
+def test_device_solve_misaligned_inputs(gpu, ctx):
+    """Input arrays at a 4-byte offset (not 16-byte aligned) and an odd
+    record count exercise the scalar tail of the twin search."""
+    torch = gpu
+    und = graphs.random_undirected(3001, 20001, 254, 77)
+    rec = graphs.records(und)
+    exp, exp_w = oracle.kruskal(rec)
+    N = len(rec)
+    n = int(max(rec["src"].max(), rec["dst"].max())) + 1
+    bufs = []
+    for col in ("src", "dst", "weight"):
+        t = torch.zeros(N + 1, dtype=torch.int32, device="cuda")
+        t[1:] = torch.from_numpy(rec[col].astype(np.uint32).view(np.int32)).cuda()
+        bufs.append(t)
+    views = [b[1:] for b in bufs]
+    assert views[1].data_ptr() % 16 != 0
+    outs = [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3)]
+    for algo in (k.BORUVKA, k.FILTER_BORUVKA):
+        cnt, tw = ctx.solve_device(views[0], views[1], views[2], N, n, outs[0], outs[1], outs[2], n,
+                                   algorithm=algo, config=k.AlgorithmConfig(filter_threshold=1))
+        assert tw == exp_w and cnt == len(exp)
+        assert np.array_equal(host(outs[0], np.uint32)[:cnt], exp["src"])
+        assert np.array_equal(host(outs[1], np.uint32)[:cnt], exp["dst"])
