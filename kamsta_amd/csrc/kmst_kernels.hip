@@ -38,37 +38,39 @@ void fill_u32(hipStream_t st, u32* p, u64 n, u32 value) {
 // ============================================================== scan
 // Three-phase exclusive scan: per-tile reduce, scan of tile sums (recursive),
 // per-tile downsweep.  Each thread owns SCAN_IPT consecutive elements.
-__global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(const u32* __restrict__ in, u64 n,
-                                                            u32* __restrict__ partial) {
-  __shared__ u32 lds[WAVES];
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(const T* __restrict__ in, u64 n,
+                                                            T* __restrict__ partial) {
+  __shared__ T lds[WAVES];
   const u64 base = u64(blockIdx.x) * SCAN_TILE + u64(threadIdx.x) * SCAN_IPT;
-  u32 sum = 0;
+  T sum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_IPT; ++k) {
     const u64 i = base + k;
-    sum += (i < n) ? in[i] : 0u;
+    sum += (i < n) ? in[i] : T(0);
   }
-  u32 total;
+  T total;
   block_exclusive_sum(sum, total, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(BLOCK) void scan_down_kernel(const u32* in, u32* out, u64 n,
-                                                          const u32* __restrict__ partial,
-                                                          u32* total_dev) {
-  __shared__ u32 lds[WAVES];
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void scan_down_kernel(const T* in, T* out, u64 n,
+                                                          const T* __restrict__ partial,
+                                                          T* total_dev) {
+  __shared__ T lds[WAVES];
   const u64 base = u64(blockIdx.x) * SCAN_TILE + u64(threadIdx.x) * SCAN_IPT;
-  u32 v[SCAN_IPT];
-  u32 sum = 0;
+  T v[SCAN_IPT];
+  T sum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_IPT; ++k) {
     const u64 i = base + k;
-    v[k] = (i < n) ? in[i] : 0u;
+    v[k] = (i < n) ? in[i] : T(0);
     sum += v[k];
   }
-  u32 total;
-  u32 run = block_exclusive_sum(sum, total, lds);
-  const u32 tile_off = partial ? partial[blockIdx.x] : 0u;
+  T total;
+  T run = block_exclusive_sum(sum, total, lds);
+  const T tile_off = partial ? partial[blockIdx.x] : T(0);
   run += tile_off;
 #pragma unroll
   for (int k = 0; k < SCAN_IPT; ++k) {
@@ -89,61 +91,126 @@ u64 scan_scratch_elems(u64 n) {
   return s + 1;
 }
 
-void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scratch,
-                        u32* total_dev) {
+template <typename T>
+void scan_exclusive(hipStream_t st, const T* in, T* out, u64 n, T* scratch, T* total_dev) {
   if (n == 0) {
-    if (total_dev) fill_u32(st, total_dev, 1, 0);
+    if (total_dev) {
+      if (sizeof(T) == 8) fill_u64(st, reinterpret_cast<u64*>(total_dev), 1, 0);
+      else fill_u32(st, reinterpret_cast<u32*>(total_dev), 1, 0);
+    }
     return;
   }
   const u32 tiles = ceil_div_u32(n, SCAN_TILE);
   if (tiles == 1) {
-    KTScope t(st, K_SCAN_DOWN, 8.0 * n);
-    scan_down_kernel<<<1, BLOCK, 0, st>>>(in, out, n, nullptr, total_dev);
+    KTScope t(st, K_SCAN_DOWN, 2.0 * sizeof(T) * n);
+    scan_down_kernel<T><<<1, BLOCK, 0, st>>>(in, out, n, nullptr, total_dev);
     KMST_HIP(hipGetLastError());
     return;
   }
-  u32* partial = scratch;
+  T* partial = scratch;
   {
-    KTScope t(st, K_SCAN_REDUCE, 4.0 * n);
-    scan_reduce_kernel<<<tiles, BLOCK, 0, st>>>(in, n, partial);
+    KTScope t(st, K_SCAN_REDUCE, 1.0 * sizeof(T) * n);
+    scan_reduce_kernel<T><<<tiles, BLOCK, 0, st>>>(in, n, partial);
     KMST_HIP(hipGetLastError());
   }
-  scan_exclusive_u32(st, partial, partial, tiles, scratch + tiles, total_dev);
-  KTScope t(st, K_SCAN_DOWN, 8.0 * n);
-  scan_down_kernel<<<tiles, BLOCK, 0, st>>>(in, out, n, partial, nullptr);
+  scan_exclusive<T>(st, partial, partial, tiles, scratch + tiles, total_dev);
+  KTScope t(st, K_SCAN_DOWN, 2.0 * sizeof(T) * n);
+  scan_down_kernel<T><<<tiles, BLOCK, 0, st>>>(in, out, n, partial, nullptr);
   KMST_HIP(hipGetLastError());
+}
+
+void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scratch,
+                        u32* total_dev) {
+  scan_exclusive<u32>(st, in, out, n, scratch, total_dev);
+}
+void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scratch,
+                        u64* total_dev) {
+  scan_exclusive<u64>(st, in, out, n, scratch, total_dev);
 }
 
 // ============================================================== setup
 // Contract check of the reference's assumed input: sorted by (src, dst), no
-// self-loops, ids < n (include/util/utils.hpp:391-395).  Counts min<max
-// records so the host can check #forward == #reverse.
+// self-loops, ids < n (include/util/utils.hpp:391-395).  Also gathers the
+// largest weight (to size the packed key), the number of min<max records,
+// and hash sums of (min, max, w) over min<max and over max<min records:
+// the input is symmetric as a multiset iff both hash sums and counts agree
+// (up to a 2^-64 collision chance).
+__device__ __forceinline__ u64 sym_hash(u32 a, u32 b, u32 w) {
+  u64 x = (u64(a) << 32 | b) ^ (u64(w) * 0x9E3779B97F4A7C15ull);
+  x ^= x >> 33;
+  x *= 0xFF51AFD7ED558CCDull;
+  x ^= x >> 33;
+  x *= 0xC4CEB9FE1A85EC53ull;
+  x ^= x >> 33;
+  return x;
+}
+
 __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__ src,
-                                                         const u32* __restrict__ dst, u64 m,
-                                                         u64 n, u32* err, u32* fwd_count) {
-  u32 bad = 0;
-  u32 fwd = 0;
+                                                         const u32* __restrict__ dst,
+                                                         const u32* __restrict__ wt, u64 m,
+                                                         u64 n, u32* err, u32* fwd_count,
+                                                         u32* max_w, u64* hsum) {
+  u32 bad = 0, fwd = 0, mw = 0;
+  u64 hf = 0, hr = 0;
   for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
-    const u32 s = src[i], d = dst[i];
+    const u32 s = src[i], d = dst[i], w = wt[i];
     if (s >= n || d >= n || s == d) bad |= ERR_INVALID;
     if (i > 0) {
       const u32 ps = src[i - 1], pd = dst[i - 1];
       if (ps > s || (ps == s && pd > d)) bad |= ERR_UNSORTED;
     }
-    fwd += (s < d) ? 1u : 0u;
+    mw = w > mw ? w : mw;
+    if (s < d) {
+      ++fwd;
+      hf += sym_hash(s, d, w);
+    } else {
+      hr += sym_hash(d, s, w);
+    }
   }
-  // wave-reduce the forward count
 #pragma unroll
-  for (int o = WAVE / 2; o > 0; o >>= 1) fwd += __shfl_xor(fwd, o, WAVE);
-  if (lane_id() == 0 && fwd) atomicAdd(fwd_count, fwd);
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    fwd += __shfl_xor(fwd, o, WAVE);
+    const u32 om = __shfl_xor(mw, o, WAVE);
+    mw = om > mw ? om : mw;
+    hf += __shfl_xor(hf, o, WAVE);
+    hr += __shfl_xor(hr, o, WAVE);
+  }
+  if (lane_id() == 0) {
+    if (fwd) atomicAdd(fwd_count, fwd);
+    if (mw) atomicMax(max_w, mw);
+    if (hf) atomicAdd(reinterpret_cast<unsigned long long*>(hsum), static_cast<unsigned long long>(hf));
+    if (hr) atomicAdd(reinterpret_cast<unsigned long long*>(hsum + 1), static_cast<unsigned long long>(hr));
+  }
   if (bad) atomicOr(err, bad);
 }
 
-void validate_records(hipStream_t st, const u32* src, const u32* dst, u64 m, u64 n, u32* err,
-                      u32* fwd_count) {
+void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m, u64 n,
+                      u32* err, u32* fwd_count, u32* max_w, u64* hsum) {
   if (m == 0) return;
-  KTScope t(st, K_VALIDATE, 8.0 * m);
-  validate_kernel<<<grid_for(m, 16384), BLOCK, 0, st>>>(src, dst, m, n, err, fwd_count);
+  KTScope t(st, K_VALIDATE, 12.0 * m);
+  validate_kernel<<<grid_for(m, 16384), BLOCK, 0, st>>>(src, dst, w, m, n, err, fwd_count, max_w,
+                                                         hsum);
+  KMST_HIP(hipGetLastError());
+}
+
+// Packed strict key: (w << 2b) | (min << b) | max with b = bits(n); orders
+// exactly like the reference check's (weight, src, dst) on the min<max copy.
+__global__ __launch_bounds__(BLOCK) void keys_packed_kernel(const u32* __restrict__ src,
+                                                            const u32* __restrict__ dst,
+                                                            const u32* __restrict__ w, u64 m,
+                                                            u32 vbits, u64* __restrict__ val) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
+    const u32 s = src[i], d = dst[i];
+    const u64 lo = s < d ? s : d, hi = s < d ? d : s;
+    val[i] = (u64(w[i]) << (2 * vbits)) | (lo << vbits) | hi;
+  }
+}
+
+void build_keys_packed(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
+                       u32 vbits, u64* val) {
+  if (m == 0) return;
+  KTScope t(st, K_KEYS, 20.0 * m);
+  keys_packed_kernel<<<grid_for(m), BLOCK, 0, st>>>(src, dst, w, m, vbits, val);
   KMST_HIP(hipGetLastError());
 }
 
@@ -169,46 +236,11 @@ void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off) {
   KMST_HIP(hipGetLastError());
 }
 
-// Strict key of every record: val = (w << 32) | uid, uid = position of the
-// (min, max) twin.  The reverse record (u, v) with u > v finds its twin in
-// v's segment: segments of up to KEY_LINEAR records are read with
-// independent 16-byte loads (one memory round trip) and the lower bound is a
-// count of smaller entries; longer segments are narrowed by binary search
-// first.  Within runs of equal (src, dst) (multigraph inputs) the k-th record
-// of weight w pairs with the k-th twin of weight w.
-constexpr u32 KEY_LINEAR = 64;
-
-template <bool VEC>
-__device__ __forceinline__ u64 lower_bound_dst(const u32* __restrict__ dst, u64 m, u64 lo, u64 hi,
-                                               u32 x) {
-  while (hi - lo > KEY_LINEAR) {
-    const u64 mid = (lo + hi) >> 1;
-    if (dst[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  // count entries < x in [lo, hi); chunks of 4 aligned to the array start,
-  // vector loads only for chunks that lie wholly inside the array
-  const u64 a0 = lo & ~u64{3};
-  u32 cnt = 0;
-#pragma unroll
-  for (u32 c = 0; c < (KEY_LINEAR / 4) + 1; ++c) {
-    const u64 base = a0 + 4ull * c;
-    if (base < hi) {
-      u32 q[4];
-      if (VEC && base + 3 < m) {
-        const uint4 v = *reinterpret_cast<const uint4*>(dst + base);
-        q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
-      } else {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) q[t] = (base + t < hi) ? dst[base + t] : 0u;
-      }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) cnt += (base + t >= lo && base + t < hi && q[t] < x) ? 1u : 0u;
-    }
-  }
-  return lo + cnt;
-}
-
-template <bool VEC>
+// Positional strict key, used when the packed key does not fit 64 bits:
+// val = (w << 32) | uid, uid = position of the (min, max) twin.  The reverse
+// record (u, v) with u > v finds its twin by binary search in v's segment;
+// within runs of equal (src, dst) (multigraph inputs) the k-th record of
+// weight w pairs with the k-th twin of weight w.
 __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src,
                                                      const u32* __restrict__ dst,
                                                      const u32* __restrict__ w, u64 m,
@@ -226,8 +258,13 @@ __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src
     if (src[j - 1] != s || dst[j - 1] != d) break;
     k += (w[j - 1] == wi) ? 1u : 0u;
   }
-  const u64 end = off[d + 1];
-  for (u64 j = lower_bound_dst<VEC>(dst, m, off[d], end, s); j < end && dst[j] == s; ++j) {
+  u64 lo = off[d], hi = off[d + 1];
+  const u64 end = hi;
+  while (lo < hi) {
+    const u64 mid = (lo + hi) >> 1;
+    if (dst[mid] < s) lo = mid + 1; else hi = mid;
+  }
+  for (u64 j = lo; j < end && dst[j] == s; ++j) {
     if (w[j] == wi) {
       if (k == 0) {
         val[i] = (u64(wi) << 32) | j;
@@ -244,10 +281,7 @@ void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u6
                 const u32* off, u64* val, u32* err) {
   if (m == 0) return;
   KTScope t(st, K_KEYS, 20.0 * m);
-  if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0)
-    keys_kernel<true><<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, dst, w, m, off, val, err);
-  else
-    keys_kernel<false><<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, dst, w, m, off, val, err);
+  keys_kernel<<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, dst, w, m, off, val, err);
   KMST_HIP(hipGetLastError());
 }
 
@@ -440,44 +474,41 @@ void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const u64*
 // ============================================================== K2 hook
 // Each vertex hooks onto the other endpoint of its lightest edge; of a mutual
 // pair the lower endpoint stays root (gbbs_reimplementation.hpp:117-127).
-// Non-roots append their edge's uid to the MST.
+// state[v] = (root with an edge ? 1 : 0) | (hooked, i.e. emits its edge to
+// the MST ? 1 << 32 : 0); one 64-bit exclusive scan of state then gives both
+// the consecutive new id of every root and the MST slot of every emitter,
+// without atomics.
+constexpr u64 ST_ROOT = 1ull;
+constexpr u64 ST_EMIT = 1ull << 32;
+
 __global__ __launch_bounds__(BLOCK) void hook_kernel(u64 n, const u64* __restrict__ best_val,
                                                      const u32* __restrict__ best_dst,
                                                      u32* __restrict__ parent,
-                                                     u32* __restrict__ root_flag,
-                                                     u32* __restrict__ mst, u32* mst_count) {
+                                                     u64* __restrict__ state) {
   const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
-  bool emit = false;
-  u32 uid = 0;
-  if (v < n) {
-    const u64 bv = best_val[v];
-    if (bv == VAL_NONE) {
-      parent[v] = static_cast<u32>(v);
-      root_flag[v] = 0;
-    } else {
-      const u32 u = best_dst[v];
-      const bool mutual = best_val[u] == bv;
-      if (mutual && v < u) {
-        parent[v] = static_cast<u32>(v);
-        root_flag[v] = 1;
-      } else {
-        parent[v] = u;
-        root_flag[v] = 0;
-        emit = true;
-        uid = static_cast<u32>(bv);
-      }
-    }
+  if (v >= n) return;
+  const u64 bv = best_val[v];
+  if (bv == VAL_NONE) {
+    parent[v] = static_cast<u32>(v);
+    state[v] = 0;
+    return;
   }
-  const u32 slot = wave_append(emit, mst_count);
-  if (emit) mst[slot] = uid;
+  const u32 u = best_dst[v];
+  const bool mutual = best_val[u] == bv;
+  if (mutual && v < u) {
+    parent[v] = static_cast<u32>(v);
+    state[v] = ST_ROOT;
+  } else {
+    parent[v] = u;
+    state[v] = ST_EMIT;
+  }
 }
 
 void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* parent,
-          u32* root_flag, u32* mst, u32* mst_count) {
+          u64* state) {
   if (n == 0) return;
-  KTScope t(st, K_HOOK, 28.0 * n);
-  hook_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, best_val, best_dst, parent, root_flag,
-                                                         mst, mst_count);
+  KTScope t(st, K_HOOK, 32.0 * n);
+  hook_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, best_val, best_dst, parent, state);
   KMST_HIP(hipGetLastError());
 }
 
@@ -512,22 +543,27 @@ void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err) {
   KMST_HIP(hipGetLastError());
 }
 
-// label[v] = new consecutive id of v's root, NONE32 for finished vertices.
+// label[v] = new consecutive id of v's root (NONE32 for finished vertices);
+// emitters write their edge key into the MST list at mst_base + their slot.
 __global__ __launch_bounds__(BLOCK) void label_kernel(u64 n, const u32* __restrict__ parent,
-                                                      const u32* __restrict__ root_flag,
-                                                      const u32* __restrict__ new_id,
-                                                      u32* __restrict__ label) {
+                                                      const u64* __restrict__ state,
+                                                      const u64* __restrict__ pos,
+                                                      const u64* __restrict__ best_val,
+                                                      u32* __restrict__ label,
+                                                      u64* __restrict__ mst, u64 mst_base) {
   const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
   if (v >= n) return;
   const u32 r = parent[v];
-  label[v] = root_flag[r] ? new_id[r] : NONE32;
+  label[v] = (state[r] & ST_ROOT) ? static_cast<u32>(pos[r]) : NONE32;
+  if (state[v] & ST_EMIT) mst[mst_base + (pos[v] >> 32)] = best_val[v];
 }
 
-void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_flag, const u32* new_id,
-                 u32* label) {
+void make_labels(hipStream_t st, u64 n, const u32* parent, const u64* state, const u64* pos,
+                 const u64* best_val, u32* label, u64* mst, u64 mst_base) {
   if (n == 0) return;
-  KTScope t(st, K_LABEL, 16.0 * n);
-  label_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, parent, root_flag, new_id, label);
+  KTScope t(st, K_LABEL, 28.0 * n);
+  label_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, parent, state, pos, best_val, label,
+                                                          mst, mst_base);
   KMST_HIP(hipGetLastError());
 }
 
@@ -848,34 +884,67 @@ void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m) {
   KMST_HIP(hipGetLastError());
 }
 
-// ============================================================== K9 gather
-__global__ __launch_bounds__(BLOCK) void gather_kernel(const u32* __restrict__ uids, u64 count,
-                                                       const u32* __restrict__ in_src,
-                                                       const u32* __restrict__ in_dst,
-                                                       const u32* __restrict__ in_w,
-                                                       u32* __restrict__ out_s,
-                                                       u32* __restrict__ out_d,
-                                                       u32* __restrict__ out_w, u64* total) {
+// ============================================================== K9 output
+// MST keys -> canonical (min, max, w): packed keys decode directly,
+// positional keys gather the twin record (GetMstEdge, misc.hpp:145-206).
+__global__ __launch_bounds__(BLOCK) void mst_triples_kernel(const u64* __restrict__ keys, u64 count,
+                                                            u32 vbits, bool packed,
+                                                            const u32* __restrict__ in_src,
+                                                            const u32* __restrict__ in_dst,
+                                                            const u32* __restrict__ in_w,
+                                                            u32* __restrict__ a, u32* __restrict__ b,
+                                                            u64* __restrict__ w) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < count; i += u64(gridDim.x) * BLOCK) {
+    const u64 key = keys[i];
+    if (packed) {
+      const u64 mask = (u64{1} << vbits) - 1;
+      a[i] = static_cast<u32>((key >> vbits) & mask);
+      b[i] = static_cast<u32>(key & mask);
+      w[i] = key >> (2 * vbits);
+    } else {
+      const u32 uid = static_cast<u32>(key);
+      const u32 x = in_src[uid], y = in_dst[uid];
+      a[i] = x < y ? x : y;
+      b[i] = x < y ? y : x;
+      w[i] = in_w[uid];
+    }
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void mst_output_kernel(const u32* __restrict__ a,
+                                                           const u32* __restrict__ b,
+                                                           const u64* __restrict__ w, u64 count,
+                                                           u32* __restrict__ out_s,
+                                                           u32* __restrict__ out_d,
+                                                           u32* __restrict__ out_w, u64* total) {
   u64 sum = 0;
   for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < count; i += u64(gridDim.x) * BLOCK) {
-    const u32 uid = uids[i];
-    const u32 a = in_src[uid], b = in_dst[uid], w = in_w[uid];
-    out_s[i] = a < b ? a : b;
-    out_d[i] = a < b ? b : a;
-    out_w[i] = w;
-    sum += w;
+    out_s[i] = a[i];
+    out_d[i] = b[i];
+    const u32 x = static_cast<u32>(w[i]);
+    out_w[i] = x;
+    sum += x;
   }
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, WAVE);
   if (lane_id() == 0 && sum) atomicAdd(reinterpret_cast<unsigned long long*>(total), sum);
 }
 
-void gather_mst(hipStream_t st, const u32* uids, u64 count, const u32* in_src, const u32* in_dst,
-                const u32* in_w, u32* out_s, u32* out_d, u32* out_w, u64* total_dev) {
+void mst_triples(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
+                 const u32* in_src, const u32* in_dst, const u32* in_w, u32* a, u32* b, u64* w) {
+  if (count == 0) return;
+  KTScope t(st, K_GATHER, packed ? 24.0 * count : 28.0 * count);
+  mst_triples_kernel<<<grid_for(count, 4096), BLOCK, 0, st>>>(keys, count, vbits, packed, in_src,
+                                                              in_dst, in_w, a, b, w);
+  KMST_HIP(hipGetLastError());
+}
+
+void mst_output(hipStream_t st, const u32* a, const u32* b, const u64* w, u64 count, u32* out_s,
+                u32* out_d, u32* out_w, u64* total_dev) {
   if (count == 0) return;
   KTScope t(st, K_GATHER, 28.0 * count);
-  gather_kernel<<<grid_for(count, 4096), BLOCK, 0, st>>>(uids, count, in_src, in_dst, in_w, out_s,
-                                                         out_d, out_w, total_dev);
+  mst_output_kernel<<<grid_for(count, 4096), BLOCK, 0, st>>>(a, b, w, count, out_s, out_d, out_w,
+                                                             total_dev);
   KMST_HIP(hipGetLastError());
 }
 
@@ -899,46 +968,44 @@ void iota_u32(hipStream_t st, u32* p, u64 n) {
 // its new id.
 __global__ __launch_bounds__(BLOCK) void gpar_kernel(u64 n, const u32* __restrict__ rep,
                                                      const u32* __restrict__ parent,
-                                                     const u32* __restrict__ root_flag,
-                                                     const u32* __restrict__ new_id,
-                                                     const u64* __restrict__ best_val,
+                                                     const u64* __restrict__ state,
+                                                     const u64* __restrict__ pos,
                                                      u32* __restrict__ gpar,
                                                      u32* __restrict__ rep_next) {
   const u64 c = u64(blockIdx.x) * BLOCK + threadIdx.x;
   if (c >= n) return;
+  const u64 sc = state[c];
+  if (sc == 0) return;  // no edge this round: keeps its parent-array entry
   const u32 name = rep ? rep[c] : static_cast<u32>(c);
-  if (best_val[c] != VAL_NONE) {
-    const u32 p = parent[c];
-    gpar[name] = rep ? rep[p] : p;
-  }
-  if (root_flag[c]) rep_next[new_id[c]] = name;
+  const u32 p = parent[c];
+  gpar[name] = rep ? rep[p] : p;
+  if (sc & ST_ROOT) rep_next[static_cast<u32>(pos[c])] = name;
 }
 
 void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* parent,
-                         const u32* root_flag, const u32* new_id, const u64* best_val, u32* gpar,
-                         u32* rep_next) {
+                         const u64* state, const u64* pos, u32* gpar, u32* rep_next) {
   if (n == 0) return;
   KTScope t(st, K_GPAR, 32.0 * n);
-  gpar_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, rep, parent, root_flag, new_id,
-                                                         best_val, gpar, rep_next);
+  gpar_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, rep, parent, state, pos, gpar,
+                                                         rep_next);
   KMST_HIP(hipGetLastError());
 }
 
 // k evenly spaced weight samples (NONE32 for dead records); the pivot only
 // changes speed, never the result.
 __global__ __launch_bounds__(BLOCK) void sample_kernel(const u64* __restrict__ v, u64 m, u32 k,
-                                                       u32* __restrict__ out) {
+                                                       u32 wshift, u32* __restrict__ out) {
   const u32 j = blockIdx.x * BLOCK + threadIdx.x;
   if (j >= k) return;
   const u64 i = (u64(j) * m) / k + m / (2ull * k);
   const u64 x = v[i < m ? i : m - 1];
-  out[j] = (x == VAL_NONE) ? NONE32 : static_cast<u32>(x >> 32);
+  out[j] = (x == VAL_NONE) ? NONE32 : static_cast<u32>(x >> wshift);
 }
 
-void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32* out) {
+void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32 wshift, u32* out) {
   if (k == 0 || m == 0) return;
   KTScope t(st, K_SAMPLE, 12.0 * k);
-  sample_kernel<<<ceil_div_u32(k, BLOCK), BLOCK, 0, st>>>(v, m, k, out);
+  sample_kernel<<<ceil_div_u32(k, BLOCK), BLOCK, 0, st>>>(v, m, k, wshift, out);
   KMST_HIP(hipGetLastError());
 }
 
@@ -949,8 +1016,9 @@ void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32* out) {
 // over the light counts; heavy offsets follow from them.
 __global__ __launch_bounds__(BLOCK) void split_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in, u64 m,
-    u32 pivot, u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out,
-    u64* lb_status, u32* lb_counter, u32 tiles, u32* light_dev, u32* err) {
+    u32 pivot, u32 wshift, u32* __restrict__ s_out, u32* __restrict__ d_out,
+    u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles, u32* light_dev,
+    u32* err) {
   __shared__ u32 s_wave_light[WAVES];
   __shared__ u64 s_excl;
   __shared__ u32 s_ok;
@@ -970,7 +1038,8 @@ __global__ __launch_bounds__(BLOCK) void split_kernel(
       ss[k] = s_in[e];
       dd[k] = d_in[e];
       vv[k] = v_in[e];
-      light = static_cast<u32>(vv[k] >> 32) <= pivot;
+      // dead records (VAL_NONE) always count as heavy
+      light = vv[k] != VAL_NONE && static_cast<u32>(vv[k] >> wshift) <= pivot;
     }
     bal[k] = __ballot(light);
     cnt += static_cast<u32>(__popcll(bal[k]));
@@ -1014,7 +1083,7 @@ __global__ __launch_bounds__(BLOCK) void split_kernel(
 }
 
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
-                       u32 pivot, u32* s_out, u32* d_out, u64* v_out, u64* lb,
+                       u32 pivot, u32 wshift, u32* s_out, u32* d_out, u64* v_out, u64* lb,
                        u32* light_count_dev, u32* err) {
   if (m == 0) {
     fill_u32(st, light_count_dev, 1, 0);
@@ -1023,7 +1092,7 @@ void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
   fill_u64(st, lb, u64(tiles) + 1, 0);
   KTScope t(st, K_SPLIT_SCATTER, 32.0 * m);
-  split_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, pivot, s_out, d_out, v_out, lb,
+  split_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, pivot, wshift, s_out, d_out, v_out, lb,
                                          reinterpret_cast<u32*>(lb + tiles), tiles,
                                          light_count_dev, err);
   KMST_HIP(hipGetLastError());

@@ -34,25 +34,31 @@ void fill_u32(hipStream_t st, u32* p, u64 n, u32 value);
 u64 scan_scratch_elems(u64 n);
 void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scratch,
                         u32* total_dev);
+void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scratch,
+                        u64* total_dev);
 
-// Setup: contract checks, CSR offsets, strict keys.
-void validate_records(hipStream_t st, const u32* src, const u32* dst, u64 m, u64 n,
-                      u32* err, u32* fwd_count);
+// Setup: contract checks (hsum[0..1] = forward / reverse multiset hashes),
+// CSR offsets, strict keys (packed when they fit 64 bits, positional else).
+void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
+                      u64 n, u32* err, u32* fwd_count, u32* max_w, u64* hsum);
 void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off);
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                 const u32* off, u64* val, u32* err);
+void build_keys_packed(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
+                       u32 vbits, u64* val);
 
 // K1 segmented argmin over records grouped by src.
 void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const u64* val,
                       u64 m, u64 n, u64* best_val, u32* best_dst, ArgPiece* pieces,
                       u32* piece_count);
 
-// K2 hook; K3 pointer jumping; vertex renumbering label pass.
+// K2 hook (state[v]: bit 0 = root with an edge, bit 32 = emits its edge);
+// K3 pointer jumping; renumbering + MST emission from the scanned state.
 void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* parent,
-          u32* root_flag, u32* mst, u32* mst_count);
+          u64* state);
 void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err);
-void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_flag,
-                 const u32* new_id, u32* label);
+void make_labels(hipStream_t st, u64 n, const u32* parent, const u64* state, const u64* pos,
+                 const u64* best_val, u32* label, u64* mst, u64 mst_base);
 
 // K4+K5 relabel, drop self-loops and dead records, order-preserving compaction
 // in one pass.  lb needs lookback_elems(m) u64 entries; *count_dev gets the
@@ -75,19 +81,20 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
 // K7 keep the lightest record of each (s, d) run (others marked dead).
 void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m);
 
-// K9 canonical MST gather + u64 weight.
-void gather_mst(hipStream_t st, const u32* uids, u64 count, const u32* in_src,
-                const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d, u32* out_w,
-                u64* total_dev);
+// K9 MST keys -> (min, max, w) triples; final copy-out + u64 weight.
+void mst_triples(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
+                 const u32* in_src, const u32* in_dst, const u32* in_w, u32* a, u32* b, u64* w);
+void mst_output(hipStream_t st, const u32* a, const u32* b, const u64* w, u64 count, u32* out_s,
+                u32* out_d, u32* out_w, u64* total_dev);
 
 // Filter-Boruvka helpers.
 void iota_u32(hipStream_t st, u32* p, u64 n);
 void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* parent,
-                         const u32* root_flag, const u32* new_id, const u64* best_val, u32* gpar,
-                         u32* rep_next);
-void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32* out);
+                         const u64* state, const u64* pos, u32* gpar, u32* rep_next);
+// weight of a key = key >> wshift (32 for positional keys, 2 bits(n) for packed)
+void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32 wshift, u32* out);
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
-                       u32 pivot, u32* s_out, u32* d_out, u64* v_out, u64* lb,
+                       u32 pivot, u32 wshift, u32* s_out, u32* d_out, u64* v_out, u64* lb,
                        u32* light_count_dev, u32* err);
 
 // Present-vertex compaction helpers (sorted records; flag must be zeroed).

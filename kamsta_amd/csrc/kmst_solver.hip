@@ -127,8 +127,11 @@ std::vector<DevBuf*> kmst_context::all_bufs() {
 
 namespace kmst {
 
-// Device counter slots (u32 words; the u64 total lives at word 16).
-enum Ctr : int { C_ERR = 0, C_FWD = 1, C_MST = 2, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6 };
+// Device counter slots (u32 words; u64 values at even word offsets >= 16).
+enum Ctr : int {
+  C_ERR = 0, C_FWD = 1, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6, C_MAXW = 7,
+  C_TOTAL64 = 16, C_HSUM64 = 18, C_ROUND64 = 22
+};
 
 struct RecView {
   u32* s = nullptr;
@@ -154,12 +157,18 @@ class Solver {
   u32 n0;
   u32* ctr = nullptr;
   u64 filter_threshold = 4;
+  bool packed = true;   // strict key packed as (w, min, max), else (w, uid)
+  u32 vbits = 0;
   RecView X, Y;
   u64* val0 = nullptr;
   u64* best_val = nullptr;
-  u32 *best_dst = nullptr, *parent = nullptr, *flag = nullptr, *newid = nullptr, *label = nullptr;
-  u32 *rep = nullptr, *rep_next = nullptr, *gpar = nullptr, *mst = nullptr;
-  u32 *tiles = nullptr, *scan_scratch = nullptr, *hist = nullptr;
+  u32 *best_dst = nullptr, *parent = nullptr, *label = nullptr;
+  u64 *state = nullptr, *pos = nullptr;  // per-round vertex state and its scan
+  u32 *flag = nullptr, *newid = nullptr;  // u32 views of the same memory (compaction)
+  u32 *rep = nullptr, *rep_next = nullptr, *gpar = nullptr;
+  u64* mst = nullptr;
+  u64 mst_count = 0;
+  u32 *scan_scratch = nullptr, *hist = nullptr;
   u64* lb = nullptr;
   ArgPiece* pieces = nullptr;
   u32* samples = nullptr;
@@ -168,13 +177,20 @@ class Solver {
   void setup();
   void check_errors(const char* where);
   void read_counters();
-  // rounds of plain Boruvka on records sorted by (src, dst) with ids < n0;
-  // A is writable scratch disjoint from recs, B writable and may alias recs.
+  u64 counter64(int word) const {
+    u64 x;
+    std::memcpy(&x, C.host_counters + word, 8);
+    return x;
+  }
+  // rounds of plain Boruvka on records sorted by (src, dst) with ids < n_start
+  // (default n0); A is writable scratch disjoint from recs, B writable and may
+  // alias recs.  With track_gpar every round updates the parent array.
   void rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_start = 0,
               bool rep_given = false);
   // base case: compact sparse vertex ids first, then run the rounds
   void base_case(RecView P, RecView Q);
   void filter_rec(RecView P, RecView Q, bool stop, int depth);
+  void filter_heavy(RecView heavy, RecView scratch, int depth);
   u32 pick_pivot(RecView r);
   u64 partition(RecView in, RecView out, u32 pivot);
   RecView sort_dedup(RecView cur, RecView alt, u32 bits);
@@ -191,21 +207,23 @@ void Solver::allocate() {
   best_val = static_cast<u64*>(C.best_val.get(n * 8));
   best_dst = static_cast<u32*>(C.best_dst.get(n * 4));
   parent = static_cast<u32*>(C.parent.get(n * 4));
-  flag = static_cast<u32*>(C.flag.get(n * 4));
-  newid = static_cast<u32*>(C.newid.get(n * 4));
+  state = static_cast<u64*>(C.flag.get(n * 8));
+  pos = static_cast<u64*>(C.newid.get(n * 8));
+  flag = reinterpret_cast<u32*>(state);
+  newid = reinterpret_cast<u32*>(pos);
   label = static_cast<u32*>(C.label.get(n * 4));
   rep = static_cast<u32*>(C.rep.get(n * 4));
   rep_next = static_cast<u32*>(C.rep_next.get(n * 4));
   gpar = static_cast<u32*>(C.gpar.get(n * 4));
-  mst = static_cast<u32*>(C.mst.get(n * 4));
+  mst = static_cast<u64*>(C.mst.get(n * 8));
   const u64 arg_tiles = (m + ARG_TILE - 1) / ARG_TILE;
   pieces = static_cast<ArgPiece*>(C.pieces.get(2 * arg_tiles * sizeof(ArgPiece) + 64));
-  tiles = static_cast<u32*>(C.tiles.get(((m + CMP_TILE - 1) / CMP_TILE + 1) * 4));
   lb = static_cast<u64*>(C.lookback.get(lookback_elems(m) * 8));
   const u64 hist_elems = sort_hist_elems(std::max(m, n));
   hist = static_cast<u32*>(C.hist.get(hist_elems * 4));
   const u64 scan_max = std::max<u64>({hist_elems, n, (m + CMP_TILE - 1) / CMP_TILE + 1});
-  scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 4 + 64));
+  // u64 scans reuse the same scratch, so size it in u64 units
+  scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 8 + 64));
   ctr = static_cast<u32*>(C.counters.get(256));
   samples = static_cast<u32*>(C.samples.get(4096 * 4));
 }
@@ -228,19 +246,22 @@ void Solver::check_errors(const char* where) {
 
 void Solver::setup() {
   fill_u32(st, ctr, 64, 0);
-  {
-    validate_records(st, in_s, in_d, m0, n0, ctr + C_ERR, ctr + C_FWD);
-  }
+  validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW,
+                   reinterpret_cast<u64*>(ctr + C_HSUM64));
   check_errors("input");
-  if (u64(C.host_counters[C_FWD]) * 2 != m0)
-    throw Error(KMST_EASYMMETRIC, "input: #(src<dst) records != #(src>dst) records");
-  {
+  if (u64(C.host_counters[C_FWD]) * 2 != m0 || counter64(C_HSUM64) != counter64(C_HSUM64 + 2))
+    throw Error(KMST_EASYMMETRIC, "input: records are not symmetric (src,dst,w) <-> (dst,src,w)");
+  vbits = bits_for(n0);
+  const u32 wbits = bits_for(u64(C.host_counters[C_MAXW]) + 1);
+  packed = wbits + 2 * vbits <= 64;
+  C.stats.packed_keys = packed;
+  if (packed) {
+    build_keys_packed(st, in_s, in_d, in_w, m0, vbits, val0);
+  } else {
     build_offsets(st, in_s, m0, n0, label /* reused as offsets, n0+1 */);
-  }
-  {
     build_keys(st, in_s, in_d, in_w, m0, label, val0, ctr + C_ERR);
+    check_errors("input");
   }
-  check_errors("input");
 }
 
 RecView Solver::sort_dedup(RecView cur, RecView alt, u32 bits) {
@@ -265,33 +286,23 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     RoundStat rs;
     rs.n = n;
     rs.m = m;
-    {
-      segmented_argmin(st, cur.s, cur.d, cur.v, m, n, best_val, best_dst, pieces, ctr + C_PIECES);
-    }
-    {
-      hook(st, n, best_val, best_dst, parent, flag, mst, ctr + C_MST);
-    }
-    {
-      pointer_jump(st, n, parent, ctr + C_ERR);
-    }
-    {
-      scan_exclusive_u32(st, flag, newid, n, scan_scratch, ctr + C_NNEXT);
-    }
-    {
-      make_labels(st, n, parent, flag, newid, label);
-    }
+    segmented_argmin(st, cur.s, cur.d, cur.v, m, n, best_val, best_dst, pieces, ctr + C_PIECES);
+    hook(st, n, best_val, best_dst, parent, state);
+    pointer_jump(st, n, parent, ctr + C_ERR);
+    scan_exclusive_u64(st, state, pos, n, reinterpret_cast<u64*>(scan_scratch),
+                       reinterpret_cast<u64*>(ctr + C_ROUND64));
+    make_labels(st, n, parent, state, pos, best_val, label, mst, mst_count);
     if (track_gpar) {
-      update_parent_array(st, n, first ? nullptr : rep, parent, flag, newid, best_val, gpar,
-                          rep_next);
+      update_parent_array(st, n, first ? nullptr : rep, parent, state, pos, gpar, rep_next);
       std::swap(rep, rep_next);
     }
     RecView out = (cur.s == A.s) ? B : A;
-    {
-      relabel_compact(st, cur.s, cur.d, cur.v, m, label, out.s, out.d, out.v, lb, ctr + C_MNEXT,
-                      ctr + C_ERR);
-    }
+    relabel_compact(st, cur.s, cur.d, cur.v, m, label, out.s, out.d, out.v, lb, ctr + C_MNEXT,
+                    ctr + C_ERR);
     check_errors("round");
-    const u64 n_next = C.host_counters[C_NNEXT];
+    const u64 round_total = counter64(C_ROUND64);
+    const u64 n_next = round_total & 0xFFFFFFFFull;
+    mst_count += round_total >> 32;
     const u64 m_next = C.host_counters[C_MNEXT];
     rs.n_next = n_next;
     rs.m_next = m_next;
@@ -304,9 +315,7 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     cur = sort_dedup(cur, alt, bits_for(n));
     first = false;
   }
-  if (track_gpar) {
-    pointer_jump(st, n0, gpar, ctr + C_ERR);
-  }
+  if (track_gpar) pointer_jump(st, n0, gpar, ctr + C_ERR);
 }
 
 void Solver::base_case(RecView P, RecView Q) {
@@ -330,7 +339,7 @@ void Solver::base_case(RecView P, RecView Q) {
 
 u32 Solver::pick_pivot(RecView r) {
   const u32 k = static_cast<u32>(std::min<u64>(1023, r.m));
-  sample_weights(st, r.v, r.m, k, samples);
+  sample_weights(st, r.v, r.m, k, packed ? 2 * vbits : 32, samples);
   std::vector<u32> h(k);
   KMST_HIP(hipMemcpyAsync(h.data(), samples, k * 4, hipMemcpyDeviceToHost, st));
   KMST_HIP(hipStreamSynchronize(st));
@@ -341,10 +350,27 @@ u32 Solver::pick_pivot(RecView r) {
 }
 
 u64 Solver::partition(RecView in, RecView out, u32 pivot) {
-  split_light_heavy(st, in.s, in.d, in.v, in.m, pivot, out.s, out.d, out.v, lb, ctr + C_LIGHT,
-                    ctr + C_ERR);
+  split_light_heavy(st, in.s, in.d, in.v, in.m, pivot, packed ? 2 * vbits : 32, out.s, out.d,
+                    out.v, lb, ctr + C_LIGHT, ctr + C_ERR);
   check_errors("split");
   return C.host_counters[C_LIGHT];
+}
+
+// Heavy half of one recursion level: relabel by the parent array, drop
+// same-component edges, sort + keep the lightest copies, recurse
+// (boruvka_recursive_setup, hybrid_boruvka_edgefilter.hpp:173-225).
+void Solver::filter_heavy(RecView heavy, RecView scratch, int depth) {
+  if (heavy.m == 0) return;
+  relabel_compact(st, heavy.s, heavy.d, heavy.v, heavy.m, gpar, scratch.s, scratch.d, scratch.v,
+                  lb, ctr + C_MNEXT, ctr + C_ERR);
+  check_errors("filter");
+  const u64 Hk = C.host_counters[C_MNEXT];
+  C.stats.filter.back().heavy_kept = Hk;
+  if (Hk == 0) return;
+  RecView kept = scratch.at(0, Hk);
+  RecView sorted = sort_dedup(kept, heavy.at(0, Hk), vbits);
+  RecView other = (sorted.s == kept.s) ? heavy.at(0, Hk) : kept;
+  filter_rec(sorted, other, false, depth + 1);
 }
 
 // boruvka_recursive (include/algorithms/hybrid_boruvka_edgefilter.hpp:227-330)
@@ -354,34 +380,21 @@ void Solver::filter_rec(RecView P, RecView Q, bool stop, int depth) {
   if (P.m == 0) return;
   const double avg_degree = double(P.m) / double(n0);
   if (avg_degree <= double(filter_threshold) || stop || P.m <= 1000 || depth > 40) {
-    FilterStat fs{depth, P.m, 0, 0, true};
-    C.stats.filter.push_back(fs);
+    C.stats.filter.push_back(FilterStat{depth, P.m, 0, 0, true, 0});
     base_case(P, Q);
     return;
   }
   const u32 pivot = pick_pivot(P);
   const u64 L = partition(P, Q, pivot);
   const bool stop_rec = double(L) / double(P.m) > 0.85;  // filter_helpers.hpp:130-132
-  FilterStat fs{depth, P.m, L, pivot, false};
-  C.stats.filter.push_back(fs);
+  C.stats.filter.push_back(FilterStat{depth, P.m, L, pivot, false, 0});
+  const size_t me = C.stats.filter.size() - 1;
   filter_rec(Q.at(0, L), P.at(0, L), stop_rec, depth + 1);
-  // heavy half: relabel by the parent array, drop same-component edges
-  // (boruvka_recursive_setup, hybrid_boruvka_edgefilter.hpp:173-225)
+  // heavy records sit in Q[L, m); P[L, m) is free scratch
   const u64 H = P.m - L;
   if (H == 0) return;
-  RecView heavy = Q.at(L, H), scratch = P.at(L, H);
-  {
-    relabel_compact(st, heavy.s, heavy.d, heavy.v, H, gpar, scratch.s, scratch.d, scratch.v, lb,
-                    ctr + C_MNEXT, ctr + C_ERR);
-  }
-  check_errors("filter");
-  const u64 Hk = C.host_counters[C_MNEXT];
-  C.stats.filter.back().heavy_kept = Hk;
-  if (Hk == 0) return;
-  RecView kept = scratch.at(0, Hk);
-  RecView sorted = sort_dedup(kept, heavy.at(0, Hk), bits_for(n0));
-  RecView other = (sorted.s == kept.s) ? heavy.at(0, Hk) : kept;
-  filter_rec(sorted, other, false, depth + 1);
+  C.stats.filter.push_back(C.stats.filter[me]);
+  filter_heavy(Q.at(L, H), P.at(L, H), depth);
 }
 
 void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64 out_cap,
@@ -395,60 +408,41 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
   setup();
   const RecView input{const_cast<u32*>(in_s), const_cast<u32*>(in_d), val0, m0};
   if (cfg.algorithm == KMST_ALGO_FILTER_BORUVKA) {
-    {
-      iota_u32(st, gpar, n0);
-    }
-    // the top level reads the (read-only) input and splits into Y, so the
-    // first level cannot run its base case in place; filter_rec handles the
-    // base case itself when the split is not taken.
+    iota_u32(st, gpar, n0);
     const double avg_degree = double(m0) / double(std::max<u32>(n0, 1));
     if (avg_degree <= double(filter_threshold) || m0 <= 1000) {
-      C.stats.filter.push_back(FilterStat{0, m0, 0, 0, true});
+      C.stats.filter.push_back(FilterStat{0, m0, 0, 0, true, 0});
       rounds(input, X, Y, true);
     } else {
+      // the top level reads the read-only input and splits it into Y
       const u32 pivot = pick_pivot(input);
       const u64 L = partition(input, Y, pivot);
       const bool stop_rec = double(L) / double(m0) > 0.85;
-      C.stats.filter.push_back(FilterStat{0, m0, L, pivot, false});
+      C.stats.filter.push_back(FilterStat{0, m0, L, pivot, false, 0});
       filter_rec(Y.at(0, L), X.at(0, L), stop_rec, 1);
       const u64 H = m0 - L;
       if (H > 0) {
-        RecView heavy = Y.at(L, H), scratch = X.at(L, H);
-        {
-          relabel_compact(st, heavy.s, heavy.d, heavy.v, H, gpar, scratch.s, scratch.d, scratch.v,
-                          lb, ctr + C_MNEXT, ctr + C_ERR);
-        }
-        check_errors("filter");
-        const u64 Hk = C.host_counters[C_MNEXT];
-        C.stats.filter.back().heavy_kept = Hk;
-        if (Hk > 0) {
-          RecView kept = scratch.at(0, Hk);
-          RecView sorted = sort_dedup(kept, heavy.at(0, Hk), bits_for(n0));
-          RecView other = (sorted.s == kept.s) ? heavy.at(0, Hk) : kept;
-          filter_rec(sorted, other, false, 1);
-        }
+        C.stats.filter.push_back(C.stats.filter.front());
+        filter_heavy(Y.at(L, H), X.at(L, H), 0);
       }
     }
   } else {
     rounds(input, X, Y, false);
   }
-  // finish: sort MST uids (uid order == canonical (min, max) order), gather
-  read_counters();
-  const u64 cnt = C.host_counters[C_MST];
+  // finish: keys -> canonical triples, sort by (min, max), copy out
+  const u64 cnt = mst_count;
   if (cnt > out_cap) throw Error(KMST_EINVAL, "output capacity too small");
-  if (cnt > 1) {
-    if (radix_sort(st, mst, nullptr, nullptr, rep_next, nullptr, nullptr, cnt, 0, 32, hist,
-                   scan_scratch))
-      KMST_HIP(hipMemcpyAsync(mst, rep_next, cnt * 4, hipMemcpyDeviceToDevice, st));
-  }
-  u64* total_dev = reinterpret_cast<u64*>(ctr + 16);
+  mst_triples(st, mst, cnt, vbits, packed, in_s, in_d, in_w, X.s, X.d, X.v);
+  RecView res{X.s, X.d, X.v, cnt};
+  if (cnt > 1 && radix_sort(st, X.s, X.d, X.v, Y.s, Y.d, Y.v, cnt, vbits, 2 * vbits, hist,
+                            scan_scratch))
+    res = RecView{Y.s, Y.d, Y.v, cnt};
+  u64* total_dev = reinterpret_cast<u64*>(ctr + C_TOTAL64);
   fill_u64(st, total_dev, 1, 0);
-  {
-    gather_mst(st, mst, cnt, in_s, in_d, in_w, out_s, out_d, out_w, total_dev);
-  }
-  read_counters();
+  mst_output(st, res.s, res.d, res.v, cnt, out_s, out_d, out_w, total_dev);
+  check_errors("finish");
   *out_n = cnt;
-  std::memcpy(total, C.host_counters + 16, 8);
+  *total = counter64(C_TOTAL64);
   C.stats.mst_edges = cnt;
   C.stats.total_weight = *total;
 }
@@ -706,6 +700,7 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
   std::ostringstream o;
   const kmst::SolveStats& s = ctx->stats;
   o << "{\"n\":" << s.n << ",\"m\":" << s.m << ",\"algorithm\":" << s.algorithm
+    << ",\"packed_keys\":" << (s.packed_keys ? "true" : "false")
     << ",\"mst_edges\":" << s.mst_edges << ",\"total_weight\":" << s.total_weight
     << ",\"rounds\":[";
   for (size_t i = 0; i < s.rounds.size(); ++i) {

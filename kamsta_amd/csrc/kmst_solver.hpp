@@ -31,6 +31,7 @@ struct FilterStat {
 struct SolveStats {
   u64 n = 0, m = 0, mst_edges = 0, total_weight = 0;
   int algorithm = 0;
+  bool packed_keys = true;
   std::vector<RoundStat> rounds;
   std::vector<FilterStat> filter;
 };
