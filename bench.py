@@ -37,6 +37,9 @@ def parse():
     p.add_argument("--cpu-log-n", type=int, default=23)
     p.add_argument("--cpu-log-m", type=int, default=27)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calibration-mib", type=int, default=0,
+                   help="first run one device-to-device copy of this many MiB (a kernel of "
+                        "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     p.add_argument("--verify-full", action="store_true",
                    help="also run the oracle's Kruskal on the full input (minutes of CPU)")
     return p.parse_args()
@@ -55,6 +58,13 @@ def main():
     n, m = 1 << args.log_n, 1 << args.log_m
     N = 2 * m
     dev = torch.device("cuda:0")
+    if args.copy_calibration_mib:
+        cal = torch.empty(args.copy_calibration_mib << 18, dtype=torch.int32, device=dev)
+        cal.fill_(1)
+        cal2 = cal.clone()
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.copy_calibration_mib} MiB done")
+        del cal, cal2
     ctx = k.Context(0)
     log(f"[bench] generating GNM n=2^{args.log_n} m=2^{args.log_m} on device")
     s = torch.empty(N, dtype=torch.int32, device=dev)

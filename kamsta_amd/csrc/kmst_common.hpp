@@ -201,6 +201,24 @@ __device__ __forceinline__ u64 lookback_exclusive(u64* status, u32 tile, u64 agg
   return excl;
 }
 
+// Where a record's strict key comes from: stored keys, or (packed-key mode on
+// the read-only input) computed from the weight array as
+// (w << 2b) | (min(s, d) << b) | max(s, d) with b = vbits.
+struct KeySrc {
+  const u64* v = nullptr;
+  const u32* w = nullptr;
+  u32 vbits = 0;
+};
+
+__device__ __forceinline__ u64 pack_key(u32 w, u32 s, u32 d, u32 vbits) {
+  const u64 lo = s < d ? s : d, hi = s < d ? d : s;
+  return (u64(w) << (2 * vbits)) | (lo << vbits) | hi;
+}
+
+__device__ __forceinline__ u64 load_key(const KeySrc& k, u64 i, u32 s, u32 d) {
+  return k.v ? k.v[i] : pack_key(k.w[i], s, d, k.vbits);
+}
+
 // Wave-aggregated append: returns the slot for lanes with pred set.
 __device__ __forceinline__ u32 wave_append(bool pred, u32* counter) {
   const u64 mask = __ballot(pred);

@@ -311,7 +311,7 @@ __device__ __forceinline__ void seg_combine(Piece& cur, const Piece& prev) {
 
 __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ src,
                                                        const u32* __restrict__ dst,
-                                                       const u64* __restrict__ val, u64 m,
+                                                       const KeySrc keys, u64 m,
                                                        u64* __restrict__ best_val,
                                                        u32* __restrict__ best_dst,
                                                        ArgPiece* __restrict__ pieces,
@@ -344,11 +344,11 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
   if (has_items) {
     const u64 my_end = (my_base + ARG_IPT < tile_end) ? my_base + ARG_IPT : tile_end;
     cur_src = src[my_base];
-    cur_val = val[my_base];
+    cur_val = load_key(keys, my_base, cur_src, keys.v ? 0u : dst[my_base]);
     cur_idx = static_cast<u32>(my_base);
     for (u64 i = my_base + 1; i < my_end; ++i) {
       const u32 s = src[i];
-      const u64 v = val[i];
+      const u64 v = load_key(keys, i, s, keys.v ? 0u : dst[i]);
       if (s != cur_src) {
         if (!head_done) {
           head_src = cur_src;
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void argmin_fixup_kernel(const ArgPiece* __r
   }
 }
 
-void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const u64* val, u64 m,
+void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const KeySrc& keys, u64 m,
                       u64 n, u64* best_val, u32* best_dst, ArgPiece* pieces, u32* piece_count) {
   fill_u64(st, best_val, n, VAL_NONE);
   fill_u32(st, piece_count, 1, 0);
@@ -461,7 +461,7 @@ void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const u64*
   const u32 tiles = ceil_div_u32(m, ARG_TILE);
   {
     KTScope t(st, K_ARGMIN, 12.0 * m + 16.0 * n);
-    argmin_kernel<<<tiles, BLOCK, 0, st>>>(src, dst, val, m, best_val, best_dst, pieces,
+    argmin_kernel<<<tiles, BLOCK, 0, st>>>(src, dst, keys, m, best_val, best_dst, pieces,
                                            piece_count);
     KMST_HIP(hipGetLastError());
   }
@@ -579,8 +579,8 @@ __device__ __forceinline__ u32 lb_tile_id(u32* counter) {
 }
 
 __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
-    const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in, u64 m,
-    const u32* __restrict__ label, u32* __restrict__ s_out, u32* __restrict__ d_out,
+    const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 key_min,
+    u64 m, const u32* __restrict__ label, u32* __restrict__ s_out, u32* __restrict__ d_out,
     u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles, u32* count_dev, u32* err) {
   __shared__ u32 s_wave_cnt[WAVES];
   __shared__ u64 s_excl;
@@ -599,10 +599,11 @@ __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
     ls[k] = ld[k] = 0;
     vv[k] = VAL_NONE;
     if (e < m) {
-      vv[k] = v_in[e];
-      if (vv[k] != VAL_NONE) {
-        ls[k] = label[s_in[e]];
-        ld[k] = label[d_in[e]];
+      const u32 se = s_in[e], de = d_in[e];
+      vv[k] = load_key(keys, e, se, de);
+      if (vv[k] != VAL_NONE && vv[k] >= key_min) {
+        ls[k] = label[se];
+        ld[k] = label[de];
         keep = ls[k] != ld[k];
       }
     }
@@ -639,9 +640,9 @@ __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
   }
 }
 
-void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
-                     const u32* label, u32* s_out, u32* d_out, u64* v_out, u64* lb, u32* count_dev,
-                     u32* err) {
+void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
+                     u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
+                     u64* lb, u32* count_dev, u32* err) {
   if (m == 0) {
     fill_u32(st, count_dev, 1, 0);
     return;
@@ -649,7 +650,8 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const u64
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
   fill_u64(st, lb, u64(tiles) + 1, 0);  // tile states + dynamic tile counter
   KTScope t(st, K_RELABEL_SCATTER, 24.0 * m + 16.0 * m);
-  relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, label, s_out, d_out, v_out,
+  relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, key_min, m, label, s_out,
+                                                   d_out, v_out,
                                                    lb, reinterpret_cast<u32*>(lb + tiles), tiles,
                                                    count_dev, err);
   KMST_HIP(hipGetLastError());
@@ -993,19 +995,26 @@ void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* paren
 
 // k evenly spaced weight samples (NONE32 for dead records); the pivot only
 // changes speed, never the result.
-__global__ __launch_bounds__(BLOCK) void sample_kernel(const u64* __restrict__ v, u64 m, u32 k,
+__global__ __launch_bounds__(BLOCK) void sample_kernel(const u64* __restrict__ v,
+                                                       const u32* __restrict__ wt, u64 m, u32 k,
                                                        u32 wshift, u32* __restrict__ out) {
   const u32 j = blockIdx.x * BLOCK + threadIdx.x;
   if (j >= k) return;
   const u64 i = (u64(j) * m) / k + m / (2ull * k);
-  const u64 x = v[i < m ? i : m - 1];
+  const u64 ii = i < m ? i : m - 1;
+  if (!v) {
+    out[j] = wt[ii];
+    return;
+  }
+  const u64 x = v[ii];
   out[j] = (x == VAL_NONE) ? NONE32 : static_cast<u32>(x >> wshift);
 }
 
-void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32 wshift, u32* out) {
+void sample_weights(hipStream_t st, const u64* v, const u32* wt, u64 m, u32 k, u32 wshift,
+                    u32* out) {
   if (k == 0 || m == 0) return;
   KTScope t(st, K_SAMPLE, 12.0 * k);
-  sample_kernel<<<ceil_div_u32(k, BLOCK), BLOCK, 0, st>>>(v, m, k, wshift, out);
+  sample_kernel<<<ceil_div_u32(k, BLOCK), BLOCK, 0, st>>>(v, wt, m, k, wshift, out);
   KMST_HIP(hipGetLastError());
 }
 
@@ -1015,8 +1024,8 @@ void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32 wshift, u32*
 // relabelled and re-sorted before use).  One pass with a decoupled look-back
 // over the light counts; heavy offsets follow from them.
 __global__ __launch_bounds__(BLOCK) void split_kernel(
-    const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in, u64 m,
-    u32 pivot, u32 wshift, u32* __restrict__ s_out, u32* __restrict__ d_out,
+    const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 m,
+    u32 pivot, u32 wshift, bool write_heavy, u32* __restrict__ s_out, u32* __restrict__ d_out,
     u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles, u32* light_dev,
     u32* err) {
   __shared__ u32 s_wave_light[WAVES];
@@ -1037,7 +1046,7 @@ __global__ __launch_bounds__(BLOCK) void split_kernel(
     if (e < m) {
       ss[k] = s_in[e];
       dd[k] = d_in[e];
-      vv[k] = v_in[e];
+      vv[k] = load_key(keys, e, ss[k], dd[k]);
       // dead records (VAL_NONE) always count as heavy
       light = vv[k] != VAL_NONE && static_cast<u32>(vv[k] >> wshift) <= pivot;
     }
@@ -1069,8 +1078,8 @@ __global__ __launch_bounds__(BLOCK) void split_kernel(
   for (int k = 0; k < CMP_IPT; ++k) {
     const u64 e = wbase + u64(k) * WAVE + lane;
     const u64 valid = __ballot(e < m);
-    if (e < m) {
-      const bool light = (bal[k] >> lane) & 1ull;
+    const bool light = (bal[k] >> lane) & 1ull;
+    if (e < m && (light || write_heavy)) {
       const u64 p = light ? lpos + mbcnt(bal[k]) : (m - 1) - (hrank + mbcnt(valid & ~bal[k]));
       s_out[p] = ss[k];
       d_out[p] = dd[k];
@@ -1082,9 +1091,9 @@ __global__ __launch_bounds__(BLOCK) void split_kernel(
   }
 }
 
-void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
-                       u32 pivot, u32 wshift, u32* s_out, u32* d_out, u64* v_out, u64* lb,
-                       u32* light_count_dev, u32* err) {
+void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
+                       u64 m, u32 pivot, u32 wshift, bool write_heavy, u32* s_out, u32* d_out,
+                       u64* v_out, u64* lb, u32* light_count_dev, u32* err) {
   if (m == 0) {
     fill_u32(st, light_count_dev, 1, 0);
     return;
@@ -1092,7 +1101,8 @@ void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
   fill_u64(st, lb, u64(tiles) + 1, 0);
   KTScope t(st, K_SPLIT_SCATTER, 32.0 * m);
-  split_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, v_in, m, pivot, wshift, s_out, d_out, v_out, lb,
+  split_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, m, pivot, wshift, write_heavy, s_out,
+                                         d_out, v_out, lb,
                                          reinterpret_cast<u32*>(lb + tiles), tiles,
                                          light_count_dev, err);
   KMST_HIP(hipGetLastError());
@@ -1126,6 +1136,29 @@ void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32*
   if (n == 0) return;
   KTScope t(st, K_OTHER, 8.0 * n);
   compact_rep_kernel<<<grid_for(n), BLOCK, 0, st>>>(flag, new_id, n, rep);
+  KMST_HIP(hipGetLastError());
+}
+
+// Number of parent-array roots that had edges in the input: the reference
+// skips the heavy half when the light edges left at most one such component
+// (global_num_vertices, distributed_parent_array.hpp:62-72, used at
+// hybrid_boruvka_edgefilter.hpp:289-296).
+__global__ __launch_bounds__(BLOCK) void count_roots_kernel(const u32* __restrict__ gpar,
+                                                            const u32* __restrict__ has_edge, u64 n,
+                                                            u32* count) {
+  u32 c = 0;
+  for (u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x; v < n; v += u64(gridDim.x) * BLOCK)
+    c += (gpar[v] == v && has_edge[v]) ? 1u : 0u;
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
+  if (lane_id() == 0 && c) atomicAdd(count, c);
+}
+
+void count_roots(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32* count_dev) {
+  fill_u32(st, count_dev, 1, 0);
+  if (n == 0) return;
+  KTScope t(st, K_OTHER, 8.0 * n);
+  count_roots_kernel<<<grid_for(n, 4096), BLOCK, 0, st>>>(gpar, has_edge, n, count_dev);
   KMST_HIP(hipGetLastError());
 }
 

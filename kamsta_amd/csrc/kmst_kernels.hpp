@@ -48,7 +48,7 @@ void build_keys_packed(hipStream_t st, const u32* src, const u32* dst, const u32
                        u32 vbits, u64* val);
 
 // K1 segmented argmin over records grouped by src.
-void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const u64* val,
+void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const KeySrc& keys,
                       u64 m, u64 n, u64* best_val, u32* best_dst, ArgPiece* pieces,
                       u32* piece_count);
 
@@ -64,8 +64,9 @@ void make_labels(hipStream_t st, u64 n, const u32* parent, const u64* state, con
 // in one pass.  lb needs lookback_elems(m) u64 entries; *count_dev gets the
 // kept count.
 inline u64 lookback_elems(u64 m) { return (m + CMP_TILE - 1) / CMP_TILE + 1; }
-void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in,
-                     u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
+// Records whose key is below key_min are dropped too (0 keeps all).
+void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
+                     u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
                      u64* lb, u32* count_dev, u32* err);
 
 // K6 stable LSD radix sort by key (s << dbits) | d over total_bits bits.
@@ -92,10 +93,12 @@ void iota_u32(hipStream_t st, u32* p, u64 n);
 void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* parent,
                          const u64* state, const u64* pos, u32* gpar, u32* rep_next);
 // weight of a key = key >> wshift (32 for positional keys, 2 bits(n) for packed)
-void sample_weights(hipStream_t st, const u64* v, u64 m, u32 k, u32 wshift, u32* out);
-void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const u64* v_in, u64 m,
-                       u32 pivot, u32 wshift, u32* s_out, u32* d_out, u64* v_out, u64* lb,
-                       u32* light_count_dev, u32* err);
+void sample_weights(hipStream_t st, const u64* v, const u32* wt, u64 m, u32 k, u32 wshift,
+                    u32* out);
+void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
+                       u64 m, u32 pivot, u32 wshift, bool write_heavy, u32* s_out, u32* d_out,
+                       u64* v_out, u64* lb, u32* light_count_dev, u32* err);
+void count_roots(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32* count_dev);
 
 // Present-vertex compaction helpers (sorted records; flag must be zeroed).
 void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag);

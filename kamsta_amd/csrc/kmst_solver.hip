@@ -120,7 +120,7 @@ kmst_context::~kmst_context() {
 }
 std::vector<DevBuf*> kmst_context::all_bufs() {
   return {&xs, &xd, &xv, &ys, &yd, &yv, &val0, &off, &best_val, &best_dst, &parent, &flag,
-          &newid, &label, &rep, &rep_next, &gpar, &mst, &pieces, &tiles, &scan_scratch, &hist,
+          &newid, &label, &rep, &rep_next, &gpar, &has_edge, &mst, &pieces, &tiles, &scan_scratch, &hist,
           &counters, &lookback, &in_s, &in_d, &in_w, &out_s, &out_d, &out_w, &samples, &gen_a, &gen_b,
           &gen_c, &gen_d};
 }
@@ -133,12 +133,15 @@ enum Ctr : int {
   C_TOTAL64 = 16, C_HSUM64 = 18, C_ROUND64 = 22
 };
 
+// A range of directed records; keys are stored in v, or (the read-only input
+// in packed-key mode, v == nullptr) computed from the weights w.
 struct RecView {
   u32* s = nullptr;
   u32* d = nullptr;
   u64* v = nullptr;
   u64 m = 0;
-  RecView at(u64 off, u64 cnt) const { return RecView{s + off, d + off, v + off, cnt}; }
+  const u32* w = nullptr;
+  RecView at(u64 off, u64 cnt) const { return RecView{s + off, d + off, v + off, cnt, nullptr}; }
 };
 
 class Solver {
@@ -165,7 +168,7 @@ class Solver {
   u32 *best_dst = nullptr, *parent = nullptr, *label = nullptr;
   u64 *state = nullptr, *pos = nullptr;  // per-round vertex state and its scan
   u32 *flag = nullptr, *newid = nullptr;  // u32 views of the same memory (compaction)
-  u32 *rep = nullptr, *rep_next = nullptr, *gpar = nullptr;
+  u32 *rep = nullptr, *rep_next = nullptr, *gpar = nullptr, *has_edge = nullptr;
   u64* mst = nullptr;
   u64 mst_count = 0;
   u32 *scan_scratch = nullptr, *hist = nullptr;
@@ -190,9 +193,13 @@ class Solver {
   // base case: compact sparse vertex ids first, then run the rounds
   void base_case(RecView P, RecView Q);
   void filter_rec(RecView P, RecView Q, bool stop, int depth);
-  void filter_heavy(RecView heavy, RecView scratch, int depth);
+  // heavy may be the read-only input; scratch and alt are writable, disjoint
+  // from each other and from heavy, each with room for heavy.m records
+  void filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth, u64 key_min = 0);
+  bool one_component();
   u32 pick_pivot(RecView r);
-  u64 partition(RecView in, RecView out, u32 pivot);
+  u64 partition(RecView in, RecView out, u32 pivot, bool write_heavy = true);
+  KeySrc keys_of(const RecView& r) const { return KeySrc{r.v, r.w, vbits}; }
   RecView sort_dedup(RecView cur, RecView alt, u32 bits);
 };
 
@@ -203,7 +210,6 @@ void Solver::allocate() {
               static_cast<u64*>(C.xv.get(m * 8)), m};
   Y = RecView{static_cast<u32*>(C.ys.get(m * 4)), static_cast<u32*>(C.yd.get(m * 4)),
               static_cast<u64*>(C.yv.get(m * 8)), m};
-  val0 = static_cast<u64*>(C.val0.get(m * 8));
   best_val = static_cast<u64*>(C.best_val.get(n * 8));
   best_dst = static_cast<u32*>(C.best_dst.get(n * 4));
   parent = static_cast<u32*>(C.parent.get(n * 4));
@@ -215,6 +221,7 @@ void Solver::allocate() {
   rep = static_cast<u32*>(C.rep.get(n * 4));
   rep_next = static_cast<u32*>(C.rep_next.get(n * 4));
   gpar = static_cast<u32*>(C.gpar.get(n * 4));
+  has_edge = static_cast<u32*>(C.has_edge.get(n * 4));
   mst = static_cast<u64*>(C.mst.get(n * 8));
   const u64 arg_tiles = (m + ARG_TILE - 1) / ARG_TILE;
   pieces = static_cast<ArgPiece*>(C.pieces.get(2 * arg_tiles * sizeof(ArgPiece) + 64));
@@ -252,12 +259,12 @@ void Solver::setup() {
   if (u64(C.host_counters[C_FWD]) * 2 != m0 || counter64(C_HSUM64) != counter64(C_HSUM64 + 2))
     throw Error(KMST_EASYMMETRIC, "input: records are not symmetric (src,dst,w) <-> (dst,src,w)");
   vbits = bits_for(n0);
+  C.stats.max_weight = C.host_counters[C_MAXW];
   const u32 wbits = bits_for(u64(C.host_counters[C_MAXW]) + 1);
   packed = wbits + 2 * vbits <= 64;
   C.stats.packed_keys = packed;
-  if (packed) {
-    build_keys_packed(st, in_s, in_d, in_w, m0, vbits, val0);
-  } else {
+  if (!packed) {
+    val0 = static_cast<u64*>(C.val0.get(m0 * 8));
     build_offsets(st, in_s, m0, n0, label /* reused as offsets, n0+1 */);
     build_keys(st, in_s, in_d, in_w, m0, label, val0, ctr + C_ERR);
     check_errors("input");
@@ -286,7 +293,8 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     RoundStat rs;
     rs.n = n;
     rs.m = m;
-    segmented_argmin(st, cur.s, cur.d, cur.v, m, n, best_val, best_dst, pieces, ctr + C_PIECES);
+    segmented_argmin(st, cur.s, cur.d, keys_of(cur), m, n, best_val, best_dst, pieces,
+                     ctr + C_PIECES);
     hook(st, n, best_val, best_dst, parent, state);
     pointer_jump(st, n, parent, ctr + C_ERR);
     scan_exclusive_u64(st, state, pos, n, reinterpret_cast<u64*>(scan_scratch),
@@ -297,8 +305,8 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
       std::swap(rep, rep_next);
     }
     RecView out = (cur.s == A.s) ? B : A;
-    relabel_compact(st, cur.s, cur.d, cur.v, m, label, out.s, out.d, out.v, lb, ctr + C_MNEXT,
-                    ctr + C_ERR);
+    relabel_compact(st, cur.s, cur.d, keys_of(cur), 0, m, label, out.s, out.d, out.v, lb,
+                    ctr + C_MNEXT, ctr + C_ERR);
     check_errors("round");
     const u64 round_total = counter64(C_ROUND64);
     const u64 n_next = round_total & 0xFFFFFFFFull;
@@ -327,7 +335,8 @@ void Solver::base_case(RecView P, RecView Q) {
     mark_present(st, P.s, P.m, flag);
     scan_exclusive_u32(st, flag, newid, n0, scan_scratch, ctr + C_NNEXT);
     compact_rep(st, flag, newid, n0, rep);
-    relabel_compact(st, P.s, P.d, P.v, P.m, newid, Q.s, Q.d, Q.v, lb, ctr + C_MNEXT, ctr + C_ERR);
+    relabel_compact(st, P.s, P.d, keys_of(P), 0, P.m, newid, Q.s, Q.d, Q.v, lb, ctr + C_MNEXT,
+                    ctr + C_ERR);
     check_errors("compact");
     const u64 nc = C.host_counters[C_NNEXT];
     const u64 mc = C.host_counters[C_MNEXT];
@@ -339,7 +348,7 @@ void Solver::base_case(RecView P, RecView Q) {
 
 u32 Solver::pick_pivot(RecView r) {
   const u32 k = static_cast<u32>(std::min<u64>(1023, r.m));
-  sample_weights(st, r.v, r.m, k, packed ? 2 * vbits : 32, samples);
+  sample_weights(st, r.v, r.w, r.m, k, packed ? 2 * vbits : 32, samples);
   std::vector<u32> h(k);
   KMST_HIP(hipMemcpyAsync(h.data(), samples, k * 4, hipMemcpyDeviceToHost, st));
   KMST_HIP(hipStreamSynchronize(st));
@@ -349,9 +358,9 @@ u32 Solver::pick_pivot(RecView r) {
   return h[h.size() / 2];  // median sampled weight (select_pivot, distributed_partitioning.hpp:92-120)
 }
 
-u64 Solver::partition(RecView in, RecView out, u32 pivot) {
-  split_light_heavy(st, in.s, in.d, in.v, in.m, pivot, packed ? 2 * vbits : 32, out.s, out.d,
-                    out.v, lb, ctr + C_LIGHT, ctr + C_ERR);
+u64 Solver::partition(RecView in, RecView out, u32 pivot, bool write_heavy) {
+  split_light_heavy(st, in.s, in.d, keys_of(in), in.m, pivot, packed ? 2 * vbits : 32,
+                    write_heavy, out.s, out.d, out.v, lb, ctr + C_LIGHT, ctr + C_ERR);
   check_errors("split");
   return C.host_counters[C_LIGHT];
 }
@@ -359,17 +368,29 @@ u64 Solver::partition(RecView in, RecView out, u32 pivot) {
 // Heavy half of one recursion level: relabel by the parent array, drop
 // same-component edges, sort + keep the lightest copies, recurse
 // (boruvka_recursive_setup, hybrid_boruvka_edgefilter.hpp:173-225).
-void Solver::filter_heavy(RecView heavy, RecView scratch, int depth) {
+bool Solver::one_component() {
+  count_roots(st, gpar, has_edge, n0, ctr + C_NNEXT);
+  read_counters();
+  return C.host_counters[C_NNEXT] <= 1;
+}
+
+void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth, u64 key_min) {
   if (heavy.m == 0) return;
-  relabel_compact(st, heavy.s, heavy.d, heavy.v, heavy.m, gpar, scratch.s, scratch.d, scratch.v,
-                  lb, ctr + C_MNEXT, ctr + C_ERR);
+  // the light edges left at most one component with edges: every heavy
+  // edge is a self-loop (hybrid_boruvka_edgefilter.hpp:289-296)
+  if (one_component()) {
+    C.stats.filter.back().skipped = true;
+    return;
+  }
+  relabel_compact(st, heavy.s, heavy.d, keys_of(heavy), key_min, heavy.m, gpar, scratch.s,
+                  scratch.d, scratch.v, lb, ctr + C_MNEXT, ctr + C_ERR);
   check_errors("filter");
   const u64 Hk = C.host_counters[C_MNEXT];
   C.stats.filter.back().heavy_kept = Hk;
   if (Hk == 0) return;
   RecView kept = scratch.at(0, Hk);
-  RecView sorted = sort_dedup(kept, heavy.at(0, Hk), vbits);
-  RecView other = (sorted.s == kept.s) ? heavy.at(0, Hk) : kept;
+  RecView sorted = sort_dedup(kept, alt.at(0, Hk), vbits);
+  RecView other = (sorted.s == kept.s) ? alt.at(0, Hk) : kept;
   filter_rec(sorted, other, false, depth + 1);
 }
 
@@ -394,7 +415,8 @@ void Solver::filter_rec(RecView P, RecView Q, bool stop, int depth) {
   const u64 H = P.m - L;
   if (H == 0) return;
   C.stats.filter.push_back(C.stats.filter[me]);
-  filter_heavy(Q.at(L, H), P.at(L, H), depth);
+  // (heavy records are consumed by the relabel, so their range is the alt)
+  filter_heavy(Q.at(L, H), P.at(L, H), Q.at(L, H), depth);
 }
 
 void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64 out_cap,
@@ -406,24 +428,31 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
   filter_threshold = cfg.filter_threshold;
   allocate();
   setup();
-  const RecView input{const_cast<u32*>(in_s), const_cast<u32*>(in_d), val0, m0};
+  const RecView input{const_cast<u32*>(in_s), const_cast<u32*>(in_d), packed ? nullptr : val0, m0,
+                      packed ? in_w : nullptr};
   if (cfg.algorithm == KMST_ALGO_FILTER_BORUVKA) {
     iota_u32(st, gpar, n0);
+    fill_u32(st, has_edge, n0, 0);
+    mark_present(st, in_s, m0, has_edge);  // init_parent_array (hybrid_boruvka_edgefilter.hpp:332-352)
     const double avg_degree = double(m0) / double(std::max<u32>(n0, 1));
     if (avg_degree <= double(filter_threshold) || m0 <= 1000) {
       C.stats.filter.push_back(FilterStat{0, m0, 0, 0, true, 0});
       rounds(input, X, Y, true);
     } else {
-      // the top level reads the read-only input and splits it into Y
+      // The top level reads the read-only input and writes only its light
+      // records (to Y); its heavy half is streamed from the input again,
+      // filtered by key >= key_min, if the light edges leave more than one
+      // component.
       const u32 pivot = pick_pivot(input);
-      const u64 L = partition(input, Y, pivot);
+      const u64 L = partition(input, Y, pivot, /*write_heavy=*/false);
       const bool stop_rec = double(L) / double(m0) > 0.85;
       C.stats.filter.push_back(FilterStat{0, m0, L, pivot, false, 0});
       filter_rec(Y.at(0, L), X.at(0, L), stop_rec, 1);
-      const u64 H = m0 - L;
-      if (H > 0) {
+      const u32 wshift = packed ? 2 * vbits : 32;
+      const u64 max_w = C.stats.max_weight;
+      if (L < m0 && pivot < max_w) {
         C.stats.filter.push_back(C.stats.filter.front());
-        filter_heavy(Y.at(L, H), X.at(L, H), 0);
+        filter_heavy(input, X, Y, 0, (u64(pivot) + 1) << wshift);
       }
     }
   } else {
@@ -653,7 +682,8 @@ kmst_status kmst_k_segmented_argmin(kmst_context* ctx, const uint32_t* src, cons
     auto* pieces = static_cast<kmst::ArgPiece*>(
         ctx->pieces.get(2 * tiles * sizeof(kmst::ArgPiece) + 64));
     auto* ctr = static_cast<uint32_t*>(ctx->counters.get(256));
-    kmst::segmented_argmin(ctx->stream, src, dst, val, m, n, best_val, best_dst, pieces,
+    kmst::segmented_argmin(ctx->stream, src, dst, kmst::KeySrc{val, nullptr, 0}, m, n, best_val,
+                           best_dst, pieces,
                            ctr + kmst::C_PIECES);
     KMST_HIP(hipStreamSynchronize(ctx->stream));
   });
@@ -713,7 +743,8 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
     const auto& f = s.filter[i];
     o << (i ? "," : "") << "{\"depth\":" << f.depth << ",\"m\":" << f.m << ",\"light\":" << f.light
       << ",\"pivot\":" << f.pivot << ",\"base\":" << (f.base ? "true" : "false")
-      << ",\"heavy_kept\":" << f.heavy_kept << "}";
+      << ",\"heavy_kept\":" << f.heavy_kept << ",\"heavy_skipped\":"
+      << (f.skipped ? "true" : "false") << "}";
   }
   o << "],\"kernels\":{";
   bool firstk = true;

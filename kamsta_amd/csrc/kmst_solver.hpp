@@ -27,11 +27,13 @@ struct FilterStat {
   u32 pivot = 0;
   bool base = false;
   u64 heavy_kept = 0;
+  bool skipped = false;
 };
 struct SolveStats {
   u64 n = 0, m = 0, mst_edges = 0, total_weight = 0;
   int algorithm = 0;
   bool packed_keys = true;
+  u64 max_weight = 0;
   std::vector<RoundStat> rounds;
   std::vector<FilterStat> filter;
 };
@@ -56,7 +58,7 @@ struct kmst_context {
   // record buffers (X, Y ping-pong), strict keys of the input
   kmst::DevBuf xs, xd, xv, ys, yd, yv, val0;
   // vertex arrays
-  kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, mst;
+  kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, has_edge, mst;
   // kernel scratch
   kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback;
   // host-API staging
