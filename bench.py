@@ -40,13 +40,129 @@ def parse():
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (a kernel of "
                         "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    p.add_argument("--serving", action="store_true",
+                   help="batched-serving benchmark instead of the one-graph throughput run")
+    p.add_argument("--requests", type=int, default=4096)
+    p.add_argument("--rate", type=float, default=20000.0, help="Poisson arrival rate, req/s")
     p.add_argument("--verify-full", action="store_true",
                    help="also run the oracle's Kruskal on the full input (minutes of CPU)")
     return p.parse_args()
 
 
+def serving(args):
+    """Batched serving: 4096 GNM requests, n log-uniform in [2^8, 2^18],
+    average degree uniform in [8, 32], weights U[1,254], seed = base + i,
+    Poisson arrivals at a fixed rate; the batched server against the
+    one-graph-per-launch server on the same arrival trace."""
+    import threading
+    import numpy as np
+    import torch
+    import kamsta_amd as k
+
+    rng = np.random.default_rng(args.seed)
+    dev = torch.device("cuda:0")
+    ctx = k.Context(0)
+    specs = []
+    for i in range(args.requests):
+        n = int(round(2 ** rng.uniform(8, 18)))
+        deg = rng.uniform(8, 32)
+        m = int(min(n * deg / 2, n * (n - 1) // 2))
+        specs.append((n, m))
+    total = sum(2 * m for _, m in specs)
+    log(f"[serve] {args.requests} requests, {total} directed records, generating on device")
+    pool = torch.empty((3, total), dtype=torch.int32, pin_memory=True)
+    t0 = time.time()
+    offs = []
+    o = 0
+    scratch = [torch.empty(2 * max(m for _, m in specs), dtype=torch.int32, device=dev)
+               for _ in range(3)]
+    for i, (n, m) in enumerate(specs):
+        ctx.gen_gnm_device(n, m, 254, args.seed * 1000003 + i, *scratch)
+        for j in range(3):
+            pool[j, o:o + 2 * m].copy_(scratch[j][:2 * m])
+        offs.append(o)
+        o += 2 * m
+    torch.cuda.synchronize()
+    del scratch
+    log(f"[serve] generated in {time.time() - t0:.1f}s")
+    pv = pool.numpy().view(np.uint32)
+    reqs = [(pv[0, a:a + 2 * m], pv[1, a:a + 2 * m], pv[2, a:a + 2 * m], n)
+            for a, (n, m) in zip(offs, specs)]
+    gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
+
+    def run(max_batch_requests):
+        srv = k.Server(algorithm=k.FILTER_BORUVKA, filter_threshold=args.filter_threshold,
+                       max_batch_requests=max_batch_requests)
+        # warm the server's buffers and kernels on a few requests
+        for t in [srv.submit(*reqs[i]) for i in range(8)]:
+            srv.wait(t)
+        tickets = [None] * len(reqs)
+        arrivals = np.zeros(len(reqs), dtype=np.int64)
+        base = time.monotonic_ns() + 2_000_000
+        sched = base + np.cumsum(gaps * 1e9).astype(np.int64)
+
+        def submitter():
+            for i, r in enumerate(reqs):
+                while time.monotonic_ns() < sched[i]:
+                    pass
+                arrivals[i] = sched[i]
+                tickets[i] = srv.submit(*r)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        results = []
+        for i in range(len(reqs)):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            results.append(srv.wait(tickets[i]))
+        th.join()
+        nb, nq = srv.counters()
+        srv.close()
+        done = np.array([r[0]["done_ns"] for r in results], dtype=np.int64)
+        lat = (done - arrivals) / 1e6
+        span = (done.max() - arrivals.min()) / 1e9
+        bad = sum(r[0]["status"] != 0 for r in results)
+        return {"req_per_s": len(reqs) / span, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "batches": nb - 1, "failed": bad,
+                "span_s": span}, results
+
+    batched, rb = run(4096)
+    log(f"[serve] batched: {batched}")
+    alone, ra = run(1)
+    log(f"[serve] one graph per launch: {alone}")
+    identical = all(x[0]["total_weight"] == y[0]["total_weight"] and
+                    all(np.array_equal(p, q) for p, q in zip(x[1], y[1]))
+                    for x, y in zip(rb, ra))
+    line = {
+        "metric": "batched-serving req/s + p50/p99",
+        "value": round(batched["req_per_s"], 1),
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "u32/u64",
+        "data": f"synthetic: {args.requests} GNM requests, n log-uniform [2^8,2^18], "
+                f"avg degree U[8,32], weights U[1,254], Poisson arrivals {args.rate:g}/s, "
+                f"inputs in pinned host memory (latency includes H2D and D2H)",
+        "config": {"workload": "batched serving", "requests": args.requests, "rate": args.rate,
+                   "directed_records": total, "algorithm": "filter_boruvka"},
+        "batched": {kk: (round(v, 3) if isinstance(v, float) else v) for kk, v in batched.items()},
+        "one_graph_per_launch": {kk: (round(v, 3) if isinstance(v, float) else v)
+                                 for kk, v in alone.items()},
+        "speedup_req_per_s": round(batched["req_per_s"] / alone["req_per_s"], 2),
+        "identical_results": bool(identical),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
+    if args.serving:
+        if args.gpus != 1:
+            print(json.dumps({"error": "one GPU only", "n_gpus": args.gpus}))
+            sys.exit(2)
+        serving(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                     "note only (see DESIGN.md)", "n_gpus": args.gpus}))

@@ -133,6 +133,55 @@ kmst_status kmst_gen_gnm_host(uint64_t n, uint64_t m, uint32_t max_weight,
                               uint64_t seed, uint32_t* src, uint32_t* dst,
                               uint32_t* weight);
 
+/* ---- Batched serving ----------------------------------------------------
+ * A persistent request loop on one GPU: requests (host SoA edge lists with
+ * the same contract as kmst_boruvka) are queued by kmst_server_submit,
+ * admitted into batches by device-memory footprint, solved as one segmented
+ * device solve (vertex-id offsets keep the requests disjoint), and completed
+ * asynchronously.  Each request's result is identical to solving it alone.
+ * Input arrays must stay valid until the ticket completes; pinned host
+ * memory (hipHostMalloc) uploads at full link speed.  Output arrays receive
+ * (min, max, weight) records sorted by (min, max) in the request's own ids.
+ * No counterpart in the reference, which solves one graph per call
+ * (benchmarks/mst.cpp:198-275). */
+typedef struct kmst_server kmst_server;
+
+typedef struct kmst_server_config {
+  int32_t device;
+  int32_t algorithm;             /* kmst_algorithm used for every batch */
+  uint64_t filter_threshold;
+  uint64_t max_batch_records;    /* admission bound per batch (directed records) */
+  uint64_t max_batch_requests;   /* 1 = one graph per launch */
+  uint64_t hbm_budget_bytes;     /* 0: half of the free device memory */
+  uint32_t queue_capacity;       /* queued requests before submit returns KMST_EBUSY */
+  uint32_t buffer_sets;          /* batches in flight (upload / compute / delivery) */
+} kmst_server_config;
+
+typedef struct kmst_request_result {
+  uint64_t ticket;
+  int32_t status;                /* kmst_status of this request */
+  int32_t done;                  /* 1 once complete */
+  uint64_t mst_edges;
+  uint64_t total_weight;
+  uint64_t batch_requests;       /* size of the batch it was solved in */
+  uint64_t submit_ns, admit_ns, start_ns, done_ns;  /* steady clock */
+} kmst_request_result;
+
+void kmst_server_config_default(kmst_server_config* cfg);
+kmst_status kmst_server_create(const kmst_server_config* cfg, kmst_server** out);
+/* KMST_ENOMEM: the request alone exceeds the admission budget;
+ * KMST_EBUSY: the queue is full. */
+kmst_status kmst_server_submit(kmst_server* s, const uint32_t* src, const uint32_t* dst,
+                               const uint32_t* weight, uint64_t m_directed, uint64_t n,
+                               uint32_t* out_src, uint32_t* out_dst, uint32_t* out_weight,
+                               uint64_t out_capacity, uint64_t* ticket);
+/* Copies the ticket's state; a completed ticket is released by this call. */
+kmst_status kmst_server_poll(kmst_server* s, uint64_t ticket, kmst_request_result* res);
+kmst_status kmst_server_wait(kmst_server* s, uint64_t ticket, kmst_request_result* res);
+kmst_status kmst_server_counters(kmst_server* s, uint64_t* batches, uint64_t* requests);
+/* Waits for every submitted request to complete, then frees the server. */
+void kmst_server_destroy(kmst_server* s);
+
 /* ---- Kernel-level entry points (device pointers), for per-kernel tests. */
 /* K1: per-vertex min of val over records grouped by src (sorted);
  * best_val[v] = UINT64_MAX and best_dst[v] untouched for empty vertices. */

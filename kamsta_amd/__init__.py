@@ -42,6 +42,8 @@ EXPORTED_SYMBOLS = [
     "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_context_set_timing",
     "kmst_last_error", "kmst_context_stats_json", "kmst_gen_gnm_device", "kmst_gen_gnm_host",
     "kmst_k_segmented_argmin", "kmst_k_sort_pairs", "kmst_k_pointer_jump",
+    "kmst_server_config_default", "kmst_server_create", "kmst_server_submit", "kmst_server_poll",
+    "kmst_server_wait", "kmst_server_counters", "kmst_server_destroy",
 ]
 
 
@@ -60,6 +62,21 @@ class _Config(ctypes.Structure):
 class _DevEdges(ctypes.Structure):
     _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("weight", ctypes.c_void_p),
                 ("m_directed", ctypes.c_uint64), ("n", ctypes.c_uint64)]
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("algorithm", ctypes.c_int32),
+                ("filter_threshold", ctypes.c_uint64), ("max_batch_records", ctypes.c_uint64),
+                ("max_batch_requests", ctypes.c_uint64), ("hbm_budget_bytes", ctypes.c_uint64),
+                ("queue_capacity", ctypes.c_uint32), ("buffer_sets", ctypes.c_uint32)]
+
+
+class _RequestResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("status", ctypes.c_int32), ("done", ctypes.c_int32),
+                ("mst_edges", ctypes.c_uint64), ("total_weight", ctypes.c_uint64),
+                ("batch_requests", ctypes.c_uint64), ("submit_ns", ctypes.c_uint64),
+                ("admit_ns", ctypes.c_uint64), ("start_ns", ctypes.c_uint64),
+                ("done_ns", ctypes.c_uint64)]
 
 
 class _DevMst(ctypes.Structure):
@@ -110,6 +127,13 @@ def load():
         "kmst_k_segmented_argmin": (i32, [vp, vp, vp, vp, u64, u64, vp, vp]),
         "kmst_k_sort_pairs": (i32, [vp, vp, vp, vp, u64, u32]),
         "kmst_k_pointer_jump": (i32, [vp, vp, u64]),
+        "kmst_server_config_default": (None, [vp]),
+        "kmst_server_create": (i32, [vp, ctypes.POINTER(vp)]),
+        "kmst_server_submit": (i32, [vp, vp, vp, vp, u64, u64, vp, vp, vp, u64, pu64]),
+        "kmst_server_poll": (i32, [vp, u64, vp]),
+        "kmst_server_wait": (i32, [vp, u64, vp]),
+        "kmst_server_counters": (i32, [vp, pu64, pu64]),
+        "kmst_server_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -224,3 +248,69 @@ class Context:
 
     def pointer_jump(self, parent, n):
         _check(self._lib.kmst_k_pointer_jump(self._h, _ptr(parent), n))
+
+
+class Server:
+    """Batched MST serving on one GPU (kmst_server_*).
+
+    submit() takes host uint32 arrays (src, dst, weight; the kmst_boruvka
+    input contract) and returns a ticket; wait() returns the result dict and
+    the (min, max, weight) arrays.  The arrays are kept alive until the ticket
+    completes.  max_batch_requests=1 gives the one-graph-per-launch path."""
+
+    def __init__(self, device=0, algorithm=FILTER_BORUVKA, filter_threshold=4,
+                 max_batch_records=None, max_batch_requests=None, hbm_budget_bytes=0,
+                 queue_capacity=None, buffer_sets=3):
+        self._lib = load()
+        cfg = _ServerConfig()
+        self._lib.kmst_server_config_default(ctypes.byref(cfg))
+        cfg.device, cfg.algorithm, cfg.filter_threshold = device, algorithm, filter_threshold
+        if max_batch_records is not None:
+            cfg.max_batch_records = max_batch_records
+        if max_batch_requests is not None:
+            cfg.max_batch_requests = max_batch_requests
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        if queue_capacity is not None:
+            cfg.queue_capacity = queue_capacity
+        cfg.buffer_sets = buffer_sets
+        self._h = ctypes.c_void_p()
+        _check(self._lib.kmst_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+        self._live = {}
+
+    def submit(self, src, dst, weight, n):
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        weight = np.ascontiguousarray(weight, dtype=np.uint32)
+        cap = max(int(n), 1)
+        outs = [np.empty(cap, np.uint32) for _ in range(3)]
+        t = ctypes.c_uint64(0)
+        _check(self._lib.kmst_server_submit(
+            self._h, src.ctypes.data, dst.ctypes.data, weight.ctypes.data, len(src), int(n),
+            outs[0].ctypes.data, outs[1].ctypes.data, outs[2].ctypes.data, cap, ctypes.byref(t)))
+        self._live[t.value] = (src, dst, weight, outs)
+        return t.value
+
+    def wait(self, ticket):
+        r = _RequestResult()
+        _check(self._lib.kmst_server_wait(self._h, ticket, ctypes.byref(r)))
+        _, _, _, outs = self._live.pop(ticket)
+        k = r.mst_edges if r.status == KMST_OK else 0
+        res = {f: getattr(r, f) for f, _ in _RequestResult._fields_}
+        return res, (outs[0][:k].copy(), outs[1][:k].copy(), outs[2][:k].copy())
+
+    def counters(self):
+        b, q = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(self._lib.kmst_server_counters(self._h, ctypes.byref(b), ctypes.byref(q)))
+        return b.value, q.value
+
+    def close(self):
+        if self._h:
+            self._lib.kmst_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

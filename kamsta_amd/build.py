@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkmst.so")
-SOURCES = ["kmst_kernels.hip", "kmst_solver.hip", "kmst_gen.hip"]
+SOURCES = ["kmst_kernels.hip", "kmst_solver.hip", "kmst_gen.hip", "kmst_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -25,6 +25,10 @@ FLAGS = [
 ]
 
 
+DRIVER_SRC = os.path.join(HERE, "driver", "mst_benchmarks.cpp")
+DRIVER = os.path.join(HERE, "mst_benchmarks")
+
+
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -33,6 +37,13 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the command-line driver is plain C++ over the C ABI
+    dcmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", DRIVER + ".tmp", DRIVER_SRC,
+            "-L" + HERE, "-lkmst", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(dcmd), flush=True)
+    subprocess.run(dcmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 

@@ -1,0 +1,540 @@
+// kmst_serve.hip -- batched MST serving: a persistent request loop that
+// admits MST solves of different sizes as they arrive, batches them into one
+// device-resident solve, and overlaps uploads, compute and result delivery.
+//
+// Batching: requests are concatenated with vertex-id offsets; because each
+// request occupies its own id range, the concatenation stays sorted by
+// (src, dst), no edge joins two requests, and the strict key order inside a
+// request is unchanged (packed keys shift both endpoints by the same offset;
+// positional keys are positions in the concatenation, monotone within a
+// request).  The batch's minimum spanning forest is therefore the disjoint
+// union of the requests' forests, edge for edge, and each request's result is
+// identical alone and in any mix.
+//
+// Pipeline (three threads, rotating device buffer sets):
+//   admission  pops queued requests while their footprint fits the batch
+//              budget, uploads them (one async copy per array, pinned host
+//              memory goes at link speed) on the copy stream;
+//   compute    adds the vertex-id offsets and runs the solve;
+//   delivery   copies the batch MST back, splits it by request (subtracting
+//              offsets) and completes the tickets.
+// Requests that arrive while a batch runs are admitted into the next one as
+// soon as a buffer set frees up (continuous batching at batch granularity).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "kmst_kernels.hpp"
+#include "kmst_solver.hpp"
+
+namespace kmst {
+
+static u64 now_ns() {
+  return static_cast<u64>(std::chrono::duration_cast<std::chrono::nanoseconds>(
+                              std::chrono::steady_clock::now().time_since_epoch())
+                              .count());
+}
+
+// Adds each request's vertex offset to its records; request of record i is
+// the last r with r_off[r] <= i.
+__global__ __launch_bounds__(BLOCK) void batch_offset_kernel(u32* __restrict__ src,
+                                                             u32* __restrict__ dst, u64 m,
+                                                             const u64* __restrict__ r_off,
+                                                             const u32* __restrict__ v_off,
+                                                             u32 count) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
+    u32 lo = 0, hi = count;  // find largest r with r_off[r] <= i
+    while (hi - lo > 1) {
+      const u32 mid = (lo + hi) >> 1;
+      if (r_off[mid] <= i) lo = mid; else hi = mid;
+    }
+    const u32 o = v_off[lo];
+    src[i] += o;
+    dst[i] += o;
+  }
+}
+
+struct Request {
+  u64 ticket = 0;
+  const u32 *src = nullptr, *dst = nullptr, *w = nullptr;
+  u64 m = 0, n = 0;
+  u32 *out_src = nullptr, *out_dst = nullptr, *out_w = nullptr;
+  u64 out_cap = 0;
+  kmst_request_result res{};
+};
+
+struct BufferSet {
+  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off;
+  std::vector<u64> h_r_off;
+  std::vector<u32> h_v_off;
+  u32* h_out = nullptr;  // pinned: 3 x capacity words
+  u64 h_out_cap = 0;
+  hipEvent_t uploaded = nullptr;
+  std::vector<std::shared_ptr<Request>> reqs;
+  u64 m = 0, n = 0;
+  u64 mst_n = 0, mst_w = 0;
+  kmst_status status = KMST_OK;
+  bool solved_individually = false;  // results already written per request
+  std::string error;
+};
+
+}  // namespace kmst
+
+using namespace kmst;
+
+struct kmst_server {
+  kmst_server_config cfg{};
+  std::unique_ptr<kmst_context> ctx;       // compute context (its own stream)
+  hipStream_t copy_stream = nullptr;       // uploads
+  hipStream_t down_stream = nullptr;       // result downloads
+  std::vector<std::unique_ptr<BufferSet>> sets;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::map<u64, std::shared_ptr<Request>> pending;  // not yet collected
+  std::deque<BufferSet*> free_sets, ready_sets, done_sets;
+  u64 next_ticket = 1;
+  bool stop = false;
+  std::thread admit_th, compute_th, deliver_th;
+  u64 batches = 0, batched_requests = 0;
+
+  void admit_loop();
+  void compute_loop();
+  void deliver_loop();
+  void solve_alone(BufferSet& b);
+};
+
+namespace {
+
+constexpr u64 BYTES_PER_RECORD = 3 * 12 + 48;  // upload sets + solver workspace
+constexpr u64 BYTES_PER_VERTEX = 64;
+
+u64 footprint(u64 m, u64 n) { return m * BYTES_PER_RECORD + n * BYTES_PER_VERTEX; }
+
+template <typename F>
+kmst_status guard(F&& f) {
+  try {
+    f();
+    return KMST_OK;
+  } catch (const kmst::Error& e) {
+    kmst::g_last_error = e.what();
+    return e.status;
+  } catch (const std::exception& e) {
+    kmst::g_last_error = e.what();
+    return KMST_EDEVICE;
+  }
+}
+
+}  // namespace
+
+void kmst_server::admit_loop() {
+  (void)hipSetDevice(cfg.device);
+  while (true) {
+    BufferSet* b = nullptr;
+    std::vector<std::shared_ptr<Request>> take;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return stop || (!queue.empty() && !free_sets.empty()); });
+      if (stop) return;  // set only after every request has completed
+      b = free_sets.front();
+      free_sets.pop_front();
+      u64 m = 0, n = 0;
+      while (!queue.empty() && take.size() < cfg.max_batch_requests) {
+        auto& r = queue.front();
+        if (!take.empty() && (m + r->m > cfg.max_batch_records ||
+                              n + r->n >= (u64{1} << 31)))
+          break;
+        m += r->m;
+        n += r->n;
+        take.push_back(r);
+        queue.pop_front();
+      }
+    }
+    const u64 t_admit = now_ns();
+    u64 m = 0, n = 0;
+    b->h_r_off.assign(take.size() + 1, 0);
+    b->h_v_off.assign(take.size(), 0);
+    for (size_t i = 0; i < take.size(); ++i) {
+      b->h_r_off[i] = m;
+      b->h_v_off[i] = static_cast<u32>(n);
+      m += take[i]->m;
+      n += take[i]->n;
+      take[i]->res.admit_ns = t_admit;
+    }
+    b->h_r_off[take.size()] = m;
+    b->m = m;
+    b->n = n;
+    b->reqs = std::move(take);
+    b->status = KMST_OK;
+    b->solved_individually = false;
+    b->error.clear();
+    const kmst_status st = guard([&] {
+      const u64 mm = m ? m : 1, nn = n ? n : 1;
+      u32* ds = static_cast<u32*>(b->s.get(mm * 4));
+      u32* dd = static_cast<u32*>(b->d.get(mm * 4));
+      u32* dw = static_cast<u32*>(b->w.get(mm * 4));
+      b->out_s.get(nn * 4);
+      b->out_d.get(nn * 4);
+      b->out_w.get(nn * 4);
+      u64* dro = static_cast<u64*>(b->r_off.get((b->reqs.size() + 1) * 8));
+      u32* dvo = static_cast<u32*>(b->v_off.get(b->reqs.size() * 4 + 4));
+      if (b->h_out_cap < 3 * nn) {
+        if (b->h_out) (void)hipHostFree(b->h_out);
+        b->h_out = nullptr;
+        KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_out), 3 * nn * 4, hipHostMallocDefault));
+        b->h_out_cap = 3 * nn;
+      }
+      for (size_t i = 0; i < b->reqs.size(); ++i) {
+        const Request& r = *b->reqs[i];
+        const u64 off = b->h_r_off[i];
+        if (r.m == 0) continue;
+        KMST_HIP(hipMemcpyAsync(ds + off, r.src, r.m * 4, hipMemcpyHostToDevice, copy_stream));
+        KMST_HIP(hipMemcpyAsync(dd + off, r.dst, r.m * 4, hipMemcpyHostToDevice, copy_stream));
+        KMST_HIP(hipMemcpyAsync(dw + off, r.w, r.m * 4, hipMemcpyHostToDevice, copy_stream));
+      }
+      KMST_HIP(hipMemcpyAsync(dro, b->h_r_off.data(), b->h_r_off.size() * 8, hipMemcpyHostToDevice,
+                              copy_stream));
+      KMST_HIP(hipMemcpyAsync(dvo, b->h_v_off.data(), b->h_v_off.size() * 4, hipMemcpyHostToDevice,
+                              copy_stream));
+      KMST_HIP(hipEventRecord(b->uploaded, copy_stream));
+      // the host vectors above must outlive the copies
+      KMST_HIP(hipEventSynchronize(b->uploaded));
+    });
+    if (st != KMST_OK) {
+      b->status = st;
+      b->error = kmst::g_last_error;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      ready_sets.push_back(b);
+    }
+    cv.notify_all();
+  }
+}
+
+void kmst_server::solve_alone(BufferSet& b) {
+  // a failed batch is re-solved request by request so that only the request
+  // that breaks the input contract reports the error
+  for (size_t i = 0; i < b.reqs.size(); ++i) {
+    Request& r = *b.reqs[i];
+    const u64 off = b.h_r_off[i];
+    kmst_dev_edges e{static_cast<u32*>(b.s.p) + off, static_cast<u32*>(b.d.p) + off,
+                     static_cast<u32*>(b.w.p) + off, r.m, r.n};
+    kmst_dev_mst o{static_cast<u32*>(b.out_s.p), static_cast<u32*>(b.out_d.p),
+                   static_cast<u32*>(b.out_w.p), std::max<u64>(r.n, 1)};
+    kmst_config c;
+    kmst_config_default(&c);
+    c.algorithm = cfg.algorithm;
+    c.filter_threshold = cfg.filter_threshold;
+    c.device = cfg.device;
+    // undo the batch offset of this request's records before solving alone
+    const u32 vo = b.h_v_off[i];
+    if (vo && r.m) {
+      const u64 rr[2] = {0, r.m};
+      const u32 neg = static_cast<u32>(0u - vo);
+      u64* dro = static_cast<u64*>(b.r_off.p);
+      u32* dvo = static_cast<u32*>(b.v_off.p);
+      (void)hipMemcpyAsync(dro, rr, 16, hipMemcpyHostToDevice, ctx->stream);
+      (void)hipMemcpyAsync(dvo, &neg, 4, hipMemcpyHostToDevice, ctx->stream);
+      batch_offset_kernel<<<static_cast<u32>(std::min<u64>((r.m + BLOCK - 1) / BLOCK, 65536)),
+                            BLOCK, 0, ctx->stream>>>(const_cast<u32*>(e.src),
+                                                     const_cast<u32*>(e.dst), r.m, dro, dvo, 1);
+      (void)hipStreamSynchronize(ctx->stream);
+    }
+    u64 cnt = 0, tw = 0;
+    r.res.status = kmst_solve_device(ctx.get(), &e, &c, &o, &cnt, &tw);
+    if (r.res.status == KMST_OK) {
+      r.res.mst_edges = cnt;
+      r.res.total_weight = tw;
+      if (cnt > r.out_cap) {
+        r.res.status = KMST_EINVAL;
+      } else if (cnt) {
+        (void)hipMemcpy(r.out_src, o.src, cnt * 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(r.out_dst, o.dst, cnt * 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(r.out_w, o.weight, cnt * 4, hipMemcpyDeviceToHost);
+      }
+    }
+  }
+}
+
+void kmst_server::compute_loop() {
+  (void)hipSetDevice(cfg.device);
+  while (true) {
+    BufferSet* b = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return stop || !ready_sets.empty(); });
+      if (ready_sets.empty()) return;
+      b = ready_sets.front();
+      ready_sets.pop_front();
+    }
+    const u64 t0 = now_ns();
+    for (auto& r : b->reqs) r->res.start_ns = t0;
+    if (b->status == KMST_OK && b->m > 0) {
+      kmst_config c;
+      kmst_config_default(&c);
+      c.algorithm = cfg.algorithm;
+      c.filter_threshold = cfg.filter_threshold;
+      c.device = cfg.device;
+      const kmst_status st = guard([&] {
+        if (b->reqs.size() > 1) {
+          batch_offset_kernel<<<static_cast<u32>(std::min<u64>((b->m + BLOCK - 1) / BLOCK, 65536)),
+                                BLOCK, 0, ctx->stream>>>(
+              static_cast<u32*>(b->s.p), static_cast<u32*>(b->d.p), b->m,
+              static_cast<const u64*>(b->r_off.p), static_cast<const u32*>(b->v_off.p),
+              static_cast<u32>(b->reqs.size()));
+          KMST_HIP(hipGetLastError());
+        }
+        kmst_dev_edges e{static_cast<u32*>(b->s.p), static_cast<u32*>(b->d.p),
+                         static_cast<u32*>(b->w.p), b->m, b->n};
+        kmst_dev_mst o{static_cast<u32*>(b->out_s.p), static_cast<u32*>(b->out_d.p),
+                       static_cast<u32*>(b->out_w.p), std::max<u64>(b->n, 1)};
+        const kmst_status s2 = kmst_solve_device(ctx.get(), &e, &c, &o, &b->mst_n, &b->mst_w);
+        if (s2 != KMST_OK) throw kmst::Error(s2, kmst_last_error());
+      });
+      if (st != KMST_OK) {
+        b->status = st;
+        b->error = kmst::g_last_error;
+      }
+      if (b->status != KMST_OK && b->status != KMST_EDEVICE && b->reqs.size() > 1) {
+        solve_alone(*b);
+        b->solved_individually = true;
+      }
+    } else if (b->status == KMST_OK) {
+      b->mst_n = 0;
+      b->mst_w = 0;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      done_sets.push_back(b);
+      ++batches;
+      batched_requests += b->reqs.size();
+    }
+    cv.notify_all();
+  }
+}
+
+void kmst_server::deliver_loop() {
+  (void)hipSetDevice(cfg.device);
+  while (true) {
+    BufferSet* b = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return stop || !done_sets.empty(); });
+      if (done_sets.empty()) return;
+      b = done_sets.front();
+      done_sets.pop_front();
+    }
+    if (b->status == KMST_OK && !b->solved_individually && b->mst_n > 0) {
+      const u64 k = b->mst_n;
+      const kmst_status st = guard([&] {
+        KMST_HIP(hipMemcpyAsync(b->h_out, b->out_s.p, k * 4, hipMemcpyDeviceToHost, down_stream));
+        KMST_HIP(hipMemcpyAsync(b->h_out + k, b->out_d.p, k * 4, hipMemcpyDeviceToHost, down_stream));
+        KMST_HIP(hipMemcpyAsync(b->h_out + 2 * k, b->out_w.p, k * 4, hipMemcpyDeviceToHost,
+                                down_stream));
+        KMST_HIP(hipStreamSynchronize(down_stream));
+      });
+      if (st != KMST_OK) {
+        b->status = st;
+        b->error = kmst::g_last_error;
+      }
+    }
+    // split the batch MST (sorted by min endpoint) by request id ranges
+    u64 pos = 0;
+    const u32* hs = b->h_out;
+    const u32* hd = b->h_out ? b->h_out + b->mst_n : nullptr;
+    const u32* hw = b->h_out ? b->h_out + 2 * b->mst_n : nullptr;
+    for (size_t i = 0; i < b->reqs.size(); ++i) {
+      Request& r = *b->reqs[i];
+      if (b->solved_individually) {  // results already written
+      } else if (b->status != KMST_OK) {
+        r.res.status = b->status;
+      } else {
+        const u32 lo = b->h_v_off[i];
+        const u64 hi = u64(lo) + r.n;
+        u64 cnt = 0, tw = 0;
+        const u64 start = pos;
+        while (pos < b->mst_n && hs[pos] < hi) ++pos;
+        cnt = pos - start;
+        if (cnt > r.out_cap) {
+          r.res.status = KMST_EINVAL;
+        } else {
+          for (u64 j = 0; j < cnt; ++j) {
+            r.out_src[j] = hs[start + j] - lo;
+            r.out_dst[j] = hd[start + j] - lo;
+            r.out_w[j] = hw[start + j];
+            tw += hw[start + j];
+          }
+          r.res.status = KMST_OK;
+          r.res.mst_edges = cnt;
+          r.res.total_weight = tw;
+        }
+      }
+      r.res.batch_requests = b->reqs.size();
+    }
+    const u64 t_done = now_ns();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto& r : b->reqs) {
+        r->res.done_ns = t_done;
+        r->res.done = 1;
+      }
+      b->reqs.clear();
+      free_sets.push_back(b);
+    }
+    cv.notify_all();
+  }
+}
+
+extern "C" {
+
+void kmst_server_config_default(kmst_server_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->device = 0;
+  c->algorithm = KMST_ALGO_FILTER_BORUVKA;
+  c->filter_threshold = 4;
+  c->max_batch_records = u64{1} << 27;
+  c->max_batch_requests = 4096;
+  c->hbm_budget_bytes = 0;
+  c->queue_capacity = 1u << 20;
+  c->buffer_sets = 3;
+}
+
+kmst_status kmst_server_create(const kmst_server_config* cfg, kmst_server** out) {
+  return guard([&] {
+    if (!cfg || !out) throw kmst::Error(KMST_EINVAL, "null argument");
+    if (cfg->max_batch_requests == 0 || cfg->max_batch_records == 0)
+      throw kmst::Error(KMST_EINVAL, "empty batch limits");
+    std::unique_ptr<kmst_server> s(new kmst_server);
+    s->cfg = *cfg;
+    if (s->cfg.buffer_sets < 2) s->cfg.buffer_sets = 2;
+    KMST_HIP(hipSetDevice(cfg->device));
+    if (s->cfg.hbm_budget_bytes == 0) {
+      size_t free_b = 0, total_b = 0;
+      KMST_HIP(hipMemGetInfo(&free_b, &total_b));
+      s->cfg.hbm_budget_bytes = free_b / 2;
+    }
+    // admission bound: a batch must fit the budget together with the other
+    // buffer sets' uploads
+    const u64 max_rec = s->cfg.hbm_budget_bytes / BYTES_PER_RECORD;
+    s->cfg.max_batch_records = std::min<u64>(s->cfg.max_batch_records, max_rec);
+    s->ctx.reset(new kmst_context(cfg->device));
+    KMST_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    KMST_HIP(hipStreamCreateWithFlags(&s->down_stream, hipStreamNonBlocking));
+    for (u32 i = 0; i < s->cfg.buffer_sets; ++i) {
+      s->sets.emplace_back(new BufferSet);
+      KMST_HIP(hipEventCreateWithFlags(&s->sets.back()->uploaded, hipEventDisableTiming));
+      s->free_sets.push_back(s->sets.back().get());
+    }
+    kmst_server* raw = s.get();
+    raw->admit_th = std::thread([raw] { raw->admit_loop(); });
+    raw->compute_th = std::thread([raw] { raw->compute_loop(); });
+    raw->deliver_th = std::thread([raw] { raw->deliver_loop(); });
+    *out = s.release();
+  });
+}
+
+kmst_status kmst_server_submit(kmst_server* s, const uint32_t* src, const uint32_t* dst,
+                               const uint32_t* weight, uint64_t m_directed, uint64_t n,
+                               uint32_t* out_src, uint32_t* out_dst, uint32_t* out_weight,
+                               uint64_t out_capacity, uint64_t* ticket) {
+  return guard([&] {
+    if (!s || !ticket) throw kmst::Error(KMST_EINVAL, "null argument");
+    if (m_directed && (!src || !dst || !weight)) throw kmst::Error(KMST_EINVAL, "null input");
+    if (n >= (u64{1} << 31)) throw kmst::Error(KMST_EINVAL, "n >= 2^31");
+    if (m_directed > s->cfg.max_batch_records || footprint(m_directed, n) > s->cfg.hbm_budget_bytes)
+      throw kmst::Error(KMST_ENOMEM, "request exceeds the HBM admission budget");
+    auto r = std::make_shared<Request>();
+    r->src = src;
+    r->dst = dst;
+    r->w = weight;
+    r->m = m_directed;
+    r->n = n;
+    r->out_src = out_src;
+    r->out_dst = out_dst;
+    r->out_w = out_weight;
+    r->out_cap = out_capacity;
+    r->res.submit_ns = now_ns();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->queue.size() >= s->cfg.queue_capacity) throw kmst::Error(KMST_EBUSY, "queue full");
+      r->ticket = s->next_ticket++;
+      r->res.ticket = r->ticket;
+      s->queue.push_back(r);
+      s->pending[r->ticket] = r;
+    }
+    s->cv.notify_all();
+    *ticket = r->ticket;
+  });
+}
+
+kmst_status kmst_server_poll(kmst_server* s, uint64_t ticket, kmst_request_result* res) {
+  return guard([&] {
+    if (!s || !res) throw kmst::Error(KMST_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->pending.find(ticket);
+    if (it == s->pending.end()) throw kmst::Error(KMST_EINVAL, "unknown ticket");
+    *res = it->second->res;
+    if (res->done) s->pending.erase(it);
+  });
+}
+
+kmst_status kmst_server_wait(kmst_server* s, uint64_t ticket, kmst_request_result* res) {
+  return guard([&] {
+    if (!s || !res) throw kmst::Error(KMST_EINVAL, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->pending.find(ticket);
+    if (it == s->pending.end()) throw kmst::Error(KMST_EINVAL, "unknown ticket");
+    std::shared_ptr<Request> r = it->second;
+    s->cv.wait(lk, [&] { return r->res.done != 0; });
+    *res = r->res;
+    s->pending.erase(ticket);
+  });
+}
+
+kmst_status kmst_server_counters(kmst_server* s, uint64_t* batches, uint64_t* requests) {
+  return guard([&] {
+    if (!s) throw kmst::Error(KMST_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (batches) *batches = s->batches;
+    if (requests) *requests = s->batched_requests;
+  });
+}
+
+void kmst_server_destroy(kmst_server* s) {
+  if (!s) return;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    // drain: wait until every admitted or queued request has completed
+    s->cv.wait(lk, [&] {
+      return s->queue.empty() && s->ready_sets.empty() && s->done_sets.empty() &&
+             s->free_sets.size() == s->sets.size();
+    });
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  s->admit_th.join();
+  s->compute_th.join();
+  s->deliver_th.join();
+  (void)hipSetDevice(s->cfg.device);
+  for (auto& b : s->sets) {
+    for (DevBuf* d : {&b->s, &b->d, &b->w, &b->out_s, &b->out_d, &b->out_w, &b->r_off, &b->v_off})
+      d->release();
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->uploaded) (void)hipEventDestroy(b->uploaded);
+  }
+  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+  if (s->down_stream) (void)hipStreamDestroy(s->down_stream);
+  delete s;
+}
+
+}  // extern "C"

@@ -80,3 +80,13 @@ def test_gnm_rejects_bad_args():
         k.gen_gnm_host(4, 7, 254, 1)  # more than n(n-1)/2 pairs
     with pytest.raises(k.KmstError):
         k.gen_gnm_host(4, 2, 0, 1)
+
+
+def test_driver_help_without_gpu():
+    import subprocess
+    exe = os.path.join(ROOT, "kamsta_amd", "mst_benchmarks")
+    assert os.path.exists(exe), "built by kamsta_amd.build"
+    r = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 0 and "--filter_threshold" in r.stdout
+    r = subprocess.run([exe, "--bogus"], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 2

@@ -1,0 +1,82 @@
+"""Batched serving: every request's result is identical alone and in the mix,
+and equals the oracle's Kruskal."""
+import numpy as np
+import pytest
+
+import graphs
+import kamsta_amd as k
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def requests(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(2 ** rng.uniform(4, 12))
+        deg = rng.uniform(2, 12)
+        m = int(min(n * deg / 2, n * (n - 1) // 2))
+        s, d, w = k.gen_gnm_host(n, m, 254, 1000 * seed + i) if m else (np.zeros(0, np.uint32),) * 3
+        out.append((s, d, w, n))
+    return out
+
+
+def solve_all(server, reqs):
+    tickets = [server.submit(s, d, w, n) for s, d, w, n in reqs]
+    return [server.wait(t) for t in tickets]
+
+
+@pytest.mark.parametrize("algo", [k.BORUVKA, k.FILTER_BORUVKA])
+def test_batched_equals_alone_and_oracle(gpu, algo):
+    reqs = requests(60, 3)
+    batched = k.Server(algorithm=algo, max_batch_requests=4096)
+    alone = k.Server(algorithm=algo, max_batch_requests=1)
+    rb = solve_all(batched, reqs)
+    ra = solve_all(alone, reqs)
+    nb, qb = batched.counters()
+    batched.close()
+    alone.close()
+    assert qb == len(reqs) and nb < len(reqs), "requests were batched"
+    assert max(r[0]["batch_requests"] for r in rb) > 1
+    for (s, d, w, n), (resb, eb), (resa, ea) in zip(reqs, rb, ra):
+        assert resb["status"] == 0 and resa["status"] == 0
+        assert resb["total_weight"] == resa["total_weight"]
+        for x, y in zip(eb, ea):
+            assert np.array_equal(x, y)
+        if len(s):
+            exp, exp_w = oracle.kruskal(k.to_wedges(s, d, w))
+            assert resb["total_weight"] == exp_w
+            assert np.array_equal(eb[0], exp["src"]) and np.array_equal(eb[1], exp["dst"])
+            assert np.array_equal(eb[2], exp["weight"])
+        assert resb["done_ns"] >= resb["start_ns"] >= resb["admit_ns"] >= resb["submit_ns"]
+
+
+def test_bad_request_does_not_poison_batch(gpu):
+    reqs = requests(12, 5)
+    s, d, w, n = reqs[4]
+    bad = (s.copy(), d.copy(), w.copy(), n)
+    bad[2][0] += 1  # asymmetric weight
+    reqs[4] = bad
+    srv = k.Server(max_batch_requests=4096)
+    res = solve_all(srv, reqs)
+    srv.close()
+    for i, ((s, d, w, n), (r, e)) in enumerate(zip(reqs, res)):
+        if i == 4:
+            assert r["status"] == k.KMST_EASYMMETRIC
+        else:
+            assert r["status"] == 0
+            if len(s):
+                assert r["total_weight"] == oracle.kruskal(k.to_wedges(s, d, w))[1]
+
+
+def test_admission_rejects_oversize(gpu):
+    srv = k.Server(max_batch_records=1000)
+    s, d, w = k.gen_gnm_host(200, 800, 254, 1)
+    with pytest.raises(k.KmstError) as e:
+        srv.submit(s, d, w, 200)
+    assert e.value.status == k.KMST_ENOMEM
+    s, d, w = k.gen_gnm_host(100, 300, 254, 2)
+    r, edges = srv.wait(srv.submit(s, d, w, 100))
+    assert r["status"] == 0 and r["total_weight"] == oracle.kruskal(k.to_wedges(s, d, w))[1]
+    srv.close()
