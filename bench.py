@@ -88,13 +88,19 @@ def serving(args):
     pv = pool.numpy().view(np.uint32)
     reqs = [(pv[0, a:a + 2 * m], pv[1, a:a + 2 * m], pv[2, a:a + 2 * m], n)
             for a, (n, m) in zip(offs, specs)]
+    # result buffers: one pinned pool per run, a slice of n entries per request
+    out_total = sum(max(n, 1) for n, _ in specs)
+    out_offs = np.concatenate([[0], np.cumsum([max(n, 1) for n, _ in specs])])
     gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
 
     def run(max_batch_requests):
         srv = k.Server(algorithm=k.FILTER_BORUVKA, filter_threshold=args.filter_threshold,
                        max_batch_requests=max_batch_requests)
+        outp = torch.empty((3, out_total), dtype=torch.int32, pin_memory=True).numpy().view(np.uint32)
+        outs = [tuple(outp[j, out_offs[i]:out_offs[i + 1]] for j in range(3))
+                for i in range(len(reqs))]
         # warm the server's buffers and kernels on a few requests
-        for t in [srv.submit(*reqs[i]) for i in range(8)]:
+        for t in [srv.submit(*reqs[i], out=outs[i]) for i in range(8)]:
             srv.wait(t)
         tickets = [None] * len(reqs)
         arrivals = np.zeros(len(reqs), dtype=np.int64)
@@ -106,7 +112,7 @@ def serving(args):
                 while time.monotonic_ns() < sched[i]:
                     pass
                 arrivals[i] = sched[i]
-                tickets[i] = srv.submit(*r)
+                tickets[i] = srv.submit(*r, out=outs[i])
 
         th = threading.Thread(target=submitter)
         th.start()
@@ -119,6 +125,19 @@ def serving(args):
         nb, nq = srv.counters()
         srv.close()
         done = np.array([r[0]["done_ns"] for r in results], dtype=np.int64)
+        if max_batch_requests > 1:
+            by = {}
+            for (res, _), (_, _, _, nn), a in zip(results, reqs, arrivals):
+                key = res["start_ns"]
+                e = by.setdefault(key, [res["admit_ns"], res["start_ns"], res["done_ns"], 0, 0,
+                                        res["solved_ns"]])
+                e[3] += 1
+                e[4] += nn
+            t0b = min(v[0] for v in by.values())
+            for v in sorted(by.values()):
+                log(f"[serve]   batch: {v[3]} req, {v[4]} vertices, admit {(v[0]-t0b)/1e6:.1f} ms, "
+                    f"start +{(v[1]-v[0])/1e6:.1f} ms, solve {(v[5]-v[1])/1e6:.1f} ms, "
+                    f"deliver {(v[2]-v[5])/1e6:.1f} ms")
         lat = (done - arrivals) / 1e6
         span = (done.max() - arrivals.min()) / 1e9
         bad = sum(r[0]["status"] != 0 for r in results)

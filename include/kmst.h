@@ -164,7 +164,7 @@ typedef struct kmst_request_result {
   uint64_t mst_edges;
   uint64_t total_weight;
   uint64_t batch_requests;       /* size of the batch it was solved in */
-  uint64_t submit_ns, admit_ns, start_ns, done_ns;  /* steady clock */
+  uint64_t submit_ns, admit_ns, start_ns, solved_ns, done_ns;  /* steady clock */
 } kmst_request_result;
 
 void kmst_server_config_default(kmst_server_config* cfg);

@@ -76,7 +76,7 @@ class _RequestResult(ctypes.Structure):
                 ("mst_edges", ctypes.c_uint64), ("total_weight", ctypes.c_uint64),
                 ("batch_requests", ctypes.c_uint64), ("submit_ns", ctypes.c_uint64),
                 ("admit_ns", ctypes.c_uint64), ("start_ns", ctypes.c_uint64),
-                ("done_ns", ctypes.c_uint64)]
+                ("solved_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64)]
 
 
 class _DevMst(ctypes.Structure):
@@ -277,15 +277,25 @@ class Server:
         _check(self._lib.kmst_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         self._live = {}
 
-    def submit(self, src, dst, weight, n):
-        src = np.ascontiguousarray(src, dtype=np.uint32)
-        dst = np.ascontiguousarray(dst, dtype=np.uint32)
-        weight = np.ascontiguousarray(weight, dtype=np.uint32)
+    def submit(self, src, dst, weight, n, out=None):
+        """src/dst/weight: numpy uint32 arrays (host) or torch tensors (host
+        or device); out: optional three uint32 numpy arrays of >= max(n, 1)
+        entries (e.g. views of a pinned pool) that receive the result."""
+        def arr(x):
+            if hasattr(x, "data_ptr"):
+                return x
+            return np.ascontiguousarray(x, dtype=np.uint32)
+
+        def ptr(x):
+            return x.data_ptr() if hasattr(x, "data_ptr") else x.ctypes.data
+
+        src, dst, weight = arr(src), arr(dst), arr(weight)
         cap = max(int(n), 1)
-        outs = [np.empty(cap, np.uint32) for _ in range(3)]
+        outs = list(out) if out is not None else [np.empty(cap, np.uint32) for _ in range(3)]
+        assert all(len(o) >= cap for o in outs)
         t = ctypes.c_uint64(0)
         _check(self._lib.kmst_server_submit(
-            self._h, src.ctypes.data, dst.ctypes.data, weight.ctypes.data, len(src), int(n),
+            self._h, ptr(src), ptr(dst), ptr(weight), len(src), int(n),
             outs[0].ctypes.data, outs[1].ctypes.data, outs[2].ctypes.data, cap, ctypes.byref(t)))
         self._live[t.value] = (src, dst, weight, outs)
         return t.value
@@ -296,7 +306,7 @@ class Server:
         _, _, _, outs = self._live.pop(ticket)
         k = r.mst_edges if r.status == KMST_OK else 0
         res = {f: getattr(r, f) for f, _ in _RequestResult._fields_}
-        return res, (outs[0][:k].copy(), outs[1][:k].copy(), outs[2][:k].copy())
+        return res, (outs[0][:k], outs[1][:k], outs[2][:k])
 
     def counters(self):
         b, q = ctypes.c_uint64(0), ctypes.c_uint64(0)

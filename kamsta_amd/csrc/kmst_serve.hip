@@ -62,6 +62,48 @@ __global__ __launch_bounds__(BLOCK) void batch_offset_kernel(u32* __restrict__ s
   }
 }
 
+// bounds[r] = first MST edge of request r (edges sorted by min endpoint,
+// request r owns vertex ids [v_off[r], v_off[r+1])); bounds[count] = k.
+__global__ __launch_bounds__(BLOCK) void batch_bounds_kernel(const u32* __restrict__ out_s, u64 k,
+                                                             const u32* __restrict__ v_off,
+                                                             u32 count, u64* __restrict__ bounds) {
+  const u32 r = blockIdx.x * BLOCK + threadIdx.x;
+  if (r > count) return;
+  if (r == count) {
+    bounds[r] = k;
+    return;
+  }
+  const u32 x = v_off[r];
+  u64 lo = 0, hi = k;
+  while (lo < hi) {
+    const u64 mid = (lo + hi) >> 1;
+    if (out_s[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  bounds[r] = lo;
+}
+
+// Back to request-local ids; per-request weight sums (one block per request).
+__global__ __launch_bounds__(BLOCK) void batch_unoffset_kernel(u32* __restrict__ out_s,
+                                                               u32* __restrict__ out_d,
+                                                               const u32* __restrict__ out_w,
+                                                               const u64* __restrict__ bounds,
+                                                               const u32* __restrict__ v_off,
+                                                               u64* __restrict__ totals) {
+  __shared__ u64 lds[WAVES];
+  const u32 r = blockIdx.x;
+  const u64 b0 = bounds[r], b1 = bounds[r + 1];
+  const u32 o = v_off[r];
+  u64 sum = 0;
+  for (u64 i = b0 + threadIdx.x; i < b1; i += BLOCK) {
+    out_s[i] -= o;
+    out_d[i] -= o;
+    sum += out_w[i];
+  }
+  u64 total;
+  block_exclusive_sum(sum, total, lds);
+  if (threadIdx.x == 0) totals[r] = total;
+}
+
 struct Request {
   u64 ticket = 0;
   const u32 *src = nullptr, *dst = nullptr, *w = nullptr;
@@ -72,11 +114,11 @@ struct Request {
 };
 
 struct BufferSet {
-  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off;
+  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off, bounds;
   std::vector<u64> h_r_off;
   std::vector<u32> h_v_off;
-  u32* h_out = nullptr;  // pinned: 3 x capacity words
-  u64 h_out_cap = 0;
+  u64* h_bounds = nullptr;  // pinned: 2 x (requests + 1) words (bounds, totals)
+  u64 h_bounds_cap = 0;
   hipEvent_t uploaded = nullptr;
   std::vector<std::shared_ptr<Request>> reqs;
   u64 m = 0, n = 0;
@@ -93,7 +135,10 @@ using namespace kmst;
 struct kmst_server {
   kmst_server_config cfg{};
   std::unique_ptr<kmst_context> ctx;       // compute context (its own stream)
-  hipStream_t copy_stream = nullptr;       // uploads
+  static constexpr int UPLOAD_STREAMS = 4;  // concurrent DMA engines for uploads
+  hipStream_t copy_stream = nullptr;       // uploads (first of up_streams)
+  hipStream_t up_streams[UPLOAD_STREAMS] = {};
+  hipEvent_t up_events[UPLOAD_STREAMS] = {};
   hipStream_t down_stream = nullptr;       // result downloads
   std::vector<std::unique_ptr<BufferSet>> sets;
   std::mutex mu;
@@ -161,7 +206,7 @@ void kmst_server::admit_loop() {
     const u64 t_admit = now_ns();
     u64 m = 0, n = 0;
     b->h_r_off.assign(take.size() + 1, 0);
-    b->h_v_off.assign(take.size(), 0);
+    b->h_v_off.assign(take.size() + 1, 0);
     for (size_t i = 0; i < take.size(); ++i) {
       b->h_r_off[i] = m;
       b->h_v_off[i] = static_cast<u32>(n);
@@ -170,6 +215,7 @@ void kmst_server::admit_loop() {
       take[i]->res.admit_ns = t_admit;
     }
     b->h_r_off[take.size()] = m;
+    b->h_v_off[take.size()] = static_cast<u32>(n);
     b->m = m;
     b->n = n;
     b->reqs = std::move(take);
@@ -186,19 +232,48 @@ void kmst_server::admit_loop() {
       b->out_w.get(nn * 4);
       u64* dro = static_cast<u64*>(b->r_off.get((b->reqs.size() + 1) * 8));
       u32* dvo = static_cast<u32*>(b->v_off.get(b->reqs.size() * 4 + 4));
-      if (b->h_out_cap < 3 * nn) {
-        if (b->h_out) (void)hipHostFree(b->h_out);
-        b->h_out = nullptr;
-        KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_out), 3 * nn * 4, hipHostMallocDefault));
-        b->h_out_cap = 3 * nn;
+      b->bounds.get(2 * (b->reqs.size() + 1) * 8);
+      if (b->h_bounds_cap < 2 * (b->reqs.size() + 1)) {
+        if (b->h_bounds) (void)hipHostFree(b->h_bounds);
+        b->h_bounds = nullptr;
+        const u64 cap = 2 * (b->reqs.size() + 1) + 1024;
+        KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_bounds), cap * 8, hipHostMallocDefault));
+        b->h_bounds_cap = cap;
       }
-      for (size_t i = 0; i < b->reqs.size(); ++i) {
-        const Request& r = *b->reqs[i];
-        const u64 off = b->h_r_off[i];
-        if (r.m == 0) continue;
-        KMST_HIP(hipMemcpyAsync(ds + off, r.src, r.m * 4, hipMemcpyHostToDevice, copy_stream));
-        KMST_HIP(hipMemcpyAsync(dd + off, r.dst, r.m * 4, hipMemcpyHostToDevice, copy_stream));
-        KMST_HIP(hipMemcpyAsync(dw + off, r.w, r.m * 4, hipMemcpyHostToDevice, copy_stream));
+      // One copy per run of requests whose host arrays are adjacent in
+      // memory (and so are their device slots), spread over several
+      // streams so that more than one DMA engine moves data.
+      int next_stream = 0;
+      auto upload = [&](u32* dst_dev, const u32* src_host, u64 count) {
+        if (count == 0) return;
+        // host (pinned or pageable) or device-resident request arrays
+        KMST_HIP(hipMemcpyAsync(dst_dev, src_host, count * 4, hipMemcpyDefault,
+                                up_streams[next_stream]));
+        next_stream = (next_stream + 1) % UPLOAD_STREAMS;
+      };
+      for (int a = 0; a < 3; ++a) {
+        u32* dbase = a == 0 ? ds : (a == 1 ? dd : dw);
+        const u32* run_src = nullptr;
+        u64 run_off = 0, run_len = 0;
+        for (size_t i = 0; i < b->reqs.size(); ++i) {
+          const Request& r = *b->reqs[i];
+          const u32* h = a == 0 ? r.src : (a == 1 ? r.dst : r.w);
+          if (r.m == 0) continue;
+          if (run_src && h == run_src + run_len && b->h_r_off[i] == run_off + run_len &&
+              run_len + r.m <= (u64{1} << 24)) {
+            run_len += r.m;
+          } else {
+            upload(dbase + run_off, run_src, run_len);
+            run_src = h;
+            run_off = b->h_r_off[i];
+            run_len = r.m;
+          }
+        }
+        upload(dbase + run_off, run_src, run_len);
+      }
+      for (int q = 1; q < UPLOAD_STREAMS; ++q) {
+        KMST_HIP(hipEventRecord(up_events[q], up_streams[q]));
+        KMST_HIP(hipStreamWaitEvent(copy_stream, up_events[q], 0));
       }
       KMST_HIP(hipMemcpyAsync(dro, b->h_r_off.data(), b->h_r_off.size() * 8, hipMemcpyHostToDevice,
                               copy_stream));
@@ -312,6 +387,8 @@ void kmst_server::compute_loop() {
       b->mst_n = 0;
       b->mst_w = 0;
     }
+    const u64 t1 = now_ns();
+    for (auto& r : b->reqs) r->res.solved_ns = t1;
     {
       std::lock_guard<std::mutex> lk(mu);
       done_sets.push_back(b);
@@ -333,53 +410,55 @@ void kmst_server::deliver_loop() {
       b = done_sets.front();
       done_sets.pop_front();
     }
-    if (b->status == KMST_OK && !b->solved_individually && b->mst_n > 0) {
-      const u64 k = b->mst_n;
+    const u32 count = static_cast<u32>(b->reqs.size());
+    if (b->status == KMST_OK && !b->solved_individually) {
+      // split on the device: request ranges, local ids, weight sums; then
+      // every request's slice goes straight into its output arrays
       const kmst_status st = guard([&] {
-        KMST_HIP(hipMemcpyAsync(b->h_out, b->out_s.p, k * 4, hipMemcpyDeviceToHost, down_stream));
-        KMST_HIP(hipMemcpyAsync(b->h_out + k, b->out_d.p, k * 4, hipMemcpyDeviceToHost, down_stream));
-        KMST_HIP(hipMemcpyAsync(b->h_out + 2 * k, b->out_w.p, k * 4, hipMemcpyDeviceToHost,
+        const u64 k = b->mst_n;
+        u64* dbounds = static_cast<u64*>(b->bounds.p);
+        u64* dtotals = dbounds + (count + 1);
+        batch_bounds_kernel<<<(count + BLOCK) / BLOCK, BLOCK, 0, down_stream>>>(
+            static_cast<const u32*>(b->out_s.p), k, static_cast<const u32*>(b->v_off.p), count,
+            dbounds);
+        KMST_HIP(hipGetLastError());
+        batch_unoffset_kernel<<<count, BLOCK, 0, down_stream>>>(
+            static_cast<u32*>(b->out_s.p), static_cast<u32*>(b->out_d.p),
+            static_cast<const u32*>(b->out_w.p), dbounds, static_cast<const u32*>(b->v_off.p),
+            dtotals);
+        KMST_HIP(hipGetLastError());
+        KMST_HIP(hipMemcpyAsync(b->h_bounds, dbounds, 2 * (count + 1) * 8, hipMemcpyDeviceToHost,
                                 down_stream));
         KMST_HIP(hipStreamSynchronize(down_stream));
-      });
-      if (st != KMST_OK) {
-        b->status = st;
-        b->error = kmst::g_last_error;
-      }
-    }
-    // split the batch MST (sorted by min endpoint) by request id ranges
-    u64 pos = 0;
-    const u32* hs = b->h_out;
-    const u32* hd = b->h_out ? b->h_out + b->mst_n : nullptr;
-    const u32* hw = b->h_out ? b->h_out + 2 * b->mst_n : nullptr;
-    for (size_t i = 0; i < b->reqs.size(); ++i) {
-      Request& r = *b->reqs[i];
-      if (b->solved_individually) {  // results already written
-      } else if (b->status != KMST_OK) {
-        r.res.status = b->status;
-      } else {
-        const u32 lo = b->h_v_off[i];
-        const u64 hi = u64(lo) + r.n;
-        u64 cnt = 0, tw = 0;
-        const u64 start = pos;
-        while (pos < b->mst_n && hs[pos] < hi) ++pos;
-        cnt = pos - start;
-        if (cnt > r.out_cap) {
-          r.res.status = KMST_EINVAL;
-        } else {
-          for (u64 j = 0; j < cnt; ++j) {
-            r.out_src[j] = hs[start + j] - lo;
-            r.out_dst[j] = hd[start + j] - lo;
-            r.out_w[j] = hw[start + j];
-            tw += hw[start + j];
+        const u64* hb = b->h_bounds;
+        const u64* ht = b->h_bounds + (count + 1);
+        for (u32 i = 0; i < count; ++i) {
+          Request& r = *b->reqs[i];
+          const u64 cnt = hb[i + 1] - hb[i];
+          if (cnt > r.out_cap) {
+            r.res.status = KMST_EINVAL;
+            continue;
           }
           r.res.status = KMST_OK;
           r.res.mst_edges = cnt;
-          r.res.total_weight = tw;
+          r.res.total_weight = ht[i];
+          if (cnt == 0) continue;
+          KMST_HIP(hipMemcpyAsync(r.out_src, static_cast<u32*>(b->out_s.p) + hb[i], cnt * 4,
+                                  hipMemcpyDefault, down_stream));
+          KMST_HIP(hipMemcpyAsync(r.out_dst, static_cast<u32*>(b->out_d.p) + hb[i], cnt * 4,
+                                  hipMemcpyDefault, down_stream));
+          KMST_HIP(hipMemcpyAsync(r.out_w, static_cast<u32*>(b->out_w.p) + hb[i], cnt * 4,
+                                  hipMemcpyDefault, down_stream));
         }
+        KMST_HIP(hipStreamSynchronize(down_stream));
+      });
+      if (st != KMST_OK) {
+        for (auto& r : b->reqs) r->res.status = st;
       }
-      r.res.batch_requests = b->reqs.size();
+    } else if (!b->solved_individually) {
+      for (auto& r : b->reqs) r->res.status = b->status;
     }
+    for (auto& r : b->reqs) r->res.batch_requests = count;
     const u64 t_done = now_ns();
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -428,7 +507,11 @@ kmst_status kmst_server_create(const kmst_server_config* cfg, kmst_server** out)
     const u64 max_rec = s->cfg.hbm_budget_bytes / BYTES_PER_RECORD;
     s->cfg.max_batch_records = std::min<u64>(s->cfg.max_batch_records, max_rec);
     s->ctx.reset(new kmst_context(cfg->device));
-    KMST_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    for (int q = 0; q < kmst_server::UPLOAD_STREAMS; ++q) {
+      KMST_HIP(hipStreamCreateWithFlags(&s->up_streams[q], hipStreamNonBlocking));
+      KMST_HIP(hipEventCreateWithFlags(&s->up_events[q], hipEventDisableTiming));
+    }
+    s->copy_stream = s->up_streams[0];
     KMST_HIP(hipStreamCreateWithFlags(&s->down_stream, hipStreamNonBlocking));
     for (u32 i = 0; i < s->cfg.buffer_sets; ++i) {
       s->sets.emplace_back(new BufferSet);
@@ -527,12 +610,16 @@ void kmst_server_destroy(kmst_server* s) {
   s->deliver_th.join();
   (void)hipSetDevice(s->cfg.device);
   for (auto& b : s->sets) {
-    for (DevBuf* d : {&b->s, &b->d, &b->w, &b->out_s, &b->out_d, &b->out_w, &b->r_off, &b->v_off})
+    for (DevBuf* d : {&b->s, &b->d, &b->w, &b->out_s, &b->out_d, &b->out_w, &b->r_off, &b->v_off,
+                      &b->bounds})
       d->release();
-    if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->h_bounds) (void)hipHostFree(b->h_bounds);
     if (b->uploaded) (void)hipEventDestroy(b->uploaded);
   }
-  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+  for (int q = 0; q < kmst_server::UPLOAD_STREAMS; ++q) {
+    if (s->up_streams[q]) (void)hipStreamDestroy(s->up_streams[q]);
+    if (s->up_events[q]) (void)hipEventDestroy(s->up_events[q]);
+  }
   if (s->down_stream) (void)hipStreamDestroy(s->down_stream);
   delete s;
 }

@@ -86,6 +86,9 @@ void* DevBuf::get(size_t need) {
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    // grow with headroom so that a stream of slightly larger requests does
+    // not reallocate (hipFree synchronises the device) on every call
+    if (need < (size_t{1} << 40)) need += need / 4;
     if (hipMalloc(&p, need) != hipSuccess) {
       (void)hipGetLastError();
       p = nullptr;
