@@ -44,6 +44,8 @@ def parse():
                    help="batched-serving benchmark instead of the one-graph throughput run")
     p.add_argument("--requests", type=int, default=4096)
     p.add_argument("--rate", type=float, default=20000.0, help="Poisson arrival rate, req/s")
+    p.add_argument("--device-inputs", action="store_true",
+                   help="serving: request graphs resident in HBM instead of pinned host memory")
     p.add_argument("--verify-full", action="store_true",
                    help="also run the oracle's Kruskal on the full input (minutes of CPU)")
     return p.parse_args()
@@ -70,7 +72,10 @@ def serving(args):
         specs.append((n, m))
     total = sum(2 * m for _, m in specs)
     log(f"[serve] {args.requests} requests, {total} directed records, generating on device")
-    pool = torch.empty((3, total), dtype=torch.int32, pin_memory=True)
+    if args.device_inputs:
+        pool = torch.empty((3, total), dtype=torch.int32, device=dev)
+    else:
+        pool = torch.empty((3, total), dtype=torch.int32, pin_memory=True)
     t0 = time.time()
     offs = []
     o = 0
@@ -85,9 +90,13 @@ def serving(args):
     torch.cuda.synchronize()
     del scratch
     log(f"[serve] generated in {time.time() - t0:.1f}s")
-    pv = pool.numpy().view(np.uint32)
-    reqs = [(pv[0, a:a + 2 * m], pv[1, a:a + 2 * m], pv[2, a:a + 2 * m], n)
-            for a, (n, m) in zip(offs, specs)]
+    if args.device_inputs:
+        reqs = [(pool[0, a:a + 2 * m], pool[1, a:a + 2 * m], pool[2, a:a + 2 * m], n)
+                for a, (n, m) in zip(offs, specs)]
+    else:
+        pv = pool.numpy().view(np.uint32)
+        reqs = [(pv[0, a:a + 2 * m], pv[1, a:a + 2 * m], pv[2, a:a + 2 * m], n)
+                for a, (n, m) in zip(offs, specs)]
     # result buffers: one pinned pool per run, a slice of n entries per request
     out_total = sum(max(n, 1) for n, _ in specs)
     out_offs = np.concatenate([[0], np.cumsum([max(n, 1) for n, _ in specs])])
@@ -162,7 +171,8 @@ def serving(args):
         "dtype": "u32/u64",
         "data": f"synthetic: {args.requests} GNM requests, n log-uniform [2^8,2^18], "
                 f"avg degree U[8,32], weights U[1,254], Poisson arrivals {args.rate:g}/s, "
-                f"inputs in pinned host memory (latency includes H2D and D2H)",
+                + ("inputs resident in HBM (latency includes D2H of results)" if args.device_inputs
+                   else "inputs in pinned host memory (latency includes H2D and D2H)"),
         "config": {"workload": "batched serving", "requests": args.requests, "rate": args.rate,
                    "directed_records": total, "algorithm": "filter_boruvka"},
         "batched": {kk: (round(v, 3) if isinstance(v, float) else v) for kk, v in batched.items()},

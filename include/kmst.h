@@ -92,6 +92,9 @@ kmst_status kmst_context_create(int device, kmst_context** out);
 void kmst_context_destroy(kmst_context* ctx);
 /* Frees the context's cached device workspaces. */
 void kmst_context_trim(kmst_context* ctx);
+/* Sizes the context's workspaces for solves of up to m_directed records and
+ * n vertices, so that such solves allocate no device memory. */
+kmst_status kmst_context_reserve(kmst_context* ctx, uint64_t m_directed, uint64_t n);
 
 /* Host edge list in, host MST out.  `out` must hold at least
  * (number of distinct vertices - 1) entries; m_directed / 2 always suffices.

@@ -39,6 +39,7 @@ BORUVKA, FILTER_BORUVKA = 0, 1
 # every symbol include/kmst.h declares
 EXPORTED_SYMBOLS = [
     "kmst_config_default", "kmst_context_create", "kmst_context_destroy", "kmst_context_trim",
+    "kmst_context_reserve",
     "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_context_set_timing",
     "kmst_last_error", "kmst_context_stats_json", "kmst_gen_gnm_device", "kmst_gen_gnm_host",
     "kmst_k_segmented_argmin", "kmst_k_sort_pairs", "kmst_k_pointer_jump",
@@ -116,6 +117,7 @@ def load():
         "kmst_context_create": (i32, [i32, ctypes.POINTER(vp)]),
         "kmst_context_destroy": (None, [vp]),
         "kmst_context_trim": (None, [vp]),
+        "kmst_context_reserve": (i32, [vp, u64, u64]),
         "kmst_boruvka": (i32, [vp, u64, vp, vp, pu64, pu64]),
         "kmst_filter_boruvka": (i32, [vp, u64, vp, vp, pu64, pu64]),
         "kmst_solve_device": (i32, [vp, vp, vp, vp, pu64, pu64]),
@@ -213,6 +215,10 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def reserve(self, m_directed, n):
+        """Pre-size the workspaces for solves up to this size."""
+        _check(self._lib.kmst_context_reserve(self._h, m_directed, n))
 
     def set_timing(self, on=True):
         self._lib.kmst_context_set_timing(self._h, 1 if on else 0)

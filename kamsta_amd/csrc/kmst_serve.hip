@@ -507,6 +507,11 @@ kmst_status kmst_server_create(const kmst_server_config* cfg, kmst_server** out)
     const u64 max_rec = s->cfg.hbm_budget_bytes / BYTES_PER_RECORD;
     s->cfg.max_batch_records = std::min<u64>(s->cfg.max_batch_records, max_rec);
     s->ctx.reset(new kmst_context(cfg->device));
+    // reserve the solver workspaces and upload buffers for full batches up
+    // front (admission by footprint: a batch never grows them)
+    const u64 n_res = std::max<u64>(s->cfg.max_batch_records / 4, 1024);
+    if (kmst_context_reserve(s->ctx.get(), s->cfg.max_batch_records, n_res) != KMST_OK)
+      throw kmst::Error(KMST_ENOMEM, std::string("reserving batch workspaces: ") + kmst_last_error());
     for (int q = 0; q < kmst_server::UPLOAD_STREAMS; ++q) {
       KMST_HIP(hipStreamCreateWithFlags(&s->up_streams[q], hipStreamNonBlocking));
       KMST_HIP(hipEventCreateWithFlags(&s->up_events[q], hipEventDisableTiming));
@@ -515,6 +520,9 @@ kmst_status kmst_server_create(const kmst_server_config* cfg, kmst_server** out)
     KMST_HIP(hipStreamCreateWithFlags(&s->down_stream, hipStreamNonBlocking));
     for (u32 i = 0; i < s->cfg.buffer_sets; ++i) {
       s->sets.emplace_back(new BufferSet);
+      BufferSet& bs = *s->sets.back();
+      for (DevBuf* d : {&bs.s, &bs.d, &bs.w}) d->get(s->cfg.max_batch_records * 4);
+      for (DevBuf* d : {&bs.out_s, &bs.out_d, &bs.out_w}) d->get(n_res * 4);
       KMST_HIP(hipEventCreateWithFlags(&s->sets.back()->uploaded, hipEventDisableTiming));
       s->free_sets.push_back(s->sets.back().get());
     }

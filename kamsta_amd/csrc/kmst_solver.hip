@@ -154,6 +154,11 @@ class Solver {
 
   void run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64 out_cap, u64* out_n,
            u64* total);
+  // sizes every workspace for solves of up to m0 records and n0 vertices
+  void reserve() {
+    allocate();
+    C.val0.get((m0 ? m0 : 1) * 8);
+  }
 
  private:
   kmst_context& C;
@@ -651,6 +656,17 @@ kmst_status kmst_solve_device(kmst_context* ctx, const kmst_dev_edges* in, const
 }
 
 const char* kmst_last_error(void) { return kmst::g_last_error.c_str(); }
+
+kmst_status kmst_context_reserve(kmst_context* ctx, uint64_t m_directed, uint64_t n) {
+  return guarded([&] {
+    if (!ctx) throw kmst::Error(KMST_EINVAL, "null context");
+    if (m_directed >= (uint64_t{1} << 32) - 1 || n >= (uint64_t{1} << 32) - 1)
+      throw kmst::Error(KMST_EINVAL, "reservation beyond 32-bit ids");
+    KMST_HIP(hipSetDevice(ctx->device));
+    kmst::Solver s(*ctx, nullptr, nullptr, nullptr, m_directed, n);
+    s.reserve();
+  });
+}
 
 void kmst_context_set_timing(kmst_context* ctx, int on) {
   if (!ctx) return;
