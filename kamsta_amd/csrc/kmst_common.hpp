@@ -81,6 +81,15 @@ struct KernelTimer {
   void begin(hipStream_t st, int kind, double bytes);
   void end(hipStream_t st) noexcept;
   void collect();  // after the stream is synchronised
+  // adds to the algorithmic bytes of the most recent launch of `kind`
+  // (outputs whose size is known only after the launch)
+  void adjust_last(int kind, double delta) {
+    for (size_t i = marks.size(); i-- > 0;)
+      if (marks[i].kind == kind) {
+        marks[i].bytes += delta;
+        return;
+      }
+  }
   void reset() { totals.clear(); }
   ~KernelTimer();
 };

@@ -649,7 +649,9 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
   }
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
   fill_u64(st, lb, u64(tiles) + 1, 0);  // tile states + dynamic tile counter
-  KTScope t(st, K_RELABEL_SCATTER, 24.0 * m + 16.0 * m);
+  // reads: record (12 B with computed keys, 16 B stored) + two 4-byte label
+  // gathers; the 16-byte writes of kept records are added by the caller
+  KTScope t(st, K_RELABEL_SCATTER, (keys.v ? 16.0 : 12.0) * m + 8.0 * m);
   relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, key_min, m, label, s_out,
                                                    d_out, v_out,
                                                    lb, reinterpret_cast<u32*>(lb + tiles), tiles,
@@ -1100,7 +1102,9 @@ void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const K
   }
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
   fill_u64(st, lb, u64(tiles) + 1, 0);
-  KTScope t(st, K_SPLIT_SCATTER, 32.0 * m);
+  // reads the records; writes (light, plus heavy when kept) are added by
+  // the caller once the light count is known
+  KTScope t(st, K_SPLIT_SCATTER, (keys.v ? 16.0 : 12.0) * m);
   split_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, m, pivot, wshift, write_heavy, s_out,
                                          d_out, v_out, lb,
                                          reinterpret_cast<u32*>(lb + tiles), tiles,

@@ -316,6 +316,7 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     relabel_compact(st, cur.s, cur.d, keys_of(cur), 0, m, label, out.s, out.d, out.v, lb,
                     ctr + C_MNEXT, ctr + C_ERR);
     check_errors("round");
+    if (g_ktimer) g_ktimer->adjust_last(K_RELABEL_SCATTER, 16.0 * C.host_counters[C_MNEXT]);
     const u64 round_total = counter64(C_ROUND64);
     const u64 n_next = round_total & 0xFFFFFFFFull;
     mst_count += round_total >> 32;
@@ -346,6 +347,7 @@ void Solver::base_case(RecView P, RecView Q) {
     relabel_compact(st, P.s, P.d, keys_of(P), 0, P.m, newid, Q.s, Q.d, Q.v, lb, ctr + C_MNEXT,
                     ctr + C_ERR);
     check_errors("compact");
+    if (g_ktimer) g_ktimer->adjust_last(K_RELABEL_SCATTER, 16.0 * C.host_counters[C_MNEXT]);
     const u64 nc = C.host_counters[C_NNEXT];
     const u64 mc = C.host_counters[C_MNEXT];
     rounds(Q.at(0, mc), P.at(0, mc), Q.at(0, mc), true, nc, true);
@@ -370,7 +372,9 @@ u64 Solver::partition(RecView in, RecView out, u32 pivot, bool write_heavy) {
   split_light_heavy(st, in.s, in.d, keys_of(in), in.m, pivot, packed ? 2 * vbits : 32,
                     write_heavy, out.s, out.d, out.v, lb, ctr + C_LIGHT, ctr + C_ERR);
   check_errors("split");
-  return C.host_counters[C_LIGHT];
+  const u64 L = C.host_counters[C_LIGHT];
+  if (g_ktimer) g_ktimer->adjust_last(K_SPLIT_SCATTER, 16.0 * (write_heavy ? in.m : L));
+  return L;
 }
 
 // Heavy half of one recursion level: relabel by the parent array, drop
@@ -394,6 +398,7 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
                   scratch.d, scratch.v, lb, ctr + C_MNEXT, ctr + C_ERR);
   check_errors("filter");
   const u64 Hk = C.host_counters[C_MNEXT];
+  if (g_ktimer) g_ktimer->adjust_last(K_RELABEL_SCATTER, 16.0 * Hk);
   C.stats.filter.back().heavy_kept = Hk;
   if (Hk == 0) return;
   RecView kept = scratch.at(0, Hk);
