@@ -136,6 +136,20 @@ kmst_status kmst_gen_gnm_host(uint64_t n, uint64_t m, uint32_t max_weight,
                               uint64_t seed, uint32_t* src, uint32_t* dst,
                               uint32_t* weight);
 
+/* RGG2D: n points uniform in the unit square (32-bit fixed point), edge iff
+ * the integer squared distance <= r^2, r = sqrt(avg_degree / (pi n)).
+ * RHG: threshold random hyperbolic graph with power-law exponent gamma.
+ * Both write symmetric records sorted by (src, dst), weights
+ * 1 + H(seed, min, max) mod max_weight.  The record count is known only after
+ * generation: *m_directed always receives it, and KMST_EINVAL is returned
+ * (nothing written) when capacity is smaller. */
+kmst_status kmst_gen_rgg2d_host(uint64_t n, double avg_degree, uint32_t max_weight,
+                                uint64_t seed, uint32_t* src, uint32_t* dst, uint32_t* weight,
+                                uint64_t capacity, uint64_t* m_directed);
+kmst_status kmst_gen_rhg_host(uint64_t n, double avg_degree, double gamma, uint32_t max_weight,
+                              uint64_t seed, uint32_t* src, uint32_t* dst, uint32_t* weight,
+                              uint64_t capacity, uint64_t* m_directed);
+
 /* ---- Batched serving ----------------------------------------------------
  * A persistent request loop on one GPU: requests (host SoA edge lists with
  * the same contract as kmst_boruvka) are queued by kmst_server_submit,

@@ -38,6 +38,7 @@ BORUVKA, FILTER_BORUVKA = 0, 1
 
 # every symbol include/kmst.h declares
 EXPORTED_SYMBOLS = [
+    "kmst_gen_rgg2d_host", "kmst_gen_rhg_host",
     "kmst_config_default", "kmst_context_create", "kmst_context_destroy", "kmst_context_trim",
     "kmst_context_reserve",
     "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_context_set_timing",
@@ -126,6 +127,9 @@ def load():
         "kmst_context_stats_json": (ctypes.c_size_t, [vp, vp, ctypes.c_size_t]),
         "kmst_gen_gnm_device": (i32, [vp, u64, u64, u32, u64, vp, vp, vp]),
         "kmst_gen_gnm_host": (i32, [u64, u64, u32, u64, vp, vp, vp]),
+        "kmst_gen_rgg2d_host": (i32, [u64, ctypes.c_double, u32, u64, vp, vp, vp, u64, pu64]),
+        "kmst_gen_rhg_host": (i32, [u64, ctypes.c_double, ctypes.c_double, u32, u64, vp, vp, vp,
+                                    u64, pu64]),
         "kmst_k_segmented_argmin": (i32, [vp, vp, vp, vp, u64, u64, vp, vp]),
         "kmst_k_sort_pairs": (i32, [vp, vp, vp, vp, u64, u32]),
         "kmst_k_pointer_jump": (i32, [vp, vp, u64]),
@@ -180,6 +184,35 @@ def gen_gnm_host(n, m, max_weight=254, seed=1):
     _check(load().kmst_gen_gnm_host(n, m, max_weight, seed, s.ctypes.data, d.ctypes.data,
                                     w.ctypes.data))
     return s, d, w
+
+
+def _gen_sized(call, guess):
+    cap = max(int(guess), 16)
+    while True:
+        s, d, w = (np.empty(cap, np.uint32) for _ in range(3))
+        m = ctypes.c_uint64(0)
+        st = call(s.ctypes.data, d.ctypes.data, w.ctypes.data, cap, ctypes.byref(m))
+        if st == KMST_OK:
+            return s[: m.value], d[: m.value], w[: m.value]
+        if st == KMST_EINVAL and m.value > cap:
+            cap = m.value
+            continue
+        _check(st)
+
+
+def gen_rgg2d_host(n, avg_degree=16.0, max_weight=(1 << 31) - 1, seed=3):
+    """Random geometric graph in the unit square (deterministic)."""
+    L = load()
+    return _gen_sized(lambda s, d, w, cap, m: L.kmst_gen_rgg2d_host(
+        n, float(avg_degree), max_weight, seed, s, d, w, cap, m), n * avg_degree * 1.2)
+
+
+def gen_rhg_host(n, avg_degree=32.0, gamma=2.8, max_weight=254, seed=4):
+    """Threshold random hyperbolic graph (deterministic, double precision)."""
+    L = load()
+    return _gen_sized(lambda s, d, w, cap, m: L.kmst_gen_rhg_host(
+        n, float(avg_degree), float(gamma), max_weight, seed, s, d, w, cap, m),
+        n * avg_degree * 1.5)
 
 
 def to_wedges(src, dst, weight):

@@ -9,7 +9,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkmst.so")
-SOURCES = ["kmst_kernels.hip", "kmst_solver.hip", "kmst_gen.hip", "kmst_serve.hip"]
+SOURCES = ["kmst_kernels.hip", "kmst_solver.hip", "kmst_gen.hip", "kmst_serve.hip",
+           "kmst_gen_geo.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

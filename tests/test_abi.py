@@ -90,3 +90,28 @@ def test_driver_help_without_gpu():
     assert r.returncode == 0 and "--filter_threshold" in r.stdout
     r = subprocess.run([exe, "--bogus"], capture_output=True, text=True, timeout=30)
     assert r.returncode == 2
+
+
+@pytest.mark.parametrize("gen", ["rgg", "rhg"])
+def test_geometric_generators_contract(gen):
+    import oracle
+    if gen == "rgg":
+        s, d, w = k.gen_rgg2d_host(5000, 16.0, (1 << 31) - 1, 3)
+        assert 12 < len(s) / 5000 < 17 and w.max() > (1 << 20)
+    else:
+        s, d, w = k.gen_rhg_host(5000, 32.0, 2.8, 254, 4)
+        assert 24 < len(s) / 5000 < 36 and np.bincount(s).max() > 200  # heavy tail
+    key = s.astype(np.uint64) << 32 | d
+    assert np.all(np.diff(key.astype(np.int64)) > 0) and np.all(s != d)
+    fwd = s < d
+    assert fwd.sum() * 2 == len(s)
+    rev = {(int(b), int(a)): int(x) for a, b, x in zip(s[~fwd], d[~fwd], w[~fwd])}
+    assert all(rev[(int(a), int(b))] == int(x) for a, b, x in zip(s[fwd], d[fwd], w[fwd]))
+    s2, d2, w2 = (k.gen_rgg2d_host(5000, 16.0, (1 << 31) - 1, 3) if gen == "rgg"
+                  else k.gen_rhg_host(5000, 32.0, 2.8, 254, 4))
+    assert np.array_equal(s, s2) and np.array_equal(d, d2) and np.array_equal(w, w2)
+    e = k.to_wedges(s, d, w)
+    kr = oracle.kruskal(e)
+    for lp in (0, 1):
+        b = oracle.boruvka(e, lp)
+        assert b[1] == kr[1] and np.array_equal(b[0], kr[0])

@@ -256,3 +256,17 @@ def test_device_solve_misaligned_inputs(gpu, ctx):
         assert tw == exp_w and cnt == len(exp)
         assert np.array_equal(host(outs[0], np.uint32)[:cnt], exp["src"])
         assert np.array_equal(host(outs[1], np.uint32)[:cnt], exp["dst"])
+
+
+@pytest.mark.parametrize("n", [1 << 12, 1 << 16])
+def test_rgg2d_positional_keys(gpu, n):
+    """Config 3 shape at reduced n: weights up to 2^31-1 force positional keys."""
+    s, d, w = k.gen_rgg2d_host(n, 16.0, (1 << 31) - 1, 3)
+    check_equal(k.to_wedges(s, d, w), threshold=4)
+
+
+@pytest.mark.parametrize("n", [1 << 13, 1 << 16])
+def test_rhg_hubs(gpu, n):
+    """Config 4 shape at reduced n: power-law degrees (hubs across many tiles)."""
+    s, d, w = k.gen_rhg_host(n, 32.0, 2.8, 254, 4)
+    check_equal(k.to_wedges(s, d, w), threshold=4)
