@@ -40,6 +40,9 @@ def parse():
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (a kernel of "
                         "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    p.add_argument("--graph", choices=["gnm", "rgg2d", "rhg"], default="gnm",
+                   help="gnm: device GNM (headline); rgg2d: n=2^log_n points, degree 16, weights "
+                        "U[1,2^31-1]; rhg: n=2^log_n, degree 32, gamma 2.8 (host-generated)")
     p.add_argument("--skip-identity", action="store_true",
                    help="skip the plain-vs-filter identity solve (profiling runs: only the "
                         "timed algorithm's launches)")
@@ -218,6 +221,7 @@ def main():
     n, m = 1 << args.log_n, 1 << args.log_m
     N = 2 * m
     dev = torch.device("cuda:0")
+    workload = f"GNM n=2^{args.log_n} m=2^{args.log_m} weights U[1,254]"
     if args.copy_calibration_mib:
         cal = torch.empty(args.copy_calibration_mib << 18, dtype=torch.int32, device=dev)
         cal.fill_(1)
@@ -226,14 +230,26 @@ def main():
         log(f"[bench] calibration copy of {args.copy_calibration_mib} MiB done")
         del cal, cal2
     ctx = k.Context(0)
-    log(f"[bench] generating GNM n=2^{args.log_n} m=2^{args.log_m} on device")
-    s = torch.empty(N, dtype=torch.int32, device=dev)
-    d = torch.empty(N, dtype=torch.int32, device=dev)
-    w = torch.empty(N, dtype=torch.int32, device=dev)
     t0 = time.time()
-    ctx.gen_gnm_device(n, m, 254, args.seed, s, d, w)
+    if args.graph == "gnm":
+        log(f"[bench] generating GNM n=2^{args.log_n} m=2^{args.log_m} on device")
+        s = torch.empty(N, dtype=torch.int32, device=dev)
+        d = torch.empty(N, dtype=torch.int32, device=dev)
+        w = torch.empty(N, dtype=torch.int32, device=dev)
+        ctx.gen_gnm_device(n, m, 254, args.seed, s, d, w)
+    else:
+        if args.graph == "rgg2d":
+            hs, hd, hw = k.gen_rgg2d_host(n, 16.0, (1 << 31) - 1, args.seed)
+            workload = f"RGG2D n=2^{args.log_n} avg degree 16 weights U[1,2^31-1]"
+        else:
+            hs, hd, hw = k.gen_rhg_host(n, 32.0, 2.8, 254, args.seed)
+            workload = f"RHG n=2^{args.log_n} avg degree 32 gamma 2.8 weights U[1,254]"
+        N = len(hs)
+        m = N // 2
+        s, d, w = (torch.from_numpy(a.view(np.int32)).to(dev) for a in (hs, hd, hw))
+        del hs, hd, hw
     torch.cuda.synchronize()
-    log(f"[bench] generated in {time.time() - t0:.2f}s")
+    log(f"[bench] {workload}: {N} records, generated in {time.time() - t0:.2f}s")
     outs = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(3)]
     outs_f = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(3)]
     cfg = k.AlgorithmConfig(filter_threshold=args.filter_threshold)
@@ -358,7 +374,8 @@ def main():
         log(f"[bench] full Kruskal check: {ok} ({time.perf_counter() - t0:.1f}s)")
 
     line = {
-        "metric": "input edges/s at GNM 2^28 (bit-exact MST)",
+        "metric": "input edges/s at GNM 2^28 (bit-exact MST)" if args.graph == "gnm"
+                  else "input edges/s (bit-exact MST)",
         "value": round(m / (ms / 1000.0), 1),
         "unit": "input edges/s",
         "n_gpus": 1,
@@ -369,8 +386,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32/u64",
-        "data": "synthetic (device GNM generator, seed %d)" % args.seed,
-        "config": {"workload": f"GNM n=2^{args.log_n} m=2^{args.log_m} weights U[1,254]",
+        "data": ("synthetic (device GNM generator, seed %d)" if args.graph == "gnm"
+                 else "synthetic (host generator, seed %d)") % args.seed,
+        "config": {"workload": workload,
                    "algorithm": best, "filter_threshold": args.filter_threshold,
                    "directed_records": N,
                    "mst_edges": cnt_p, "mst_weight": tw_p,
