@@ -32,6 +32,7 @@ enum ErrBit : u32 {
   ERR_INVALID = 2u,
   ERR_ASYMMETRIC = 4u,
   ERR_SPIN = 8u,      // a bounded wait or bounded loop hit its limit
+  ERR_BOUNDS = 16u,   // a computed output position fell outside its buffer
 };
 
 struct Error : std::runtime_error {

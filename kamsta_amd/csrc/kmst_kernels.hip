@@ -145,51 +145,160 @@ __device__ __forceinline__ u64 sym_hash(u32 a, u32 b, u32 w) {
   return x;
 }
 
+// Weight class of w for ascending pivots q[0..K): the number of pivots below
+// w (0 = lightest class, K = heavier than every pivot).
+__device__ __forceinline__ u32 weight_class(u32 w, const WeightPivots& P) {
+  u32 c = 0;
+#pragma unroll
+  for (u32 j = 0; j < MAX_PIVOTS; ++j) c += (j < P.K && w > P.q[j]) ? 1u : 0u;
+  return c;
+}
+
+// Sums of one column of a per-wave reduction table (block-wide).
+template <int COLS>
+__device__ __forceinline__ u64 s_red_sum(const u64 (&t)[WAVES][COLS], int col) {
+  u64 x = 0;
+#pragma unroll
+  for (int w = 0; w < WAVES; ++w) x += t[w][col];
+  return x;
+}
+template <int COLS>
+__device__ __forceinline__ u64 s_red_total(const u64 (&t)[WAVES][COLS]) {
+  return s_red_sum(t, 3);  // column 3 holds the record counts
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__ src,
                                                          const u32* __restrict__ dst,
                                                          const u32* __restrict__ wt, u64 m,
                                                          u64 n, u32* err, u32* fwd_count,
-                                                         u32* max_w, u64* hsum) {
-  u32 bad = 0, fwd = 0, mw = 0;
+                                                         u32* max_w, u64* hsum,
+                                                         const WeightPivots P,
+                                                         u64* class_counts,
+                                                         u32* __restrict__ present) {
+  __shared__ u64 s_red[WAVES][MAX_PIVOTS + 4];
+  const int lane = lane_id();
+  u32 bad = 0, fwd = 0, mw = 0, seen = 0;
   u64 hf = 0, hr = 0;
-  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
-    const u32 s = src[i], d = dst[i], w = wt[i];
+  u32 above[MAX_PIVOTS];  // records with w > q[j]
+#pragma unroll
+  for (int j = 0; j < MAX_PIVOTS; ++j) above[j] = 0;
+  auto one = [&](u32 s, u32 d, u32 w, u32 ps, u32 pd, bool has_prev) {
     if (s >= n || d >= n || s == d) bad |= ERR_INVALID;
-    if (i > 0) {
-      const u32 ps = src[i - 1], pd = dst[i - 1];
-      if (ps > s || (ps == s && pd > d)) bad |= ERR_UNSORTED;
-    }
+    if (has_prev && (ps > s || (ps == s && pd > d))) bad |= ERR_UNSORTED;
+    if (present && s < n && (!has_prev || ps != s)) present[s] = 1u;
     mw = w > mw ? w : mw;
-    if (s < d) {
-      ++fwd;
-      hf += sym_hash(s, d, w);
-    } else {
-      hr += sym_hash(d, s, w);
+    ++seen;
+    const bool f = s < d;
+    const u64 h = sym_hash(f ? s : d, f ? d : s, w);
+    fwd += f ? 1u : 0u;
+    hf += f ? h : 0ull;
+    hr += f ? 0ull : h;
+#pragma unroll
+    for (u32 j = 0; j < MAX_PIVOTS; ++j)
+      if (j < P.K) above[j] += (w > P.q[j]) ? 1u : 0u;
+  };
+  if (VEC) {
+    // chunks of 4 consecutive records (16-byte loads); consecutive lanes take
+    // consecutive chunks, so the record before a chunk is the previous lane's
+    // last one.  Two chunks per thread and iteration are loaded before use.
+    constexpr int VU = 2;
+    const u64 full = m / 4;
+    const u64 stride = u64(gridDim.x) * BLOCK;
+    for (u64 base = u64(blockIdx.x) * BLOCK; base < full; base += VU * stride) {
+      uint4 vs[VU], vd[VU], vw[VU];
+#pragma unroll
+      for (int u = 0; u < VU; ++u) {
+        const u64 ch = min(base + u * stride + threadIdx.x, full - 1);
+        vs[u] = reinterpret_cast<const uint4*>(src)[ch];
+        vd[u] = reinterpret_cast<const uint4*>(dst)[ch];
+        vw[u] = reinterpret_cast<const uint4*>(wt)[ch];
+      }
+#pragma unroll
+      for (int u = 0; u < VU; ++u) {
+        const u64 ch = base + u * stride + threadIdx.x;
+        u32 ps = __shfl_up(vs[u].w, 1, WAVE), pd = __shfl_up(vd[u].w, 1, WAVE);
+        if (ch < full) {
+          if (lane == 0 && ch > 0) {
+            ps = src[ch * 4 - 1];
+            pd = dst[ch * 4 - 1];
+          }
+          one(vs[u].x, vd[u].x, vw[u].x, ps, pd, ch > 0);
+          one(vs[u].y, vd[u].y, vw[u].y, vs[u].x, vd[u].x, true);
+          one(vs[u].z, vd[u].z, vw[u].z, vs[u].y, vd[u].y, true);
+          one(vs[u].w, vd[u].w, vw[u].w, vs[u].z, vd[u].z, true);
+        }
+      }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      for (u64 i = full * 4; i < m; ++i)
+        one(src[i], dst[i], wt[i], i ? src[i - 1] : 0u, i ? dst[i - 1] : 0u, i > 0);
+    }
+  } else {
+    for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK)
+      one(src[i], dst[i], wt[i], i ? src[i - 1] : 0u, i ? dst[i - 1] : 0u, i > 0);
   }
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) {
     fwd += __shfl_xor(fwd, o, WAVE);
+    seen += __shfl_xor(seen, o, WAVE);
     const u32 om = __shfl_xor(mw, o, WAVE);
     mw = om > mw ? om : mw;
     hf += __shfl_xor(hf, o, WAVE);
     hr += __shfl_xor(hr, o, WAVE);
+#pragma unroll
+    for (u32 j = 0; j < MAX_PIVOTS; ++j)
+      if (j < P.K) above[j] += __shfl_xor(above[j], o, WAVE);
   }
-  if (lane_id() == 0) {
-    if (fwd) atomicAdd(fwd_count, fwd);
+  const int w_ = wave_id();
+  if (lane == 0) {
+    s_red[w_][0] = fwd;
+    s_red[w_][1] = hf;
+    s_red[w_][2] = hr;
+    s_red[w_][3] = seen;
+    for (int j = 0; j < MAX_PIVOTS; ++j) s_red[w_][4 + j] = above[j];
     if (mw) atomicMax(max_w, mw);
-    if (hf) atomicAdd(reinterpret_cast<unsigned long long*>(hsum), static_cast<unsigned long long>(hf));
-    if (hr) atomicAdd(reinterpret_cast<unsigned long long*>(hsum + 1), static_cast<unsigned long long>(hr));
   }
   if (bad) atomicOr(err, bad);
+  __syncthreads();
+  if (threadIdx.x < MAX_PIVOTS + 4) {
+    u64 t = 0;
+    for (int ww = 0; ww < WAVES; ++ww) t += s_red[ww][threadIdx.x];
+    const u32 x = threadIdx.x;
+    if (x == 0) {
+      if (t) atomicAdd(fwd_count, static_cast<u32>(t));
+    } else if (x <= 2) {
+      if (t) atomicAdd(reinterpret_cast<unsigned long long*>(hsum + x - 1),
+                       static_cast<unsigned long long>(t));
+    } else if (P.K) {
+      // class c counts: total - above[0], above[c-1] - above[c], above[K-1]
+      // (wrapping u64 arithmetic; the block's contributions sum correctly)
+      const u32 c = x - 3;  // 0..MAX_PIVOTS
+      if (c <= P.K) {
+        const u64 lo = (c == 0) ? s_red_total(s_red) : s_red_sum(s_red, 4 + c - 1);
+        const u64 hi = (c == P.K) ? 0ull : s_red_sum(s_red, 4 + c);
+        const u64 v = lo - hi;
+        if (v) atomicAdd(reinterpret_cast<unsigned long long*>(class_counts + c),
+                         static_cast<unsigned long long>(v));
+      }
+    }
+  }
 }
 
 void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m, u64 n,
-                      u32* err, u32* fwd_count, u32* max_w, u64* hsum) {
+                      u32* err, u32* fwd_count, u32* max_w, u64* hsum, const WeightPivots& P,
+                      u64* class_counts, u32* present) {
   if (m == 0) return;
   KTScope t(st, K_VALIDATE, 12.0 * m);
-  validate_kernel<<<grid_for(m, 16384), BLOCK, 0, st>>>(src, dst, w, m, n, err, fwd_count, max_w,
-                                                         hsum);
+  const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst) |
+                     reinterpret_cast<uintptr_t>(w)) & 15) == 0;
+  const u32 grid = grid_for((m + 3) / 4, 4096);
+  if (vec)
+    validate_kernel<true><<<grid, BLOCK, 0, st>>>(src, dst, w, m, n, err, fwd_count, max_w, hsum,
+                                                  P, class_counts, present);
+  else
+    validate_kernel<false><<<grid, BLOCK, 0, st>>>(src, dst, w, m, n, err, fwd_count, max_w, hsum,
+                                                   P, class_counts, present);
   KMST_HIP(hipGetLastError());
 }
 
@@ -578,10 +687,45 @@ __device__ __forceinline__ u32 lb_tile_id(u32* counter) {
   return s_tile;
 }
 
+// Loads of one wave's sub-tile: record e = wbase + k * WAVE + lane for k <
+// CMP_IPT.  The index is clamped to m - 1, so every load is unconditional and
+// all of them can be in flight together; callers mask the e >= m slots.
+__device__ __forceinline__ void load_records_sd(const u32* __restrict__ s_in,
+                                                const u32* __restrict__ d_in, u64 wbase, u64 m,
+                                                u32 (&ss)[CMP_IPT], u32 (&dd)[CMP_IPT]) {
+  const u64 lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u64 e = min(wbase + u64(k) * WAVE + lane, m - 1);
+    ss[k] = s_in[e];
+    dd[k] = d_in[e];
+  }
+}
+
+__device__ __forceinline__ void load_records(const u32* __restrict__ s_in,
+                                             const u32* __restrict__ d_in, const KeySrc& keys,
+                                             u64 wbase, u64 m, u32 (&ss)[CMP_IPT],
+                                             u32 (&dd)[CMP_IPT], u64 (&vv)[CMP_IPT]) {
+  const u64 lane = lane_id();
+  load_records_sd(s_in, d_in, wbase, m, ss, dd);
+  if (keys.v) {
+#pragma unroll
+    for (int k = 0; k < CMP_IPT; ++k) vv[k] = keys.v[min(wbase + u64(k) * WAVE + lane, m - 1)];
+  } else {
+    u32 ww[CMP_IPT];
+#pragma unroll
+    for (int k = 0; k < CMP_IPT; ++k) ww[k] = keys.w[min(wbase + u64(k) * WAVE + lane, m - 1)];
+#pragma unroll
+    for (int k = 0; k < CMP_IPT; ++k) vv[k] = pack_key(ww[k], ss[k], dd[k], keys.vbits);
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 key_min,
-    u64 m, const u32* __restrict__ label, u32* __restrict__ s_out, u32* __restrict__ d_out,
-    u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles, u32* count_dev, u32* err) {
+    u64 key_max, u64 m, const u32* __restrict__ label, const u64* __restrict__ giant,
+    u32* __restrict__ s_out,
+    u32* __restrict__ d_out, u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles,
+    u32* count_dev, u32* err) {
   __shared__ u32 s_wave_cnt[WAVES];
   __shared__ u64 s_excl;
   __shared__ u32 s_ok;
@@ -592,22 +736,34 @@ __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
   u64 vv[CMP_IPT];
   u64 bal[CMP_IPT];
   u32 cnt = 0;
+  // phased so that each thread has all its loads of one kind in flight at
+  // once: (src, dst), then the bitmap, then keys, then the label gathers
+  u32 se[CMP_IPT], de[CMP_IPT];
+  load_records_sd(s_in, d_in, wbase, m, se, de);
+  bool live[CMP_IPT];
 #pragma unroll
   for (int k = 0; k < CMP_IPT; ++k) {
     const u64 e = wbase + u64(k) * WAVE + lane;
-    bool keep = false;
-    ls[k] = ld[k] = 0;
-    vv[k] = VAL_NONE;
-    if (e < m) {
-      const u32 se = s_in[e], de = d_in[e];
-      vv[k] = load_key(keys, e, se, de);
-      if (vv[k] != VAL_NONE && vv[k] >= key_min) {
-        ls[k] = label[se];
-        ld[k] = label[de];
-        keep = ls[k] != ld[k];
-      }
-    }
-    bal[k] = __ballot(keep);
+    // both ends in the marked component: a self-loop, decided without the
+    // key or the label gathers
+    const bool inside = giant && ((giant[se[k] >> 6] >> (se[k] & 63)) &
+                                  (giant[de[k] >> 6] >> (de[k] & 63)) & 1ull);
+    live[k] = e < m && !inside;
+  }
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u64 e = wbase + u64(k) * WAVE + lane;
+    vv[k] = live[k] ? load_key(keys, e, se[k], de[k]) : VAL_NONE;
+    live[k] = live[k] && vv[k] != VAL_NONE && vv[k] >= key_min && vv[k] <= key_max;
+  }
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    ls[k] = live[k] ? label[se[k]] : 0u;
+    ld[k] = live[k] ? label[de[k]] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    bal[k] = __ballot(live[k] && ls[k] != ld[k]);
     cnt += static_cast<u32>(__popcll(bal[k]));
   }
   if (lane == 0) s_wave_cnt[w] = cnt;
@@ -642,7 +798,7 @@ __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
 
 void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                      u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
-                     u64* lb, u32* count_dev, u32* err) {
+                     u64* lb, u32* count_dev, u32* err, const u64* giant, u64 n, u64 key_max) {
   if (m == 0) {
     fill_u32(st, count_dev, 1, 0);
     return;
@@ -650,10 +806,14 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
   fill_u64(st, lb, u64(tiles) + 1, 0);  // tile states + dynamic tile counter
   // reads: record (12 B with computed keys, 16 B stored) + two 4-byte label
-  // gathers; the 16-byte writes of kept records are added by the caller
-  KTScope t(st, K_RELABEL_SCATTER, (keys.v ? 16.0 : 12.0) * m + 8.0 * m);
-  relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, key_min, m, label, s_out,
-                                                   d_out, v_out,
+  // gathers; with the component bitmap only (src, dst) and the bitmap are
+  // counted (a lower bound).  The 16-byte writes of kept records are added by
+  // the caller.
+  KTScope t(st, K_RELABEL_SCATTER,
+            giant ? 8.0 * m + double(n) / 8.0 : (keys.v ? 16.0 : 12.0) * m + 8.0 * m);
+  relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, key_min, key_max, m, label,
+                                                   giant,
+                                                   s_out, d_out, v_out,
                                                    lb, reinterpret_cast<u32*>(lb + tiles), tiles,
                                                    count_dev, err);
   KMST_HIP(hipGetLastError());
@@ -1041,17 +1201,13 @@ __global__ __launch_bounds__(BLOCK) void split_kernel(
   u64 vv[CMP_IPT];
   u64 bal[CMP_IPT];
   u32 cnt = 0;
+  load_records(s_in, d_in, keys, wbase, m, ss, dd, vv);
 #pragma unroll
   for (int k = 0; k < CMP_IPT; ++k) {
     const u64 e = wbase + u64(k) * WAVE + lane;
-    bool light = false;
-    if (e < m) {
-      ss[k] = s_in[e];
-      dd[k] = d_in[e];
-      vv[k] = load_key(keys, e, ss[k], dd[k]);
-      // dead records (VAL_NONE) always count as heavy
-      light = vv[k] != VAL_NONE && static_cast<u32>(vv[k] >> wshift) <= pivot;
-    }
+    // dead records (VAL_NONE) always count as heavy
+    const bool light =
+        e < m && vv[k] != VAL_NONE && static_cast<u32>(vv[k] >> wshift) <= pivot;
     bal[k] = __ballot(light);
     cnt += static_cast<u32>(__popcll(bal[k]));
   }
@@ -1150,19 +1306,349 @@ void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32*
 __global__ __launch_bounds__(BLOCK) void count_roots_kernel(const u32* __restrict__ gpar,
                                                             const u32* __restrict__ has_edge, u64 n,
                                                             u32* count) {
+  __shared__ u32 s_c[WAVES];
   u32 c = 0;
-  for (u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x; v < n; v += u64(gridDim.x) * BLOCK)
+  // 4 vertices per thread and step with 16-byte loads (the arrays come from
+  // the cached allocator, so they are 16-byte aligned)
+  const u64 n4 = n / 4;
+  const u64 stride = u64(gridDim.x) * BLOCK;
+  for (u64 q = u64(blockIdx.x) * BLOCK + threadIdx.x; q < n4; q += stride) {
+    const uint4 g = reinterpret_cast<const uint4*>(gpar)[q];
+    const uint4 h = reinterpret_cast<const uint4*>(has_edge)[q];
+    const u32 v = static_cast<u32>(q * 4);
+    c += (g.x == v && h.x) + (g.y == v + 1 && h.y) + (g.z == v + 2 && h.z) + (g.w == v + 3 && h.w);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - n4 * 4) {
+    const u64 v = n4 * 4 + threadIdx.x;
     c += (gpar[v] == v && has_edge[v]) ? 1u : 0u;
+  }
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
-  if (lane_id() == 0 && c) atomicAdd(count, c);
+  if (lane_id() == 0) s_c[wave_id()] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 t = 0;
+    for (int w = 0; w < WAVES; ++w) t += s_c[w];
+    if (t) atomicAdd(count, t);
+  }
 }
 
 void count_roots(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32* count_dev) {
   fill_u32(st, count_dev, 1, 0);
   if (n == 0) return;
   KTScope t(st, K_OTHER, 8.0 * n);
-  count_roots_kernel<<<grid_for(n, 4096), BLOCK, 0, st>>>(gpar, has_edge, n, count_dev);
+  count_roots_kernel<<<grid_for((n + 3) / 4, 2048), BLOCK, 0, st>>>(gpar, has_edge, n, count_dev);
+  KMST_HIP(hipGetLastError());
+}
+
+// First index i in [0, m) with a[i] >= x (m if none); a sorted ascending.
+__device__ __forceinline__ u64 lower_bound_u32(const u32* __restrict__ a, u64 m, u32 x) {
+  u64 lo = 0, hi = m;
+  while (lo < hi) {
+    const u64 mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Vertices with records whose parent-array root is not `root`, with their
+// record ranges in the (src, dst)-sorted input, in vertex order.  Each block
+// owns OUT_VPT * BLOCK consecutive vertices and takes one slot range with one
+// global atomic.  counters[0] = list length, counters[2..3] = u64 total range
+// length.
+constexpr int OUT_VPT = 16;
+
+__global__ __launch_bounds__(BLOCK) void outside_list_kernel(
+    const u32* __restrict__ gpar, const u32* __restrict__ has_edge, u64 n, u32 root,
+    const u32* __restrict__ src, u64 m, u32* __restrict__ list, u32* __restrict__ seg_lo,
+    u32* __restrict__ seg_hi, u32* counters) {
+  __shared__ u32 s_scan[WAVES];
+  __shared__ u32 s_base;
+  __shared__ u64 s_len[WAVES];
+  const u64 v0 = u64(blockIdx.x) * (BLOCK * OUT_VPT) + u64(threadIdx.x) * OUT_VPT;
+  u32 g[OUT_VPT], h[OUT_VPT];
+#pragma unroll
+  for (int k = 0; k < OUT_VPT; ++k) {
+    const u64 v = min(v0 + k, n - 1);
+    g[k] = gpar[v];
+    h[k] = has_edge[v];
+  }
+  u32 mine = 0;
+#pragma unroll
+  for (int k = 0; k < OUT_VPT; ++k) mine += (v0 + k < n && h[k] && g[k] != root) ? 1u : 0u;
+  u32 total;
+  const u32 off = block_exclusive_sum(mine, total, s_scan);
+  if (threadIdx.x == 0) s_base = total ? atomicAdd(counters, total) : 0u;
+  __syncthreads();
+  u32 slot = s_base + off;
+  u64 len = 0;
+#pragma unroll
+  for (int k = 0; k < OUT_VPT; ++k) {
+    if (v0 + k < n && h[k] && g[k] != root) {
+      const u32 v = static_cast<u32>(v0 + k);
+      const u64 lo = lower_bound_u32(src, m, v);
+      const u64 hi = lower_bound_u32(src, m, v + 1);
+      list[slot] = v;
+      seg_lo[slot] = static_cast<u32>(lo);
+      seg_hi[slot] = static_cast<u32>(hi);
+      len += hi - lo;
+      ++slot;
+    }
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) len += __shfl_xor(len, o, WAVE);
+  if (lane_id() == 0) s_len[wave_id()] = len;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 t = 0;
+    for (int w = 0; w < WAVES; ++w) t += s_len[w];
+    if (t)
+      atomicAdd(reinterpret_cast<unsigned long long*>(counters + 2),
+                static_cast<unsigned long long>(t));
+  }
+}
+
+void outside_list(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32 root,
+                  const u32* src, u64 m, u32* list, u32* seg_lo, u32* seg_hi, u32* counters) {
+  fill_u32(st, counters, 4, 0);
+  if (n == 0) return;
+  KTScope t(st, K_OTHER, 8.0 * n);
+  const u32 blocks = ceil_div_u32(n, BLOCK * OUT_VPT);
+  outside_list_kernel<<<blocks, BLOCK, 0, st>>>(gpar, has_edge, n, root, src, m, list, seg_lo,
+                                                seg_hi, counters);
+  KMST_HIP(hipGetLastError());
+}
+
+// Each block owns BLOCK consecutive listed vertices u and walks their records
+// as one flattened range (a thread finds its vertex by binary search in LDS).
+// Records (u, d, key) with key in [kmin, kmax] and gpar[d] != gpar[u] leave as
+// (gpar[u], gpar[d], key); when d's root is `root` (d is not listed, so its
+// own range is not walked) the reverse record is emitted as well.  A counting
+// pass sizes the block's output, one global atomic places it, and an emitting
+// pass (whose reads hit the cache) writes it.  Output order is arbitrary;
+// positions at or past cap raise ERR_BOUNDS.
+__global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
+    const u32* __restrict__ list, const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
+    u32 n_list, const u32* __restrict__ in_d, const KeySrc keys, u64 kmin, u64 kmax,
+    const u32* __restrict__ gpar, u32 root, u32* __restrict__ s_out, u32* __restrict__ d_out,
+    u64* __restrict__ v_out, u64 cap, u32* count, u32* err) {
+  __shared__ u64 s_start[BLOCK + 1];
+  __shared__ u32 s_u[BLOCK], s_gu[BLOCK], s_lo[BLOCK];
+  __shared__ u64 s_scan[WAVES];
+  __shared__ u32 s_cnt[WAVES];
+  __shared__ u32 s_base, s_cursor;
+  const u32 i0 = blockIdx.x * BLOCK;
+  const u32 nv = min<u32>(BLOCK, n_list - i0);
+  u64 len = 0;
+  if (threadIdx.x < nv) {
+    const u32 u = list[i0 + threadIdx.x];
+    const u32 lo = seg_lo[i0 + threadIdx.x];
+    s_u[threadIdx.x] = u;
+    s_gu[threadIdx.x] = gpar[u];
+    s_lo[threadIdx.x] = lo;
+    len = seg_hi[i0 + threadIdx.x] - lo;
+  }
+  u64 total;
+  const u64 start = block_exclusive_sum(len, total, s_scan);
+  s_start[threadIdx.x] = start;
+  if (threadIdx.x == 0) {
+    s_start[BLOCK] = total;
+    s_cursor = 0;
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  // record t of the block's range: vertex j = last with s_start[j] <= t
+  auto visit = [&](u64 t, bool& keep, bool& both, u32& gu, u32& gd, u64& key) {
+    keep = both = false;
+    if (t >= total) return;
+    u32 lo = 0, hi = nv;  // s_start[lo] <= t < s_start[hi]
+    while (hi - lo > 1) {
+      const u32 mid = (lo + hi) >> 1;
+      if (s_start[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const u64 e = u64(s_lo[lo]) + (t - s_start[lo]);
+    const u32 u = s_u[lo];
+    gu = s_gu[lo];
+    const u32 d = in_d[e];
+    key = load_key(keys, e, u, d);
+    if (key < kmin || key > kmax) return;
+    gd = gpar[d];
+    keep = gd != gu;
+    both = keep && gd == root;
+  };
+  u32 mine = 0;
+  for (u64 t0 = 0; t0 < total; t0 += BLOCK) {
+    bool keep, both;
+    u32 gu, gd;
+    u64 key;
+    visit(t0 + threadIdx.x, keep, both, gu, gd, key);
+    mine += (keep ? 1u : 0u) + (both ? 1u : 0u);
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) mine += __shfl_xor(mine, o, WAVE);
+  if (lane == 0) s_cnt[wave_id()] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 t = 0;
+    for (int w = 0; w < WAVES; ++w) t += s_cnt[w];
+    s_base = t ? atomicAdd(count, t) : 0u;
+  }
+  __syncthreads();
+  for (u64 t0 = 0; t0 < total; t0 += BLOCK) {
+    bool keep, both;
+    u32 gu = 0, gd = 0;
+    u64 key = 0;
+    visit(t0 + threadIdx.x, keep, both, gu, gd, key);
+    const u64 bk = __ballot(keep), bb = __ballot(both);
+    const u32 nw = static_cast<u32>(__popcll(bk) + __popcll(bb));
+    if (nw == 0) continue;  // uniform per wave
+    u32 wbase = 0;
+    if (lane == 0) wbase = atomicAdd(&s_cursor, nw);
+    wbase = __shfl(wbase, 0, WAVE) + s_base;
+    if (keep) {
+      const u64 p = u64(wbase) + mbcnt(bk);
+      if (p < cap) {
+        s_out[p] = gu;
+        d_out[p] = gd;
+        v_out[p] = key;
+      } else {
+        atomicOr(err, ERR_BOUNDS);
+      }
+    }
+    if (both) {
+      const u64 p = u64(wbase) + static_cast<u32>(__popcll(bk)) + mbcnt(bb);
+      if (p < cap) {
+        s_out[p] = gd;
+        d_out[p] = gu;
+        v_out[p] = key;
+      } else {
+        atomicOr(err, ERR_BOUNDS);
+      }
+    }
+  }
+}
+
+void outside_gather(hipStream_t st, const u32* list, const u32* seg_lo, const u32* seg_hi,
+                    u32 n_list, u64 total, const u32* in_d, const KeySrc& keys, u64 kmin, u64 kmax,
+                    const u32* gpar, u32 root, u32* s_out, u32* d_out, u64* v_out, u64 cap,
+                    u32* count, u32* err) {
+  fill_u32(st, count, 1, 0);
+  if (n_list == 0) return;
+  // reads of the ranges (dst + key) and the root gathers; writes are added by
+  // the caller
+  KTScope t(st, K_RELABEL_SCATTER, (keys.v ? 16.0 : 8.0) * double(total) + 12.0 * n_list);
+  outside_gather_kernel<<<ceil_div_u32(n_list, BLOCK), BLOCK, 0, st>>>(
+      list, seg_lo, seg_hi, n_list, in_d, keys, kmin, kmax, gpar, root, s_out, d_out, v_out, cap,
+      count, err);
+  KMST_HIP(hipGetLastError());
+}
+
+// bits[v / 64] bit v % 64 = (gpar[v] == root), for v < n.
+__global__ __launch_bounds__(BLOCK) void component_bits_kernel(const u32* __restrict__ gpar, u64 n,
+                                                               u32 root, u64* __restrict__ bits) {
+  const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  const u64 b = __ballot(v < n && gpar[v] == root);
+  if (lane_id() == 0 && v < n) bits[v >> 6] = b;
+}
+
+void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits) {
+  if (n == 0) return;
+  KTScope t(st, K_OTHER, 4.0 * n + n / 8.0);
+  component_bits_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(gpar, n, root, bits);
+  KMST_HIP(hipGetLastError());
+}
+
+// Multi-way weight-class scatter (all levels of the filter recursion's
+// light/heavy partition in one pass; the pivots are quantiles of one sample).
+// Class 0 (the lightest) keeps input order, so it stays sorted by (src, dst):
+// its offsets come from a decoupled look-back.  Classes 1..K-1 are relabelled
+// and re-sorted before use, so each tile takes one atomic slot range per
+// class.  Class K (heavier than every pivot) is not written.
+__global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
+    const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 m,
+    const WeightPivots P, u32 wshift, const ClassBases B, u64 out_cap, u32* __restrict__ s_out,
+    u32* __restrict__ d_out, u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles,
+    u64* __restrict__ class_cursor, u32* err) {
+  const u32 K = P.K;
+  __shared__ u32 s_wave0[WAVES];
+  __shared__ u32 s_cnt[MAX_CLASSES];
+  __shared__ u64 s_base[MAX_CLASSES];
+  __shared__ u64 s_excl;
+  __shared__ u32 s_ok;
+  if (threadIdx.x < MAX_CLASSES) s_cnt[threadIdx.x] = 0;
+  const u32 tile = lb_tile_id(lb_counter);  // includes a barrier
+  const int lane = lane_id(), w = wave_id();
+  const u64 wbase = u64(tile) * CMP_TILE + u64(w) * (WAVE * CMP_IPT);
+  u32 ss[CMP_IPT], dd[CMP_IPT], slot[CMP_IPT];
+  u64 vv[CMP_IPT];
+  u8 cls[CMP_IPT];
+  u64 bal[CMP_IPT];
+  u32 cnt0 = 0;
+  load_records(s_in, d_in, keys, wbase, m, ss, dd, vv);
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u64 e = wbase + u64(k) * WAVE + lane;
+    const u32 c = e < m ? weight_class(static_cast<u32>(vv[k] >> wshift), P) : K;  // K: not written
+    cls[k] = static_cast<u8>(c);
+    bal[k] = __ballot(c == 0);
+    cnt0 += static_cast<u32>(__popcll(bal[k]));
+    slot[k] = (c > 0 && c < K) ? atomicAdd(&s_cnt[c], 1u) : 0u;
+  }
+  if (lane == 0) s_wave0[w] = cnt0;
+  __syncthreads();
+  if (w == 0) {
+    u64 agg = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) agg += s_wave0[ww];
+    bool ok;
+    const u64 excl = lookback_exclusive(lb_status, tile, agg, err, ok);
+    if (lane == 0) {
+      s_excl = excl;
+      s_ok = ok ? 1u : 0u;
+    }
+  } else if (w == 1 && lane > 0 && u32(lane) < K) {
+    const u32 c = s_cnt[lane];
+    s_base[lane] = B.base[lane] +
+                   (c ? atomicAdd(reinterpret_cast<unsigned long long*>(class_cursor + lane),
+                                  static_cast<unsigned long long>(c))
+                      : 0ull);
+  }
+  __syncthreads();
+  if (!s_ok) return;
+  u64 pos0 = B.base[0] + s_excl;
+  for (int ww = 0; ww < w; ++ww) pos0 += s_wave0[ww];
+#pragma unroll
+  for (int k = 0; k < CMP_IPT; ++k) {
+    const u32 c = cls[k];
+    if (c < K) {
+      const u64 p = (c == 0) ? pos0 + mbcnt(bal[k]) : s_base[c] + slot[k];
+      if (p < out_cap) {
+        s_out[p] = ss[k];
+        d_out[p] = dd[k];
+        v_out[p] = vv[k];
+      } else {
+        atomicOr(err, ERR_BOUNDS);
+      }
+    }
+    pos0 += static_cast<u32>(__popcll(bal[k]));
+  }
+}
+
+void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys, u64 m,
+                   const WeightPivots& P, u32 wshift, const ClassBases& B, u64 written,
+                   u32* s_out, u32* d_out, u64* v_out, u64* lb, u64* class_cursor, u32* err) {
+  if (m == 0 || P.K == 0) return;
+  const u32 tiles = ceil_div_u32(m, CMP_TILE);
+  fill_u64(st, lb, u64(tiles) + 1, 0);
+  fill_u64(st, class_cursor, MAX_CLASSES, 0);
+  KTScope t(st, K_SPLIT_SCATTER, (keys.v ? 16.0 : 12.0) * m + 16.0 * double(written));
+  class_scatter_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, m, P, wshift, B, written, s_out,
+                                                d_out,
+                                                v_out, lb,
+                                                reinterpret_cast<u32*>(lb + tiles), tiles,
+                                                class_cursor, err);
   KMST_HIP(hipGetLastError());
 }
 

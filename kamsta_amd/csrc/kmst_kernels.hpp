@@ -39,8 +39,22 @@ void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scr
 
 // Setup: contract checks (hsum[0..1] = forward / reverse multiset hashes),
 // CSR offsets, strict keys (packed when they fit 64 bits, positional else).
+// Ascending weight pivots q[0..K); weight class of w = #{j : w > q[j]}.
+constexpr u32 MAX_PIVOTS = 8;
+constexpr u32 MAX_CLASSES = MAX_PIVOTS + 1;
+struct WeightPivots {
+  u32 q[MAX_PIVOTS];
+  u32 K;
+};
+struct ClassBases {
+  u64 base[MAX_CLASSES];
+};
+// With P.K > 0 it also counts the records of every weight class into
+// class_counts[0..K] (zeroed by the caller); with present (n entries, zeroed
+// by the caller) it sets present[v] = 1 for every vertex with a record.
 void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
-                      u64 n, u32* err, u32* fwd_count, u32* max_w, u64* hsum);
+                      u64 n, u32* err, u32* fwd_count, u32* max_w, u64* hsum,
+                      const WeightPivots& P, u64* class_counts, u32* present = nullptr);
 void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off);
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                 const u32* off, u64* val, u32* err);
@@ -64,10 +78,32 @@ void make_labels(hipStream_t st, u64 n, const u32* parent, const u64* state, con
 // in one pass.  lb needs lookback_elems(m) u64 entries; *count_dev gets the
 // kept count.
 inline u64 lookback_elems(u64 m) { return (m + CMP_TILE - 1) / CMP_TILE + 1; }
-// Records whose key is below key_min are dropped too (0 keeps all).
+// Records whose key is below key_min are dropped too (0 keeps all).  With
+// giant (a bitmap over the n vertex ids of one component of label, see
+// component_bits) records with both ends inside it are dropped without reading
+// their key or labels.
 void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                      u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
-                     u64* lb, u32* count_dev, u32* err);
+                     u64* lb, u32* count_dev, u32* err, const u64* giant = nullptr, u64 n = 0,
+                     u64 key_max = ~0ull);
+// bits[v / 64] bit v % 64 = (gpar[v] == root) for v < n; bits needs
+// ceil(n / 64) entries.
+void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits);
+
+// Heavy records by the ranges of the vertices outside one component: every
+// record whose ends lie in different components has an end outside the
+// component of `root`, and in a symmetric (src, dst)-sorted input that end's
+// range holds it.  outside_list writes the vertices with records whose root is
+// not `root` (list, seg_lo, seg_hi: up to n entries each); counters[0] = count,
+// counters[2..3] = u64 total range length.  outside_gather then emits, for
+// keys in [kmin, kmax], the relabelled cross-component records in both
+// directions (see the kernel), at most 2 * total of them.
+void outside_list(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32 root,
+                  const u32* src, u64 m, u32* list, u32* seg_lo, u32* seg_hi, u32* counters);
+void outside_gather(hipStream_t st, const u32* list, const u32* seg_lo, const u32* seg_hi,
+                    u32 n_list, u64 total, const u32* in_d, const KeySrc& keys, u64 kmin, u64 kmax,
+                    const u32* gpar, u32 root, u32* s_out, u32* d_out, u64* v_out, u64 cap,
+                    u32* count, u32* err);
 
 // K6 stable LSD radix sort by key (s << dbits) | d over total_bits bits.
 // Returns true when the sorted data ended in the *_alt arrays.  d / v (and
@@ -103,5 +139,13 @@ void count_roots(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u3
 // Present-vertex compaction helpers (sorted records; flag must be zeroed).
 void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag);
 void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32* rep);
+
+// Weight classes 0..K-1 of the input go to out[B.base[c] ...]; class 0
+// keeps input order, the others are in no particular order.  written = total
+// size of classes 0..K-1 (positions at or past it raise ERR_BOUNDS and are
+// not written).  class_cursor needs MAX_CLASSES u64 entries.
+void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys, u64 m,
+                   const WeightPivots& P, u32 wshift, const ClassBases& B, u64 written,
+                   u32* s_out, u32* d_out, u64* v_out, u64* lb, u64* class_cursor, u32* err);
 
 }  // namespace kmst

@@ -112,7 +112,8 @@ using namespace kmst;
 kmst_context::kmst_context(int dev) : device(dev) {
   KMST_HIP(hipSetDevice(device));
   KMST_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_counters), 256, hipHostMallocDefault));
+  KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_counters), HOST_COUNTER_BYTES,
+                         hipHostMallocDefault));
 }
 kmst_context::~kmst_context() {
   (void)hipSetDevice(device);
@@ -133,7 +134,10 @@ namespace kmst {
 // Device counter slots (u32 words; u64 values at even word offsets >= 16).
 enum Ctr : int {
   C_ERR = 0, C_FWD = 1, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6, C_MAXW = 7,
-  C_TOTAL64 = 16, C_HSUM64 = 18, C_ROUND64 = 22
+  C_TOTAL64 = 16, C_HSUM64 = 18, C_ROUND64 = 22,
+  C_CLASS64 = 24,   // MAX_CLASSES u64 class counts (words 24..41)
+  C_CURSOR64 = 42,  // MAX_CLASSES u64 class cursors (words 42..59)
+  C_OUTSIDE = 60    // outside_list: count (60), u64 total range length (62..63)
 };
 
 // A range of directed records; keys are stored in v, or (the read-only input
@@ -184,8 +188,14 @@ class Solver {
   ArgPiece* pieces = nullptr;
   u32* samples = nullptr;
 
+  WeightPivots pivots{};  // weight classes of the input (filter-Boruvka)
+
   void allocate();
-  void setup();
+  // validates the input; for filter-Boruvka also marks vertices with records
+  // (has_edge) and, with plan_classes, counts the weight classes
+  void setup(bool filter, bool plan_classes);
+  void plan_pivots();
+  void filter_classes();
   void check_errors(const char* where);
   void read_counters();
   u64 counter64(int word) const {
@@ -203,8 +213,17 @@ class Solver {
   void filter_rec(RecView P, RecView Q, bool stop, int depth);
   // heavy may be the read-only input; scratch and alt are writable, disjoint
   // from each other and from heavy, each with room for heavy.m records
-  void filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth, u64 key_min = 0);
+  // With from_input, heavy is the read-only input (sorted by (src, dst)) and
+  // the records come from the ranges of the vertices outside the largest
+  // component when those are few.
+  void filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth, u64 key_min = 0,
+                    u64 key_max = ~0ull, bool from_input = false);
+  // true when the parent array has at most one root with edges; also picks
+  // the most frequent root of a sample of vertices (giant_root) and its
+  // sample share
   bool one_component();
+  u32 giant_root = 0;
+  double giant_share = 0.0;
   u32 pick_pivot(RecView r);
   u64 partition(RecView in, RecView out, u32 pivot, bool write_heavy = true);
   KeySrc keys_of(const RecView& r) const { return KeySrc{r.v, r.w, vbits}; }
@@ -244,7 +263,7 @@ void Solver::allocate() {
 }
 
 void Solver::read_counters() {
-  KMST_HIP(hipMemcpyAsync(C.host_counters, ctr, 128, hipMemcpyDeviceToHost, st));
+  KMST_HIP(hipMemcpyAsync(C.host_counters, ctr, 256, hipMemcpyDeviceToHost, st));
   KMST_HIP(hipStreamSynchronize(st));
 }
 
@@ -252,6 +271,7 @@ void Solver::check_errors(const char* where) {
   read_counters();
   const u32 e = C.host_counters[C_ERR];
   if (e & ERR_SPIN) throw Error(KMST_EDEVICE, std::string(where) + ": bounded wait expired");
+  if (e & ERR_BOUNDS) throw Error(KMST_EDEVICE, std::string(where) + ": output position out of range");
   if (e & ERR_INVALID)
     throw Error(KMST_EINVAL, std::string(where) + ": self-loop or vertex id out of range");
   if (e & ERR_UNSORTED) throw Error(KMST_EUNSORTED, std::string(where) + ": records not sorted by (src, dst)");
@@ -259,10 +279,16 @@ void Solver::check_errors(const char* where) {
     throw Error(KMST_EASYMMETRIC, std::string(where) + ": a record has no reverse twin");
 }
 
-void Solver::setup() {
+void Solver::setup(bool filter, bool plan_classes) {
   fill_u32(st, ctr, 64, 0);
+  pivots = WeightPivots{};
+  if (plan_classes) plan_pivots();
+  // init_parent_array (hybrid_boruvka_edgefilter.hpp:332-352): vertices with
+  // records, marked during validation
+  if (filter) fill_u32(st, has_edge, n0, 0);
   validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW,
-                   reinterpret_cast<u64*>(ctr + C_HSUM64));
+                   reinterpret_cast<u64*>(ctr + C_HSUM64), pivots,
+                   reinterpret_cast<u64*>(ctr + C_CLASS64), filter ? has_edge : nullptr);
   check_errors("input");
   if (u64(C.host_counters[C_FWD]) * 2 != m0 || counter64(C_HSUM64) != counter64(C_HSUM64 + 2))
     throw Error(KMST_EASYMMETRIC, "input: records are not symmetric (src,dst,w) <-> (dst,src,w)");
@@ -368,6 +394,94 @@ u32 Solver::pick_pivot(RecView r) {
   return h[h.size() / 2];  // median sampled weight (select_pivot, distributed_partitioning.hpp:92-120)
 }
 
+// All pivots of the top levels of the recursion from one sample of the input
+// weights: level l splits the light part of level l-1 at its median
+// (select_pivot, distributed_partitioning.hpp:92-120), so the pivots are
+// sample quantiles near 1/2, 1/4, 1/8, ...  The chain stops where the
+// reference would stop splitting: the light part's average degree is at most
+// the filter threshold, it has at most 1000 records, or its share of the part
+// exceeds 0.85 (hybrid_boruvka_edgefilter.hpp:233-240, filter_helpers.hpp:130-132).
+void Solver::plan_pivots() {
+  const u32 k = static_cast<u32>(std::min<u64>(4095, m0));
+  sample_weights(st, nullptr, in_w, m0, k, 32, samples);
+  std::vector<u32> h(k);
+  KMST_HIP(hipMemcpyAsync(h.data(), samples, k * 4, hipMemcpyDeviceToHost, st));
+  KMST_HIP(hipStreamSynchronize(st));
+  std::sort(h.begin(), h.end());
+  std::vector<u32> chain;  // descending
+  size_t cnt = h.size();
+  double est_m = double(m0);
+  while (chain.size() < MAX_PIVOTS && cnt > 0) {
+    if (est_m / double(n0) <= double(filter_threshold) || est_m <= 1000.0) break;
+    const u32 p = h[cnt / 2];
+    const size_t light = std::upper_bound(h.begin(), h.begin() + cnt, p) - h.begin();
+    chain.push_back(p);
+    if (double(light) / double(cnt) > 0.85 || light == 0) break;
+    est_m = est_m * double(light) / double(cnt);
+    cnt = light;
+  }
+  pivots.K = static_cast<u32>(chain.size());
+  for (u32 j = 0; j < pivots.K; ++j) pivots.q[j] = chain[pivots.K - 1 - j];
+}
+
+// The top K levels of the filter recursion in one pass over the input: the
+// class scatter writes weight classes 0..K-1 (class 0 sorted) into Y, class K
+// (above every pivot) stays in the input.  Then, in the recursion's order:
+// class 0 recurses as the lightest part; class c = 1..K-1 is the heavy half of
+// the level with pivot q[c-1]; class K is the top level's heavy half, streamed
+// from the input again with key >= (q[K-1] + 1) << wshift.
+void Solver::filter_classes() {
+  const u32 K = pivots.K;
+  u64 cnt[MAX_CLASSES] = {};
+  for (u32 c = 0; c <= K; ++c) cnt[c] = counter64(C_CLASS64 + 2 * c);
+  // only the two lightest classes are written out; the heavier ones are
+  // taken from the input when their turn comes, by then mostly through the
+  // ranges of the few vertices outside the largest component
+  const u32 J = std::min<u32>(K, 2);
+  WeightPivots PJ{};
+  PJ.K = J;
+  for (u32 j = 0; j < J; ++j) PJ.q[j] = pivots.q[j];
+  ClassBases B{};
+  u64 written = 0;
+  for (u32 c = 0; c < J; ++c) {
+    B.base[c] = written;
+    written += cnt[c];
+  }
+  const u32 wshift = packed ? 2 * vbits : 32;
+  const RecView input{const_cast<u32*>(in_s), const_cast<u32*>(in_d), packed ? nullptr : val0, m0,
+                      packed ? in_w : nullptr};
+  class_scatter(st, in_s, in_d, keys_of(input), m0, PJ, wshift, B, written, Y.s, Y.d, Y.v, lb,
+                reinterpret_cast<u64*>(ctr + C_CURSOR64), ctr + C_ERR);
+  check_errors("split");
+  // level l (0 = top) splits classes 0..K-l at pivot q[K-1-l]
+  std::vector<FilterStat> level(K);
+  u64 part = m0;
+  for (u32 l = 0; l < K; ++l) {
+    const u32 c = K - l;  // heavy class of level l
+    level[l] = FilterStat{static_cast<int>(l), part, part - cnt[c], pivots.q[K - 1 - l], false, 0};
+    C.stats.filter.push_back(level[l]);
+    part -= cnt[c];
+  }
+  const bool stop = double(cnt[0]) / double(cnt[0] + cnt[1]) > 0.85;
+  filter_rec(Y.at(0, cnt[0]), X.at(0, cnt[0]), stop, static_cast<int>(K));
+  if (J >= 2 && cnt[1] > 0) {
+    C.stats.filter.push_back(level[K - 1]);
+    RecView cls = Y.at(B.base[1], cnt[1]);
+    filter_heavy(cls, X.at(0, cnt[1]), cls, static_cast<int>(K - 1));
+  }
+  // class c holds keys in [(q[c-1] + 1) << wshift, ((q[c] + 1) << wshift) - 1]
+  auto key_upper = [&](u32 q) {
+    return (u64(q) + 1) >= (u64{1} << (64 - wshift)) ? ~0ull : ((u64(q) + 1) << wshift) - 1;
+  };
+  for (u32 c = J; c <= K; ++c) {
+    if (cnt[c] == 0) continue;
+    C.stats.filter.push_back(level[K - c]);
+    const u64 kmin = (u64(pivots.q[c - 1]) + 1) << wshift;
+    const u64 kmax = c < K ? key_upper(pivots.q[c]) : ~0ull;
+    filter_heavy(input, X, Y, static_cast<int>(K - c), kmin, kmax, /*from_input=*/true);
+  }
+}
+
 u64 Solver::partition(RecView in, RecView out, u32 pivot, bool write_heavy) {
   split_light_heavy(st, in.s, in.d, keys_of(in), in.m, pivot, packed ? 2 * vbits : 32,
                     write_heavy, out.s, out.d, out.v, lb, ctr + C_LIGHT, ctr + C_ERR);
@@ -382,11 +496,30 @@ u64 Solver::partition(RecView in, RecView out, u32 pivot, bool write_heavy) {
 // (boruvka_recursive_setup, hybrid_boruvka_edgefilter.hpp:173-225).
 bool Solver::one_component() {
   count_roots(st, gpar, has_edge, n0, ctr + C_NNEXT);
-  read_counters();
+  const u32 k = std::min<u32>(GPAR_SAMPLES, n0);
+  sample_weights(st, nullptr, gpar, n0, k, 32, samples);
+  KMST_HIP(hipMemcpyAsync(C.host_counters, ctr, 256, hipMemcpyDeviceToHost, st));
+  KMST_HIP(hipMemcpyAsync(C.host_counters + HOST_SAMPLE_WORD, samples, k * 4,
+                          hipMemcpyDeviceToHost, st));
+  KMST_HIP(hipStreamSynchronize(st));
+  std::vector<u32> h(C.host_counters + HOST_SAMPLE_WORD, C.host_counters + HOST_SAMPLE_WORD + k);
+  std::sort(h.begin(), h.end());
+  size_t best = 0;
+  for (size_t i = 0; i < h.size();) {
+    size_t j = i;
+    while (j < h.size() && h[j] == h[i]) ++j;
+    if (j - i > best) {
+      best = j - i;
+      giant_root = h[i];
+    }
+    i = j;
+  }
+  giant_share = k ? double(best) / double(k) : 0.0;
   return C.host_counters[C_NNEXT] <= 1;
 }
 
-void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth, u64 key_min) {
+void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth, u64 key_min,
+                          u64 key_max, bool from_input) {
   if (heavy.m == 0) return;
   // the light edges left at most one component with edges: every heavy
   // edge is a self-loop (hybrid_boruvka_edgefilter.hpp:289-296)
@@ -394,8 +527,39 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
     C.stats.filter.back().skipped = true;
     return;
   }
-  relabel_compact(st, heavy.s, heavy.d, keys_of(heavy), key_min, heavy.m, gpar, scratch.s,
-                  scratch.d, scratch.v, lb, ctr + C_MNEXT, ctr + C_ERR);
+  bool gathered = false;
+  if (from_input && giant_share >= 0.25) {
+    // few vertices outside the largest component: scan only their ranges
+    // (the per-vertex arrays of the rounds are free here)
+    outside_list(st, gpar, has_edge, n0, giant_root, in_s, m0, best_dst, parent, label,
+                 ctr + C_OUTSIDE);
+    read_counters();
+    const u64 n_list = C.host_counters[C_OUTSIDE];
+    const u64 total = counter64(C_OUTSIDE + 2);
+    if (n_list == 0) {
+      C.stats.filter.back().skipped = true;
+      return;
+    }
+    if (3 * total <= m0 && 2 * total <= scratch.m) {
+      outside_gather(st, best_dst, parent, label, static_cast<u32>(n_list), total, in_d,
+                     keys_of(heavy), key_min, key_max, gpar, giant_root, scratch.s, scratch.d,
+                     scratch.v, scratch.m, ctr + C_MNEXT, ctr + C_ERR);
+      gathered = true;
+    }
+  }
+  if (!gathered) {
+    // most vertices usually sit in one component by now: mark it in a bitmap
+    // (n/8 bytes, cache resident) so that records inside it are dropped
+    // without the key read and the two parent-array gathers (best_val is free
+    // here)
+    const u64* giant = nullptr;
+    if (giant_share >= 0.25) {
+      component_bits(st, gpar, n0, giant_root, best_val);
+      giant = best_val;
+    }
+    relabel_compact(st, heavy.s, heavy.d, keys_of(heavy), key_min, heavy.m, gpar, scratch.s,
+                    scratch.d, scratch.v, lb, ctr + C_MNEXT, ctr + C_ERR, giant, n0, key_max);
+  }
   check_errors("filter");
   const u64 Hk = C.host_counters[C_MNEXT];
   if (g_ktimer) g_ktimer->adjust_last(K_RELABEL_SCATTER, 16.0 * Hk);
@@ -440,33 +604,19 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
   C.stats.algorithm = cfg.algorithm;
   filter_threshold = cfg.filter_threshold;
   allocate();
-  setup();
+  const double avg_degree = double(m0) / double(std::max<u32>(n0, 1));
+  const bool filter = cfg.algorithm == KMST_ALGO_FILTER_BORUVKA;
+  const bool filter_base = avg_degree <= double(filter_threshold) || m0 <= 1000;
+  setup(filter, filter && !filter_base);
   const RecView input{const_cast<u32*>(in_s), const_cast<u32*>(in_d), packed ? nullptr : val0, m0,
                       packed ? in_w : nullptr};
-  if (cfg.algorithm == KMST_ALGO_FILTER_BORUVKA) {
+  if (filter) {
     iota_u32(st, gpar, n0);
-    fill_u32(st, has_edge, n0, 0);
-    mark_present(st, in_s, m0, has_edge);  // init_parent_array (hybrid_boruvka_edgefilter.hpp:332-352)
-    const double avg_degree = double(m0) / double(std::max<u32>(n0, 1));
-    if (avg_degree <= double(filter_threshold) || m0 <= 1000) {
+    if (filter_base || pivots.K == 0) {
       C.stats.filter.push_back(FilterStat{0, m0, 0, 0, true, 0});
       rounds(input, X, Y, true);
     } else {
-      // The top level reads the read-only input and writes only its light
-      // records (to Y); its heavy half is streamed from the input again,
-      // filtered by key >= key_min, if the light edges leave more than one
-      // component.
-      const u32 pivot = pick_pivot(input);
-      const u64 L = partition(input, Y, pivot, /*write_heavy=*/false);
-      const bool stop_rec = double(L) / double(m0) > 0.85;
-      C.stats.filter.push_back(FilterStat{0, m0, L, pivot, false, 0});
-      filter_rec(Y.at(0, L), X.at(0, L), stop_rec, 1);
-      const u32 wshift = packed ? 2 * vbits : 32;
-      const u64 max_w = C.stats.max_weight;
-      if (L < m0 && pivot < max_w) {
-        C.stats.filter.push_back(C.stats.filter.front());
-        filter_heavy(input, X, Y, 0, (u64(pivot) + 1) << wshift);
-      }
+      filter_classes();
     }
   } else {
     rounds(input, X, Y, false);

@@ -40,6 +40,11 @@ struct SolveStats {
 
 extern thread_local std::string g_last_error;
 
+// pinned host words: counters in [0, 64), vertex samples from HOST_SAMPLE_WORD
+constexpr u32 GPAR_SAMPLES = 256;
+constexpr u32 HOST_SAMPLE_WORD = 64;
+constexpr size_t HOST_COUNTER_BYTES = (HOST_SAMPLE_WORD + GPAR_SAMPLES) * 4;
+
 }  // namespace kmst
 
 struct kmst_context {
@@ -50,7 +55,7 @@ struct kmst_context {
 
   int device = 0;
   hipStream_t stream = nullptr;
-  kmst::u32* host_counters = nullptr;  // pinned, 256 bytes
+  kmst::u32* host_counters = nullptr;  // pinned, HOST_COUNTER_BYTES
   bool timing = false;
   kmst::KernelTimer timer;
   kmst::SolveStats stats;
