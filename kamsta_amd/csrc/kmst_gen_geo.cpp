@@ -7,14 +7,21 @@
 // point coordinates; {u, v} is an edge iff the integer squared distance is at
 // most r^2 with r = sqrt(avg_degree / (pi n)) (in fixed point).  The test is
 // exact integer arithmetic, so the graph does not depend on floating point.
+// Vertex ids are assigned cell by cell (row-major over a grid of cells of
+// side >= r, points of a cell in generation order), the way KaGen numbers the
+// vertices of its geometric generators, so that neighbours have nearby ids.
 //
 // RHG(n, avg_degree, gamma): threshold random hyperbolic graph.  alpha =
 // (gamma - 1) / 2, disk radius R = 2 ln((2/pi) xi^2 n / avg_degree) with
 // xi = alpha / (alpha - 1/2); radii from the density alpha sinh(alpha r) /
-// (cosh(alpha R) - 1), angles uniform.  {u, v} is an edge iff
-// cosh r_u cosh r_v - sinh r_u sinh r_v cos(theta_u - theta_v) <= cosh R,
+// (cosh(alpha R) - 1), angles uniform.  {u, v} is an edge iff their
+// hyperbolic distance is at most R, i.e. (law of cosines in haversine form,
+// well conditioned for the tiny angles of boundary pairs)
+//   2 sin^2(dtheta / 2) sinh r_u sinh r_v <= cosh R - cosh(r_u - r_v),
 // evaluated in double precision with the lower id first, so the test is the
-// same from both sides.  Candidates come from radial bands sorted by angle.
+// same from both sides.  Candidates come from radial bands sorted by angle;
+// a band's angular window is taken at its smallest member radius, where it is
+// widest (the bound above decreases in r_v), with a relative margin of 1e-9.
 // Vertex ids follow the angular order (neighbours have nearby ids).
 #include <algorithm>
 #include <cmath>
@@ -75,10 +82,17 @@ Out rgg2d(u64 n, double avg_degree, u64 seed) {
   std::vector<u64> start(cells * cells + 1, 0);
   for (u64 i = 0; i < n; ++i) start[cell_of(y[i]) * cells + cell_of(x[i]) + 1]++;
   for (u64 c = 0; c < cells * cells; ++c) start[c + 1] += start[c];
-  std::vector<u32> bucket(n);
+  // renumber: vertex id = position in cell order
   {
     std::vector<u64> fill(start.begin(), start.end() - 1);
-    for (u64 i = 0; i < n; ++i) bucket[fill[cell_of(y[i]) * cells + cell_of(x[i])]++] = u32(i);
+    std::vector<u32> nx(n), ny(n);
+    for (u64 i = 0; i < n; ++i) {
+      const u64 p = fill[cell_of(y[i]) * cells + cell_of(x[i])]++;
+      nx[p] = x[i];
+      ny[p] = y[i];
+    }
+    x.swap(nx);
+    y.swap(ny);
   }
   Out o;
   o.pairs.reserve(static_cast<size_t>(avg_degree * double(n) * 1.1));
@@ -92,7 +106,7 @@ Out rgg2d(u64 n, double avg_degree, u64 seed) {
         if (gx < 0 || gy < 0 || gx >= long(cells) || gy >= long(cells)) continue;
         const u64 c = u64(gy) * cells + u64(gx);
         for (u64 k = start[c]; k < start[c + 1]; ++k) {
-          const u32 j = bucket[k];
+          const u32 j = static_cast<u32>(k);  // ids are cell-ordered
           if (j == i) continue;
           const u64 ddx = x[i] > x[j] ? x[i] - x[j] : x[j] - x[i];
           const u64 ddy = y[i] > y[j] ? y[i] - y[j] : y[j] - y[i];
@@ -124,25 +138,23 @@ Out rhg(u64 n, double avg_degree, double gamma, u64 seed) {
   std::sort(pts.begin(), pts.end(), [](const P& a, const P& b) {
     return a.theta < b.theta || (a.theta == b.theta && a.r < b.r);
   });
-  std::vector<double> chr(n), shr(n);
-  for (u64 i = 0; i < n; ++i) {
-    chr[i] = std::cosh(pts[i].r);
-    shr[i] = std::sinh(pts[i].r);
-  }
+  std::vector<double> shr(n);
+  for (u64 i = 0; i < n; ++i) shr[i] = std::sinh(pts[i].r);
   const double coshR = std::cosh(R);
   auto adjacent = [&](u64 a, u64 b) {  // a < b: same expression from both sides
-    const double c = chr[a] * chr[b] - shr[a] * shr[b] * std::cos(pts[a].theta - pts[b].theta);
-    return c <= coshR;
+    const double h = std::sin(0.5 * (pts[a].theta - pts[b].theta));
+    return 2.0 * h * h * shr[a] * shr[b] <= coshR - std::cosh(pts[a].r - pts[b].r);
   };
-  // radial bands of width about 1 (more near the boundary where points are)
-  std::vector<double> edges_r;
+  // radial bands of width about 1
   const int nb = std::max(1, static_cast<int>(std::ceil(R)));
-  for (int b = 0; b <= nb; ++b) edges_r.push_back(R * b / nb);
   std::vector<std::vector<u32>> band(nb);  // ids sorted by angle (ids are angle-ordered)
+  std::vector<double> band_rmin(nb, R);
   for (u64 i = 0; i < n; ++i) {
     int b = static_cast<int>(pts[i].r / R * nb);
     if (b >= nb) b = nb - 1;
+    if (b < 0) b = 0;
     band[b].push_back(static_cast<u32>(i));
+    band_rmin[b] = std::min(band_rmin[b], pts[i].r);
   }
   Out o;
   o.pairs.reserve(static_cast<size_t>(avg_degree * double(n) * 1.2));
@@ -152,16 +164,13 @@ Out rhg(u64 n, double avg_degree, double gamma, u64 seed) {
     const double ti = pts[i].theta;
     for (int b = 0; b < nb; ++b) {
       if (band[b].empty()) continue;
-      const double rb = edges_r[b];  // inner radius: the widest angular window in the band
-      double dmax;
-      const double num = chr[i] * std::cosh(rb) - coshR;
-      const double den = shr[i] * std::sinh(rb);
-      if (rb <= 0.0 || den <= 0.0 || num <= 0.0) {
-        dmax = M_PI;
-      } else {
-        const double c = num / den;
-        dmax = c >= 1.0 ? 0.0 : std::acos(std::max(-1.0, c));
-        dmax = std::min(M_PI, dmax * 1.0000001 + 1e-12);
+      const double rb = band_rmin[b];  // smallest radius: the widest angular window
+      double dmax = M_PI;
+      const double den = 2.0 * shr[i] * std::sinh(rb);
+      if (den > 0.0) {
+        const double x = (coshR - std::cosh(pts[i].r - rb)) / den;  // sin^2(dmax / 2)
+        if (x < 1.0) dmax = std::min(M_PI, 2.0 * std::asin(std::sqrt(std::max(0.0, x))) *
+                                               (1.0 + 1e-9) + 1e-15);
       }
       const auto& ids = band[b];
       if (dmax >= M_PI) {

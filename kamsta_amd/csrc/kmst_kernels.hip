@@ -362,24 +362,27 @@ __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src
     val[i] = (u64(wi) << 32) | i;
     return;
   }
-  u32 k = 0;  // equal-weight records of the own run before i
+  // within runs of equal (src, dst) (multigraph inputs) the k-th record of
+  // weight w pairs with the k-th twin of weight w
+  u32 before = 0;
   for (u64 j = i; j > 0; --j) {
     if (src[j - 1] != s || dst[j - 1] != d) break;
-    k += (w[j - 1] == wi) ? 1u : 0u;
+    before += (w[j - 1] == wi) ? 1u : 0u;
   }
   u64 lo = off[d], hi = off[d + 1];
   const u64 end = hi;
   while (lo < hi) {
     const u64 mid = (lo + hi) >> 1;
-    if (dst[mid] < s) lo = mid + 1; else hi = mid;
+    if (dst[mid] < s) lo = mid + 1;
+    else hi = mid;
   }
   for (u64 j = lo; j < end && dst[j] == s; ++j) {
     if (w[j] == wi) {
-      if (k == 0) {
+      if (before == 0) {
         val[i] = (u64(wi) << 32) | j;
         return;
       }
-      --k;
+      --before;
     }
   }
   val[i] = VAL_NONE;
