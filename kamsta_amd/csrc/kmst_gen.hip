@@ -130,10 +130,11 @@ void gen_gnm_device(kmst_context& C, u64 n, u64 m, u32 max_w, u64 seed, u32* out
     u64* c2 = static_cast<u64*>(C.xv.get(cap * 8));
     u32* flag = static_cast<u32*>(C.gen_d.get(2 * k * 4));
     u32* rank = flag + k;
-    const u64 hist_elems = sort_hist_elems(std::max(k, 2 * m));
-    u32* hist = static_cast<u32*>(C.hist.get(hist_elems * 4));
-    u32* scan_scratch = static_cast<u32*>(
-        C.scan_scratch.get(scan_scratch_elems(std::max(hist_elems, k)) * 4 + 64));
+    const u64 sort_m = std::max(k, 2 * m);
+    C.sort_work.bind(st, static_cast<u32*>(C.hist.get(sort_work_words(sort_m) * 4)), sort_m);
+    u32* scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(k) * 4 + 64));
+    u32* err = ctr + 8;  // ctr[0] receives scan totals
+    fill_u32(st, ctr, 64, 0);
     const u32 grid = static_cast<u32>(std::min<u64>((k + BLOCK - 1) / BLOCK, 65536));
     {
       KTScope t(st, K_GEN, 16.0 * k);
@@ -142,7 +143,7 @@ void gen_gnm_device(kmst_context& C, u64 n, u64 m, u32 max_w, u64 seed, u32* out
     }
     u32 *sa = a, *sb = b;
     u64* sc = cv;
-    if (radix_sort(st, a, b, cv, a2, b2, c2, k, pair_bits, 2 * pair_bits, hist, scan_scratch)) {
+    if (radix_sort(st, a, b, cv, a2, b2, c2, k, pair_bits, 2 * pair_bits, C.sort_work, err)) {
       sa = a2;
       sb = b2;
       sc = c2;
@@ -174,7 +175,7 @@ void gen_gnm_device(kmst_context& C, u64 n, u64 m, u32 max_w, u64 seed, u32* out
       KMST_HIP(hipGetLastError());
     }
     const u64 N = 2 * m;
-    if (radix_sort(st, rs, rd, rw, ts, td, tw, N, pair_bits, 2 * pair_bits, hist, scan_scratch)) {
+    if (radix_sort(st, rs, rd, rw, ts, td, tw, N, pair_bits, 2 * pair_bits, C.sort_work, err)) {
       rs = ts;
       rd = td;
       rw = tw;
@@ -187,7 +188,10 @@ void gen_gnm_device(kmst_context& C, u64 n, u64 m, u32 max_w, u64 seed, u32* out
                       st>>>(rw, N, out_w);
       KMST_HIP(hipGetLastError());
     }
+    u32 err_host = 0;
+    KMST_HIP(hipMemcpyAsync(&err_host, err, 4, hipMemcpyDeviceToHost, st));
     KMST_HIP(hipStreamSynchronize(st));
+    if (err_host) throw Error(KMST_EDEVICE, "GNM: sort wait expired");
     return;
   }
   throw Error(KMST_EINVAL, "GNM: could not draw m distinct pairs");

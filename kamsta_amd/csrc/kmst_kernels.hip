@@ -175,7 +175,8 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
                                                          u32* max_w, u64* hsum,
                                                          const WeightPivots P,
                                                          u64* class_counts,
-                                                         u32* __restrict__ present) {
+                                                         u32* __restrict__ seg_start,
+                                                         u32* __restrict__ seg_end) {
   __shared__ u64 s_red[WAVES][MAX_PIVOTS + 4];
   const int lane = lane_id();
   u32 bad = 0, fwd = 0, mw = 0, seen = 0;
@@ -183,10 +184,15 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
   u32 above[MAX_PIVOTS];  // records with w > q[j]
 #pragma unroll
   for (int j = 0; j < MAX_PIVOTS; ++j) above[j] = 0;
-  auto one = [&](u32 s, u32 d, u32 w, u32 ps, u32 pd, bool has_prev) {
+  auto one = [&](u64 i, u32 s, u32 d, u32 w, u32 ps, u32 pd, bool has_prev) {
     if (s >= n || d >= n || s == d) bad |= ERR_INVALID;
     if (has_prev && (ps > s || (ps == s && pd > d))) bad |= ERR_UNSORTED;
-    if (present && s < n && (!has_prev || ps != s)) present[s] = 1u;
+    if (seg_end && (!has_prev || ps != s)) {
+      // record i opens s's range and closes the previous vertex's
+      if (s < n) seg_start[s] = static_cast<u32>(i);
+      if (has_prev && ps < n) seg_end[ps] = static_cast<u32>(i);
+    }
+    if (seg_end && i + 1 == m && s < n) seg_end[s] = static_cast<u32>(m);
     mw = w > mw ? w : mw;
     ++seen;
     const bool f = s < d;
@@ -223,20 +229,20 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
             ps = src[ch * 4 - 1];
             pd = dst[ch * 4 - 1];
           }
-          one(vs[u].x, vd[u].x, vw[u].x, ps, pd, ch > 0);
-          one(vs[u].y, vd[u].y, vw[u].y, vs[u].x, vd[u].x, true);
-          one(vs[u].z, vd[u].z, vw[u].z, vs[u].y, vd[u].y, true);
-          one(vs[u].w, vd[u].w, vw[u].w, vs[u].z, vd[u].z, true);
+          one(ch * 4, vs[u].x, vd[u].x, vw[u].x, ps, pd, ch > 0);
+          one(ch * 4 + 1, vs[u].y, vd[u].y, vw[u].y, vs[u].x, vd[u].x, true);
+          one(ch * 4 + 2, vs[u].z, vd[u].z, vw[u].z, vs[u].y, vd[u].y, true);
+          one(ch * 4 + 3, vs[u].w, vd[u].w, vw[u].w, vs[u].z, vd[u].z, true);
         }
       }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       for (u64 i = full * 4; i < m; ++i)
-        one(src[i], dst[i], wt[i], i ? src[i - 1] : 0u, i ? dst[i - 1] : 0u, i > 0);
+        one(i, src[i], dst[i], wt[i], i ? src[i - 1] : 0u, i ? dst[i - 1] : 0u, i > 0);
     }
   } else {
     for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK)
-      one(src[i], dst[i], wt[i], i ? src[i - 1] : 0u, i ? dst[i - 1] : 0u, i > 0);
+      one(i, src[i], dst[i], wt[i], i ? src[i - 1] : 0u, i ? dst[i - 1] : 0u, i > 0);
   }
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) {
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
 
 void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m, u64 n,
                       u32* err, u32* fwd_count, u32* max_w, u64* hsum, const WeightPivots& P,
-                      u64* class_counts, u32* present) {
+                      u64* class_counts, u32* seg_start, u32* seg_end) {
   if (m == 0) return;
   KTScope t(st, K_VALIDATE, 12.0 * m);
   const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst) |
@@ -295,10 +301,10 @@ void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32*
   const u32 grid = grid_for((m + 3) / 4, 4096);
   if (vec)
     validate_kernel<true><<<grid, BLOCK, 0, st>>>(src, dst, w, m, n, err, fwd_count, max_w, hsum,
-                                                  P, class_counts, present);
+                                                  P, class_counts, seg_start, seg_end);
   else
     validate_kernel<false><<<grid, BLOCK, 0, st>>>(src, dst, w, m, n, err, fwd_count, max_w, hsum,
-                                                   P, class_counts, present);
+                                                   P, class_counts, seg_start, seg_end);
   KMST_HIP(hipGetLastError());
 }
 
@@ -823,81 +829,166 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
 }
 
 // ============================================================== K6 sort
-// Classic LSD pass: per-tile digit histogram, exclusive scan over
-// (digit, tile), stable scatter.  Each wave owns a contiguous sub-tile and
-// ranks its items in index order with 8-ballot digit matching, so the pass is
-// stable without block-wide synchronisation per item.
+// Stable LSD radix sort with 8-bit digits and one kernel per pass
+// ("onesweep"): one read of the keys builds the digit histograms of every
+// pass, their exclusive scans give each digit's global base, and inside a
+// scatter pass each tile learns how many records of each digit the tiles
+// before it hold through a decoupled look-back per digit.  Each wave
+// owns a contiguous sub-tile and ranks its items in index order with 8-ballot
+// digit matching, so a pass is stable; the tile is staged in LDS in (digit,
+// input order) and leaves in runs of consecutive addresses: (src, dst) first,
+// then the keys through the same LDS buffer.  No host synchronisation.
+constexpr u32 MAX_PASSES = 8;
+static_assert(RADIX == BLOCK, "one thread per digit");
+
+// Sort key: (s << dbits) | d, or s alone when dbits is 0 (d, if present, is
+// then only carried along).
 template <bool HAS_D>
-__device__ __forceinline__ u64 sort_key(const u32* s, const u32* d, u64 i, u32 dbits) {
-  return HAS_D ? ((u64(s[i]) << dbits) | d[i]) : u64(s[i]);
+__device__ __forceinline__ u64 sort_key_of(u32 s, u32 d, u32 dbits) {
+  return (HAS_D && dbits) ? ((u64(s) << dbits) | d) : u64(s);
 }
 
+// ghist[p * RADIX + x] = number of records whose digit p is x.  Per-wave
+// LDS histograms; each thread loads UP_RPT keys before counting them.
+constexpr int UP_RPT = 8;
+
 template <bool HAS_D>
-__global__ __launch_bounds__(BLOCK) void radix_hist_kernel(const u32* __restrict__ s,
-                                                           const u32* __restrict__ d, u64 m,
-                                                           u32 dbits, u32 shift,
-                                                           u32* __restrict__ hist, u32 tiles) {
-  __shared__ u32 h[WAVES][RADIX];
-  for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&h[0][0])[i] = 0;
+__global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const u32* __restrict__ s,
+                                                              const u32* __restrict__ d, u64 m,
+                                                              u32 dbits, u32 passes,
+                                                              u32* __restrict__ ghist) {
+  __shared__ u32 h[WAVES][MAX_PASSES][RADIX];
+  for (int i = threadIdx.x; i < WAVES * MAX_PASSES * RADIX; i += BLOCK) (&h[0][0][0])[i] = 0;
   __syncthreads();
-  const int w = wave_id();
-  const u64 base = u64(blockIdx.x) * SORT_TILE;
-#pragma unroll 4
-  for (int k = 0; k < SORT_IPT; ++k) {
-    const u64 e = base + u64(k) * BLOCK + threadIdx.x;
-    if (e < m) {
-      const u32 digit = static_cast<u32>((sort_key<HAS_D>(s, d, e, dbits) >> shift) & (RADIX - 1));
-      atomicAdd(&h[w][digit], 1u);
+  const int w = wave_id(), lane = lane_id();
+  const u64 stride = u64(gridDim.x) * (BLOCK * UP_RPT);
+  for (u64 base = u64(blockIdx.x) * (BLOCK * UP_RPT); base < m; base += stride) {  // uniform
+    u64 key[UP_RPT];
+#pragma unroll
+    for (int k = 0; k < UP_RPT; ++k) {
+      const u64 e = min(base + u64(k) * BLOCK + threadIdx.x, m - 1);
+      key[k] = sort_key_of<HAS_D>(s[e], HAS_D ? d[e] : 0u, dbits);
+    }
+#pragma unroll
+    for (int k = 0; k < UP_RPT; ++k) {
+      const bool valid = base + u64(k) * BLOCK + threadIdx.x < m;
+      const u64 vmask = __ballot(valid);
+      for (u32 p = 0; p < passes; ++p) {
+        const u32 digit = static_cast<u32>((key[k] >> (p * RADIX_BITS)) & (RADIX - 1));
+        // a wave whose records share the digit adds once (the usual case for
+        // the high digits); lane 0 is valid whenever any lane is
+        const u32 first = __builtin_amdgcn_readfirstlane(digit);
+        const u64 same = __ballot(valid && digit == first);
+        if (same == vmask) {
+          if (lane == 0 && vmask) h[w][p][first] += static_cast<u32>(__popcll(vmask));
+        } else if (valid) {
+          atomicAdd(&h[w][p][digit], 1u);
+        }
+      }
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < RADIX; i += BLOCK) {
+  for (u32 i = threadIdx.x; i < passes * RADIX; i += BLOCK) {
     u32 c = 0;
 #pragma unroll
-    for (int ww = 0; ww < WAVES; ++ww) c += h[ww][i];
-    hist[u64(i) * tiles + blockIdx.x] = c;
+    for (int ww = 0; ww < WAVES; ++ww) c += (&h[ww][0][0])[i];
+    if (c) atomicAdd(ghist + i, c);
   }
 }
 
-// Scatter pass: rank every item inside its wave (8-ballot digit matching, so
-// the pass is stable), place the tile in LDS in (digit, input order), then
-// write it out row by row so that each digit's run leaves as consecutive
-// addresses.  In the last pass the LDS tile is sorted by the full key, and
-// with DEDUP the first record of every run of equal (src, dst) inside the
-// tile keeps the run's lightest key while the others are marked dead
-// (EdgeProcessor::remove_duplicates, duplicate_detection.hpp:26-64; runs
-// split across tiles keep one survivor per tile, which later rounds remove).
+// base[p * RADIX + x] = number of records whose digit p is below x.
+__global__ __launch_bounds__(BLOCK) void radix_digit_base_kernel(const u32* __restrict__ ghist,
+                                                                 u32 passes,
+                                                                 u32* __restrict__ base) {
+  __shared__ u32 tmp[WAVES];
+  for (u32 p = 0; p < passes; ++p) {
+    u32 total;
+    const u32 x = ghist[p * RADIX + threadIdx.x];
+    base[p * RADIX + threadIdx.x] = block_exclusive_sum(x, total, tmp);
+  }
+}
+
+// Sort status word: bits 63..62 flag (1 aggregate, 2 inclusive prefix),
+// bits 61..48 the pass epoch, bits 47..0 the count.  A word from another
+// epoch reads as "not published", so the array is zeroed only when it is
+// allocated.
+constexpr u32 SORT_EPOCH_BITS = 14;
+constexpr u64 SORT_VAL = (u64{1} << 48) - 1;
+
+__device__ __forceinline__ u64 sort_status(u64 flag, u32 epoch, u64 value) {
+  return (flag << 62) | (u64(epoch) << 48) | (value & SORT_VAL);
+}
+
+// Serial look-back of one digit over the tiles before `tile`; returns the
+// number of records with this digit in them and publishes the inclusive
+// count.  Bounded like lookback_exclusive.
+__device__ __forceinline__ u64 digit_lookback(u64* status, u32 epoch, u32 tile, u32 digit,
+                                              u64 count, u32* err, bool& ok) {
+  ok = true;
+  u64* me = status + u64(tile) * RADIX + digit;
+  if (tile == 0) {
+    lb_store(me, sort_status(2, epoch, count));
+    return 0;
+  }
+  lb_store(me, sort_status(1, epoch, count));
+  u64 excl = 0;
+  i64 t = static_cast<i64>(tile) - 1;
+  u32 spins = 0;
+  while (t >= 0) {
+    const u64 w = lb_load(status + u64(t) * RADIX + digit);
+    const bool current = ((w >> 48) & ((1u << SORT_EPOCH_BITS) - 1)) == epoch;
+    const u64 flag = current ? (w >> 62) : 0;
+    if (flag == 0) {
+      if (++spins > LB_SPIN_LIMIT) {
+        atomicOr(err, ERR_SPIN);
+        ok = false;
+        return 0;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += w & SORT_VAL;
+    if (flag == 2) break;
+    --t;
+  }
+  lb_store(me, sort_status(2, epoch, excl + count));
+  return excl;
+}
+
 template <bool HAS_D, bool HAS_V, bool DEDUP>
-__global__ __launch_bounds__(BLOCK) void radix_scatter_kernel(
+__global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in,
     u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out, u64 m, u32 dbits,
-    u32 shift, const u32* __restrict__ hist_scan, u32 tiles) {
+    u32 shift, const u32* __restrict__ digit_base, u64* status, u32 epoch,
+    unsigned long long* tile_counter, unsigned long long tile_base_id, u32* err) {
   __shared__ u32 wcnt[WAVES][RADIX];
   __shared__ u32 delta[RADIX];  // global position - tile position, per digit
-  __shared__ u32 lds_s[SORT_TILE];
-  __shared__ u32 lds_d[HAS_D ? SORT_TILE : 1];
-  __shared__ u64 lds_v[HAS_V ? SORT_TILE : 1];
+  __shared__ u64 stage[SORT_TILE];
   __shared__ u32 scan_tmp[WAVES];
+  __shared__ u32 s_ok;
   for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&wcnt[0][0])[i] = 0;
+  if (threadIdx.x == 0) s_ok = 1;
+  // dynamic tile ids in dispatch order from a counter that only grows
+  __shared__ u32 s_tile;
+  if (threadIdx.x == 0) s_tile = static_cast<u32>(atomicAdd(tile_counter, 1ull) - tile_base_id);
   __syncthreads();
+  const u32 tile = s_tile;
   const int lane = lane_id(), w = wave_id();
-  const u64 tile_base = u64(blockIdx.x) * SORT_TILE;
+  const u64 tile_base = u64(tile) * SORT_TILE;
   const u64 base = tile_base + u64(w) * (WAVE * SORT_IPT);
   u32 ks[SORT_IPT], kd[SORT_IPT], dig[SORT_IPT], rank[SORT_IPT];
   u64 kv[SORT_IPT];
 #pragma unroll
   for (int k = 0; k < SORT_IPT; ++k) {
-    const u64 e = base + u64(k) * WAVE + lane;
-    const bool valid = e < m;
-    ks[k] = valid ? s_in[e] : 0u;
-    kd[k] = (HAS_D && valid) ? d_in[e] : 0u;
-    kv[k] = (HAS_V && valid) ? v_in[e] : 0ull;
+    const u64 e = min(base + u64(k) * WAVE + lane, m - 1);
+    ks[k] = s_in[e];
+    kd[k] = HAS_D ? d_in[e] : 0u;
+    kv[k] = HAS_V ? v_in[e] : 0ull;
   }
 #pragma unroll
   for (int k = 0; k < SORT_IPT; ++k) {
-    const u64 e = base + u64(k) * WAVE + lane;
-    const bool valid = e < m;
-    const u64 key = HAS_D ? ((u64(ks[k]) << dbits) | kd[k]) : u64(ks[k]);
+    const bool valid = base + u64(k) * WAVE + lane < m;
+    const u64 key = sort_key_of<HAS_D>(ks[k], kd[k], dbits);
     const u32 digit = valid ? static_cast<u32>((key >> shift) & (RADIX - 1)) : u32(RADIX);
     u64 peers = __ballot(valid);
 #pragma unroll
@@ -917,8 +1008,8 @@ __global__ __launch_bounds__(BLOCK) void radix_scatter_kernel(
   }
   __syncthreads();
   {
-    // thread t owns digit t: tile-local start of the digit and per-wave offsets
-    const int dg = threadIdx.x;
+    // thread t owns digit t: tile-local start, per-wave offsets, global base
+    const u32 dg = threadIdx.x;
     u32 tot = 0;
 #pragma unroll
     for (int ww = 0; ww < WAVES; ++ww) tot += wcnt[ww][dg];
@@ -931,88 +1022,158 @@ __global__ __launch_bounds__(BLOCK) void radix_scatter_kernel(
       wcnt[ww][dg] = run;
       run += c;
     }
-    delta[dg] = hist_scan[u64(dg) * tiles + blockIdx.x] - start;
+    bool ok;
+    const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
+    delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
+    if (!ok) s_ok = 0;
   }
   __syncthreads();
+  if (!s_ok) return;
+  const u32 count = static_cast<u32>((m - tile_base) < SORT_TILE ? (m - tile_base) : SORT_TILE);
+  u32* st_s = reinterpret_cast<u32*>(stage);
+  u32* st_d = st_s + SORT_TILE;
 #pragma unroll
   for (int k = 0; k < SORT_IPT; ++k) {
     if (dig[k] < RADIX) {
       const u32 lp = wcnt[w][dig[k]] + rank[k];
-      lds_s[lp] = ks[k];
-      if (HAS_D) lds_d[lp] = kd[k];
-      if (HAS_V) lds_v[lp] = kv[k];
+      st_s[lp] = ks[k];
+      if (HAS_D) st_d[lp] = kd[k];
     }
   }
   __syncthreads();
-  const u32 count = static_cast<u32>((m - tile_base) < SORT_TILE ? (m - tile_base) : SORT_TILE);
-  for (u32 j = threadIdx.x; j < count; j += BLOCK) {
-    const u32 sj = lds_s[j];
-    const u32 dj = HAS_D ? lds_d[j] : 0u;
-    const u64 key = HAS_D ? ((u64(sj) << dbits) | dj) : u64(sj);
-    const u32 digit = static_cast<u32>((key >> shift) & (RADIX - 1));
-    const u32 pos = delta[digit] + j;
-    s_out[pos] = sj;
-    if (HAS_D) d_out[pos] = dj;
-    if (HAS_V) {
-      u64 vj = lds_v[j];
-      if (DEDUP) {
-        const bool head = (j == 0) || lds_s[j - 1] != sj || lds_d[j - 1] != dj;
-        if (head) {
-          for (u32 t = j + 1; t < count && lds_s[t] == sj && lds_d[t] == dj; ++t) {
-            const u64 x = lds_v[t];
-            vj = x < vj ? x : vj;
-          }
-        } else {
-          vj = VAL_NONE;
-        }
-      }
-      v_out[pos] = vj;
+  // Position j of the sorted tile is handled by lane j % 64 of wave
+  // (j / 64) % WAVES, so each wave owns aligned groups of 64 positions.
+  u32 pos[SORT_IPT];
+  u64 heads[SORT_IPT];  // with DEDUP: lanes that start a run of equal (src, dst)
+#pragma unroll
+  for (int r = 0; r < SORT_IPT; ++r) {
+    const u32 j = r * BLOCK + threadIdx.x;
+    pos[r] = 0;
+    bool head = true;
+    if (j < count) {
+      const u32 sj = st_s[j];
+      const u32 dj = HAS_D ? st_d[j] : 0u;
+      const u32 digit = static_cast<u32>((sort_key_of<HAS_D>(sj, dj, dbits) >> shift) & (RADIX - 1));
+      pos[r] = delta[digit] + j;
+      s_out[pos[r]] = sj;
+      if (HAS_D) d_out[pos[r]] = dj;
+      if (DEDUP) head = (lane == 0) || st_s[j - 1] != sj || st_d[j - 1] != dj;
     }
+    heads[r] = DEDUP ? __ballot(head) : 0ull;
+  }
+  if (!HAS_V) return;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SORT_IPT; ++k)
+    if (dig[k] < RADIX) stage[wcnt[w][dig[k]] + rank[k]] = kv[k];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < SORT_IPT; ++r) {
+    const u32 j = r * BLOCK + threadIdx.x;
+    u64 vj = (j < count) ? stage[j] : VAL_NONE;
+    if (DEDUP) {
+      // the head of each run keeps the run's lightest key, the others are
+      // marked dead.  Runs are cut at the 64-position groups; a longer run
+      // keeps one survivor per group, which later rounds remove.  Segmented
+      // suffix minimum over the group: after the step with offset o, lane l
+      // holds the minimum over [l, min(l + 2o, end of l's run)).
+      const u64 above = (lane == WAVE - 1) ? 0ull : (heads[r] & (~0ull << (lane + 1)));
+      const u32 end = above ? static_cast<u32>(__ffsll(static_cast<long long>(above)) - 1) : WAVE;
+      u64 x = vj;
+#pragma unroll
+      for (int off = 1; off < WAVE; off <<= 1) {
+        const u64 y = __shfl_down(static_cast<unsigned long long>(x), off, WAVE);
+        if (u32(lane) + off < end) x = y < x ? y : x;
+      }
+      vj = ((heads[r] >> lane) & 1ull) ? x : VAL_NONE;
+    }
+    if (j < count) v_out[pos[r]] = vj;
   }
 }
 
-u64 sort_hist_elems(u64 m) { return u64(RADIX) * ceil_div_u32(m ? m : 1, SORT_TILE); }
+u64 sort_work_words(u64 m) {
+  const u64 tiles = ceil_div_u32(m ? m : 1, SORT_TILE);
+  return 2 * (tiles * RADIX + 1) + 2 * u64(MAX_PASSES) * RADIX;
+}
+
+void SortWork::bind(hipStream_t st, u32* p, u64 m) {
+  const u64 tiles = ceil_div_u32(m ? m : 1, SORT_TILE);
+  if (p != words || tiles > tiles_cap) {
+    // fresh memory: zero the status words and the tile counter once
+    words = p;
+    tiles_cap = tiles;
+    fill_u64(st, reinterpret_cast<u64*>(p), tiles * RADIX + 1, 0);
+    epoch = 0;
+    tiles_issued = 0;
+  }
+}
+
+template <bool HAS_D, bool HAS_V, bool DEDUP>
+static void launch_onesweep(hipStream_t st, u32 tiles, const u32* si, const u32* di,
+                            const u64* vi, u32* so, u32* dout, u64* vo, u64 m, u32 dbits, u32 shift,
+                            const u32* base, SortWork& w, u32* err) {
+  u64* status = reinterpret_cast<u64*>(w.words);
+  auto* counter = reinterpret_cast<unsigned long long*>(status + w.tiles_cap * RADIX);
+  radix_onesweep_kernel<HAS_D, HAS_V, DEDUP><<<tiles, BLOCK, 0, st>>>(
+      si, di, vi, so, dout, vo, m, dbits, shift, base, status, w.epoch, counter, w.tiles_issued,
+      err);
+}
 
 bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, u64* v_alt, u64 m,
-                u32 dbits, u32 total_bits, u32* hist, u32* scan_scratch, bool dedup_last) {
+                u32 dbits, u32 total_bits, SortWork& work, u32* err, bool dedup_last) {
   if (m <= 1 || total_bits == 0) return false;
   const u32 tiles = ceil_div_u32(m, SORT_TILE);
+  if (tiles > work.tiles_cap) throw Error(KMST_EINVAL, "sort workspace too small");
   const u32 passes = (total_bits + RADIX_BITS - 1) / RADIX_BITS;
+  if (passes > MAX_PASSES) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
+  u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);
+  u32* base = ghist + MAX_PASSES * RADIX;
+  fill_u32(st, ghist, u64(passes) * RADIX, 0);
+  {
+    KTScope t(st, K_SORT_HIST, (d ? 8.0 : 4.0) * m);
+    // few, fat blocks: every block adds its histograms to the same global bins
+    const u32 grid = grid_for((m + UP_RPT - 1) / UP_RPT, 512);
+    if (d) radix_upsweep_kernel<true><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, passes, ghist);
+    else radix_upsweep_kernel<false><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, passes, ghist);
+    KMST_HIP(hipGetLastError());
+    radix_digit_base_kernel<<<1, BLOCK, 0, st>>>(ghist, passes, base);
+    KMST_HIP(hipGetLastError());
+  }
+  const bool can_dedup = dedup_last && d && v;
   bool alt = false;
   for (u32 p = 0; p < passes; ++p) {
-    const u32 shift = p * RADIX_BITS;
     u32* si = alt ? s_alt : s;
     u32* di = alt ? d_alt : d;
     u64* vi = alt ? v_alt : v;
     u32* so = alt ? s : s_alt;
     u32* dout = alt ? d : d_alt;
     u64* vo = alt ? v : v_alt;
-    {
-      KTScope t(st, K_SORT_HIST, (d ? 8.0 : 4.0) * m);
-      if (d) radix_hist_kernel<true><<<tiles, BLOCK, 0, st>>>(si, di, m, dbits, shift, hist, tiles);
-      else radix_hist_kernel<false><<<tiles, BLOCK, 0, st>>>(si, di, m, dbits, shift, hist, tiles);
-      KMST_HIP(hipGetLastError());
+    if (++work.epoch == (1u << SORT_EPOCH_BITS)) {
+      // epochs wrapped: clear the status words (the counter keeps growing)
+      fill_u64(st, reinterpret_cast<u64*>(work.words), work.tiles_cap * RADIX, 0);
+      work.epoch = 1;
     }
-    scan_exclusive_u32(st, hist, hist, u64(RADIX) * tiles, scan_scratch, nullptr);
     KTScope t(st, K_SORT_SCATTER, 2.0 * m * (4.0 + (d ? 4.0 : 0.0) + (v ? 8.0 : 0.0)));
-    const bool dd = dedup_last && (p + 1 == passes) && d && v;
+    const u32 shift = p * RADIX_BITS;
+    const u32* b = base + p * RADIX;
+    const bool dd = can_dedup && p + 1 == passes;
     if (dd)
-      radix_scatter_kernel<true, true, true><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
-                                                                      dbits, shift, hist, tiles);
+      launch_onesweep<true, true, true>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
+                                        work, err);
     else if (d && v)
-      radix_scatter_kernel<true, true, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
-                                                                       dbits, shift, hist, tiles);
+      launch_onesweep<true, true, false>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
+                                         work, err);
     else if (d)
-      radix_scatter_kernel<true, false, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
-                                                                        dbits, shift, hist, tiles);
+      launch_onesweep<true, false, false>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
+                                          work, err);
     else if (v)
-      radix_scatter_kernel<false, true, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo, m,
-                                                                        dbits, shift, hist, tiles);
+      launch_onesweep<false, true, false>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
+                                          work, err);
     else
-      radix_scatter_kernel<false, false, false><<<tiles, BLOCK, 0, st>>>(si, di, vi, so, dout, vo,
-                                                                         m, dbits, shift, hist,
-                                                                         tiles);
+      launch_onesweep<false, false, false>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
+                                           work, err);
     KMST_HIP(hipGetLastError());
+    work.tiles_issued += tiles;
     alt = !alt;
   }
   return alt;
@@ -1307,7 +1468,7 @@ void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32*
 // (global_num_vertices, distributed_parent_array.hpp:62-72, used at
 // hybrid_boruvka_edgefilter.hpp:289-296).
 __global__ __launch_bounds__(BLOCK) void count_roots_kernel(const u32* __restrict__ gpar,
-                                                            const u32* __restrict__ has_edge, u64 n,
+                                                            const u32* __restrict__ seg_end, u64 n,
                                                             u32* count) {
   __shared__ u32 s_c[WAVES];
   u32 c = 0;
@@ -1317,13 +1478,13 @@ __global__ __launch_bounds__(BLOCK) void count_roots_kernel(const u32* __restric
   const u64 stride = u64(gridDim.x) * BLOCK;
   for (u64 q = u64(blockIdx.x) * BLOCK + threadIdx.x; q < n4; q += stride) {
     const uint4 g = reinterpret_cast<const uint4*>(gpar)[q];
-    const uint4 h = reinterpret_cast<const uint4*>(has_edge)[q];
+    const uint4 h = reinterpret_cast<const uint4*>(seg_end)[q];
     const u32 v = static_cast<u32>(q * 4);
     c += (g.x == v && h.x) + (g.y == v + 1 && h.y) + (g.z == v + 2 && h.z) + (g.w == v + 3 && h.w);
   }
   if (blockIdx.x == 0 && threadIdx.x < n - n4 * 4) {
     const u64 v = n4 * 4 + threadIdx.x;
-    c += (gpar[v] == v && has_edge[v]) ? 1u : 0u;
+    c += (gpar[v] == v && seg_end[v]) ? 1u : 0u;
   }
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
@@ -1336,36 +1497,26 @@ __global__ __launch_bounds__(BLOCK) void count_roots_kernel(const u32* __restric
   }
 }
 
-void count_roots(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32* count_dev) {
+void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32* count_dev) {
   fill_u32(st, count_dev, 1, 0);
   if (n == 0) return;
   KTScope t(st, K_OTHER, 8.0 * n);
-  count_roots_kernel<<<grid_for((n + 3) / 4, 2048), BLOCK, 0, st>>>(gpar, has_edge, n, count_dev);
+  count_roots_kernel<<<grid_for((n + 3) / 4, 2048), BLOCK, 0, st>>>(gpar, seg_end, n, count_dev);
   KMST_HIP(hipGetLastError());
 }
 
-// First index i in [0, m) with a[i] >= x (m if none); a sorted ascending.
-__device__ __forceinline__ u64 lower_bound_u32(const u32* __restrict__ a, u64 m, u32 x) {
-  u64 lo = 0, hi = m;
-  while (lo < hi) {
-    const u64 mid = (lo + hi) >> 1;
-    if (a[mid] < x) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
 // Vertices with records whose parent-array root is not `root`, with their
-// record ranges in the (src, dst)-sorted input, in vertex order.  Each block
+// record ranges [seg_start, seg_end) in the (src, dst)-sorted input (seg_end
+// is 0 for vertices without records), in vertex order.  Each block
 // owns OUT_VPT * BLOCK consecutive vertices and takes one slot range with one
 // global atomic.  counters[0] = list length, counters[2..3] = u64 total range
 // length.
 constexpr int OUT_VPT = 16;
 
 __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
-    const u32* __restrict__ gpar, const u32* __restrict__ has_edge, u64 n, u32 root,
-    const u32* __restrict__ src, u64 m, u32* __restrict__ list, u32* __restrict__ seg_lo,
-    u32* __restrict__ seg_hi, u32* counters) {
+    const u32* __restrict__ gpar, const u32* __restrict__ seg_start,
+    const u32* __restrict__ seg_end, u64 n, u32 root, u32* __restrict__ list,
+    u32* __restrict__ seg_lo, u32* __restrict__ seg_hi, u32* counters) {
   __shared__ u32 s_scan[WAVES];
   __shared__ u32 s_base;
   __shared__ u64 s_len[WAVES];
@@ -1375,7 +1526,7 @@ __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
   for (int k = 0; k < OUT_VPT; ++k) {
     const u64 v = min(v0 + k, n - 1);
     g[k] = gpar[v];
-    h[k] = has_edge[v];
+    h[k] = seg_end[v];
   }
   u32 mine = 0;
 #pragma unroll
@@ -1390,12 +1541,11 @@ __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
   for (int k = 0; k < OUT_VPT; ++k) {
     if (v0 + k < n && h[k] && g[k] != root) {
       const u32 v = static_cast<u32>(v0 + k);
-      const u64 lo = lower_bound_u32(src, m, v);
-      const u64 hi = lower_bound_u32(src, m, v + 1);
+      const u32 lo = seg_start[v];
       list[slot] = v;
-      seg_lo[slot] = static_cast<u32>(lo);
-      seg_hi[slot] = static_cast<u32>(hi);
-      len += hi - lo;
+      seg_lo[slot] = lo;
+      seg_hi[slot] = h[k];
+      len += h[k] - lo;
       ++slot;
     }
   }
@@ -1412,13 +1562,13 @@ __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
   }
 }
 
-void outside_list(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32 root,
-                  const u32* src, u64 m, u32* list, u32* seg_lo, u32* seg_hi, u32* counters) {
+void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u32* seg_end,
+                  u64 n, u32 root, u32* list, u32* seg_lo, u32* seg_hi, u32* counters) {
   fill_u32(st, counters, 4, 0);
   if (n == 0) return;
   KTScope t(st, K_OTHER, 8.0 * n);
   const u32 blocks = ceil_div_u32(n, BLOCK * OUT_VPT);
-  outside_list_kernel<<<blocks, BLOCK, 0, st>>>(gpar, has_edge, n, root, src, m, list, seg_lo,
+  outside_list_kernel<<<blocks, BLOCK, 0, st>>>(gpar, seg_start, seg_end, n, root, list, seg_lo,
                                                 seg_hi, counters);
   KMST_HIP(hipGetLastError());
 }

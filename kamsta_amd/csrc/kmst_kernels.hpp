@@ -50,11 +50,14 @@ struct ClassBases {
   u64 base[MAX_CLASSES];
 };
 // With P.K > 0 it also counts the records of every weight class into
-// class_counts[0..K] (zeroed by the caller); with present (n entries, zeroed
-// by the caller) it sets present[v] = 1 for every vertex with a record.
+// class_counts[0..K] (zeroed by the caller); with seg_start / seg_end (n
+// entries each, seg_end zeroed by the caller) it records every vertex's record
+// range [seg_start[v], seg_end[v]) (seg_end stays 0 for vertices without
+// records).
 void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                       u64 n, u32* err, u32* fwd_count, u32* max_w, u64* hsum,
-                      const WeightPivots& P, u64* class_counts, u32* present = nullptr);
+                      const WeightPivots& P, u64* class_counts, u32* seg_start = nullptr,
+                      u32* seg_end = nullptr);
 void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off);
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                 const u32* off, u64* val, u32* err);
@@ -93,26 +96,42 @@ void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits)
 // Heavy records by the ranges of the vertices outside one component: every
 // record whose ends lie in different components has an end outside the
 // component of `root`, and in a symmetric (src, dst)-sorted input that end's
-// range holds it.  outside_list writes the vertices with records whose root is
-// not `root` (list, seg_lo, seg_hi: up to n entries each); counters[0] = count,
-// counters[2..3] = u64 total range length.  outside_gather then emits, for
-// keys in [kmin, kmax], the relabelled cross-component records in both
-// directions (see the kernel), at most 2 * total of them.
-void outside_list(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32 root,
-                  const u32* src, u64 m, u32* list, u32* seg_lo, u32* seg_hi, u32* counters);
+// range [seg_start, seg_end) holds it.  outside_list writes the vertices with
+// records whose root is not `root` (list, seg_lo, seg_hi: up to n entries
+// each); counters[0] = count, counters[2..3] = u64 total range length.
+// outside_gather then emits, for keys in [kmin, kmax], the relabelled
+// cross-component records in both directions (see the kernel), at most
+// 2 * total of them.
+void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u32* seg_end,
+                  u64 n, u32 root, u32* list, u32* seg_lo, u32* seg_hi, u32* counters);
 void outside_gather(hipStream_t st, const u32* list, const u32* seg_lo, const u32* seg_hi,
                     u32 n_list, u64 total, const u32* in_d, const KeySrc& keys, u64 kmin, u64 kmax,
                     const u32* gpar, u32 root, u32* s_out, u32* d_out, u64* v_out, u64 cap,
                     u32* count, u32* err);
 
-// K6 stable LSD radix sort by key (s << dbits) | d over total_bits bits.
-// Returns true when the sorted data ended in the *_alt arrays.  d / v (and
-// their alt) may be null.  hist needs sort_hist_elems(m) entries.
-u64 sort_hist_elems(u64 m);
-// With dedup_last (needs d and v) the last pass also keeps only the lightest
-// record of each run of equal (s, d) inside a tile; the rest are marked dead.
+// K6 stable LSD radix sort by key (s << dbits) | d over total_bits bits; with
+// dbits == 0 the key is s alone and d is carried along.  Returns true when the
+// sorted data ended in the *_alt arrays.  d / v (and their alt) may be null.
+// A bounded look-back that expires raises ERR_SPIN in *err.
+//
+// Workspace: sort_work_words(m) u32 words (8-byte aligned) holding the
+// per-(tile, digit) look-back status words, a tile counter and the digit
+// histograms.  Status words carry a pass epoch, so they are cleared only when
+// the memory is (re)bound; bind() before every sort with the buffer for the
+// largest m it will see.
+u64 sort_work_words(u64 m);
+struct SortWork {
+  u32* words = nullptr;
+  u64 tiles_cap = 0;     // tiles the status area holds
+  u32 epoch = 0;         // epoch of the last pass
+  u64 tiles_issued = 0;  // value of the tile counter
+  void bind(hipStream_t st, u32* p, u64 m);
+};
+// With dedup_last (needs d, v and dbits > 0) the last pass also keeps only
+// the lightest record of each run of equal (s, d) inside an aligned group of
+// 64 positions; the rest are marked dead.
 bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, u64* v_alt,
-                u64 m, u32 dbits, u32 total_bits, u32* hist, u32* scan_scratch,
+                u64 m, u32 dbits, u32 total_bits, SortWork& work, u32* err,
                 bool dedup_last = false);
 
 // K7 keep the lightest record of each (s, d) run (others marked dead).
@@ -134,7 +153,7 @@ void sample_weights(hipStream_t st, const u64* v, const u32* wt, u64 m, u32 k, u
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                        u64 m, u32 pivot, u32 wshift, bool write_heavy, u32* s_out, u32* d_out,
                        u64* v_out, u64* lb, u32* light_count_dev, u32* err);
-void count_roots(hipStream_t st, const u32* gpar, const u32* has_edge, u64 n, u32* count_dev);
+void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32* count_dev);
 
 // Present-vertex compaction helpers (sorted records; flag must be zeroed).
 void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag);

@@ -123,10 +123,12 @@ kmst_context::~kmst_context() {
   (void)hipStreamDestroy(stream);
 }
 std::vector<DevBuf*> kmst_context::all_bufs() {
-  return {&xs, &xd, &xv, &ys, &yd, &yv, &val0, &off, &best_val, &best_dst, &parent, &flag,
-          &newid, &label, &rep, &rep_next, &gpar, &has_edge, &mst, &pieces, &tiles, &scan_scratch, &hist,
-          &counters, &lookback, &in_s, &in_d, &in_w, &out_s, &out_d, &out_w, &samples, &gen_a, &gen_b,
-          &gen_c, &gen_d};
+  return {&xs,       &xd,         &xv,      &ys,        &yd,    &yv,      &val0,
+          &off,      &best_val,   &best_dst, &parent,   &flag,  &newid,   &label,
+          &rep,      &rep_next,   &gpar,    &seg_start, &seg_end, &mst,   &pieces,
+          &tiles,    &scan_scratch, &hist,  &counters,  &lookback, &in_s, &in_d,
+          &in_w,     &out_s,      &out_d,   &out_w,     &samples, &gen_a, &gen_b,
+          &gen_c,    &gen_d};
 }
 
 namespace kmst {
@@ -180,10 +182,13 @@ class Solver {
   u32 *best_dst = nullptr, *parent = nullptr, *label = nullptr;
   u64 *state = nullptr, *pos = nullptr;  // per-round vertex state and its scan
   u32 *flag = nullptr, *newid = nullptr;  // u32 views of the same memory (compaction)
-  u32 *rep = nullptr, *rep_next = nullptr, *gpar = nullptr, *has_edge = nullptr;
+  u32 *rep = nullptr, *rep_next = nullptr, *gpar = nullptr;
+  // input record range of every vertex (filter-Boruvka); seg_end[v] == 0
+  // marks vertices without records
+  u32 *seg_start = nullptr, *seg_end = nullptr;
   u64* mst = nullptr;
   u64 mst_count = 0;
-  u32 *scan_scratch = nullptr, *hist = nullptr;
+  u32* scan_scratch = nullptr;
   u64* lb = nullptr;
   ArgPiece* pieces = nullptr;
   u32* samples = nullptr;
@@ -192,7 +197,7 @@ class Solver {
 
   void allocate();
   // validates the input; for filter-Boruvka also marks vertices with records
-  // (has_edge) and, with plan_classes, counts the weight classes
+  // and their record ranges and, with plan_classes, counts the weight classes
   void setup(bool filter, bool plan_classes);
   void plan_pivots();
   void filter_classes();
@@ -203,7 +208,7 @@ class Solver {
     std::memcpy(&x, C.host_counters + word, 8);
     return x;
   }
-  // rounds of plain Boruvka on records sorted by (src, dst) with ids < n_start
+  // rounds of plain Boruvka on records grouped by src with ids < n_start
   // (default n0); A is writable scratch disjoint from recs, B writable and may
   // alias recs.  With track_gpar every round updates the parent array.
   void rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_start = 0,
@@ -227,7 +232,10 @@ class Solver {
   u32 pick_pivot(RecView r);
   u64 partition(RecView in, RecView out, u32 pivot, bool write_heavy = true);
   KeySrc keys_of(const RecView& r) const { return KeySrc{r.v, r.w, vbits}; }
+  // records grouped by src are all the argmin needs; where parallel records
+  // abound, sort by (src, dst) and keep only the lightest of each pair
   RecView sort_dedup(RecView cur, RecView alt, u32 bits);
+  RecView sort_by_src(RecView cur, RecView alt, u32 bits);
 };
 
 void Solver::allocate() {
@@ -248,14 +256,15 @@ void Solver::allocate() {
   rep = static_cast<u32*>(C.rep.get(n * 4));
   rep_next = static_cast<u32*>(C.rep_next.get(n * 4));
   gpar = static_cast<u32*>(C.gpar.get(n * 4));
-  has_edge = static_cast<u32*>(C.has_edge.get(n * 4));
+  seg_start = static_cast<u32*>(C.seg_start.get(n * 4));
+  seg_end = static_cast<u32*>(C.seg_end.get(n * 4));
   mst = static_cast<u64*>(C.mst.get(n * 8));
   const u64 arg_tiles = (m + ARG_TILE - 1) / ARG_TILE;
   pieces = static_cast<ArgPiece*>(C.pieces.get(2 * arg_tiles * sizeof(ArgPiece) + 64));
   lb = static_cast<u64*>(C.lookback.get(lookback_elems(m) * 8));
-  const u64 hist_elems = sort_hist_elems(std::max(m, n));
-  hist = static_cast<u32*>(C.hist.get(hist_elems * 4));
-  const u64 scan_max = std::max<u64>({hist_elems, n, (m + CMP_TILE - 1) / CMP_TILE + 1});
+  const u64 sort_m = std::max(m, n);
+  C.sort_work.bind(st, static_cast<u32*>(C.hist.get(sort_work_words(sort_m) * 4)), sort_m);
+  const u64 scan_max = std::max<u64>(n, (m + CMP_TILE - 1) / CMP_TILE + 1);
   // u64 scans reuse the same scratch, so size it in u64 units
   scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 8 + 64));
   ctr = static_cast<u32*>(C.counters.get(256));
@@ -285,10 +294,11 @@ void Solver::setup(bool filter, bool plan_classes) {
   if (plan_classes) plan_pivots();
   // init_parent_array (hybrid_boruvka_edgefilter.hpp:332-352): vertices with
   // records, marked during validation
-  if (filter) fill_u32(st, has_edge, n0, 0);
+  if (filter) fill_u32(st, seg_end, n0, 0);
   validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW,
                    reinterpret_cast<u64*>(ctr + C_HSUM64), pivots,
-                   reinterpret_cast<u64*>(ctr + C_CLASS64), filter ? has_edge : nullptr);
+                   reinterpret_cast<u64*>(ctr + C_CLASS64), filter ? seg_start : nullptr,
+                   filter ? seg_end : nullptr);
   check_errors("input");
   if (u64(C.host_counters[C_FWD]) * 2 != m0 || counter64(C_HSUM64) != counter64(C_HSUM64 + 2))
     throw Error(KMST_EASYMMETRIC, "input: records are not symmetric (src,dst,w) <-> (dst,src,w)");
@@ -305,11 +315,19 @@ void Solver::setup(bool filter, bool plan_classes) {
   }
 }
 
+RecView Solver::sort_by_src(RecView cur, RecView alt, u32 bits) {
+  if (cur.m > 1 && bits > 0 &&
+      radix_sort(st, cur.s, cur.d, cur.v, alt.s, alt.d, alt.v, cur.m, 0, bits, C.sort_work,
+                 ctr + C_ERR))
+    cur = RecView{alt.s, alt.d, alt.v, cur.m};
+  return cur;
+}
+
 RecView Solver::sort_dedup(RecView cur, RecView alt, u32 bits) {
   if (cur.m > 1 && bits > 0) {
     // the last scatter pass keeps the lightest copy of each (src, dst) run
-    if (radix_sort(st, cur.s, cur.d, cur.v, alt.s, alt.d, alt.v, cur.m, bits, 2 * bits, hist,
-                   scan_scratch, /*dedup_last=*/true))
+    if (radix_sort(st, cur.s, cur.d, cur.v, alt.s, alt.d, alt.v, cur.m, bits, 2 * bits,
+                   C.sort_work, ctr + C_ERR, /*dedup_last=*/true))
       cur = RecView{alt.s, alt.d, alt.v, cur.m};
   } else {
     dedup_mark(st, cur.s, cur.d, cur.v, cur.m);
@@ -355,7 +373,8 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     cur = RecView{out.s, out.d, out.v, m};
     if (m == 0 || n <= 1) break;
     RecView alt = (cur.s == A.s) ? B : A;
-    cur = sort_dedup(cur, alt, bits_for(n));
+    // dense rounds (more records than n^2 / 8) carry many parallel records
+    cur = (m * 8 > n * n) ? sort_dedup(cur, alt, bits_for(n)) : sort_by_src(cur, alt, bits_for(n));
     first = false;
   }
   if (track_gpar) pointer_jump(st, n0, gpar, ctr + C_ERR);
@@ -363,9 +382,9 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
 
 void Solver::base_case(RecView P, RecView Q) {
   // a base case with few records touches few vertices: renumber them to
-  // [0, n_c) (monotone, so the records stay sorted) instead of running the
+  // [0, n_c) (monotone, so the records stay grouped) instead of running the
   // first round over all n0 vertex slots
-  if (P.m * 2 < u64(n0)) {
+  if (P.m < u64(n0)) {
     fill_u32(st, flag, n0, 0);
     mark_present(st, P.s, P.m, flag);
     scan_exclusive_u32(st, flag, newid, n0, scan_scratch, ctr + C_NNEXT);
@@ -495,7 +514,7 @@ u64 Solver::partition(RecView in, RecView out, u32 pivot, bool write_heavy) {
 // same-component edges, sort + keep the lightest copies, recurse
 // (boruvka_recursive_setup, hybrid_boruvka_edgefilter.hpp:173-225).
 bool Solver::one_component() {
-  count_roots(st, gpar, has_edge, n0, ctr + C_NNEXT);
+  count_roots(st, gpar, seg_end, n0, ctr + C_NNEXT);
   const u32 k = std::min<u32>(GPAR_SAMPLES, n0);
   sample_weights(st, nullptr, gpar, n0, k, 32, samples);
   KMST_HIP(hipMemcpyAsync(C.host_counters, ctr, 256, hipMemcpyDeviceToHost, st));
@@ -531,7 +550,7 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
   if (from_input && giant_share >= 0.25) {
     // few vertices outside the largest component: scan only their ranges
     // (the per-vertex arrays of the rounds are free here)
-    outside_list(st, gpar, has_edge, n0, giant_root, in_s, m0, best_dst, parent, label,
+    outside_list(st, gpar, seg_start, seg_end, n0, giant_root, best_dst, parent, label,
                  ctr + C_OUTSIDE);
     read_counters();
     const u64 n_list = C.host_counters[C_OUTSIDE];
@@ -566,13 +585,13 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
   C.stats.filter.back().heavy_kept = Hk;
   if (Hk == 0) return;
   RecView kept = scratch.at(0, Hk);
-  RecView sorted = sort_dedup(kept, alt.at(0, Hk), vbits);
+  RecView sorted = sort_by_src(kept, alt.at(0, Hk), vbits);
   RecView other = (sorted.s == kept.s) ? alt.at(0, Hk) : kept;
   filter_rec(sorted, other, false, depth + 1);
 }
 
 // boruvka_recursive (include/algorithms/hybrid_boruvka_edgefilter.hpp:227-330)
-// on records P (sorted by (src, dst), ids are parent-array representatives);
+// on records P (grouped by src, ids are parent-array representatives);
 // Q is scratch of the same extent.
 void Solver::filter_rec(RecView P, RecView Q, bool stop, int depth) {
   if (P.m == 0) return;
@@ -626,8 +645,8 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
   if (cnt > out_cap) throw Error(KMST_EINVAL, "output capacity too small");
   mst_triples(st, mst, cnt, vbits, packed, in_s, in_d, in_w, X.s, X.d, X.v);
   RecView res{X.s, X.d, X.v, cnt};
-  if (cnt > 1 && radix_sort(st, X.s, X.d, X.v, Y.s, Y.d, Y.v, cnt, vbits, 2 * vbits, hist,
-                            scan_scratch))
+  if (cnt > 1 && radix_sort(st, X.s, X.d, X.v, Y.s, Y.d, Y.v, cnt, vbits, 2 * vbits,
+                            C.sort_work, ctr + C_ERR))
     res = RecView{Y.s, Y.d, Y.v, cnt};
   u64* total_dev = reinterpret_cast<u64*>(ctr + C_TOTAL64);
   fill_u64(st, total_dev, 1, 0);
@@ -793,6 +812,7 @@ void kmst_context_trim(kmst_context* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (kmst::DevBuf* b : ctx->all_bufs())
     if (b != &ctx->counters) b->release();
+  ctx->sort_work = kmst::SortWork{};  // its status words are gone with `hist`
 }
 
 kmst_status kmst_boruvka(const kmst_wedge* in, uint64_t m, const kmst_config* cfg, kmst_wedge* out,
@@ -873,15 +893,20 @@ kmst_status kmst_k_sort_pairs(kmst_context* ctx, uint32_t* src, uint32_t* dst, u
     auto* as = static_cast<uint32_t*>(ctx->xs.get(mm * 4));
     auto* ad = static_cast<uint32_t*>(ctx->xd.get(mm * 4));
     auto* av = static_cast<uint64_t*>(ctx->xv.get(mm * 8));
-    const uint64_t he = kmst::sort_hist_elems(mm);
-    auto* hist = static_cast<uint32_t*>(ctx->hist.get(he * 4));
-    auto* sc = static_cast<uint32_t*>(ctx->scan_scratch.get(kmst::scan_scratch_elems(he) * 4 + 64));
-    if (kmst::radix_sort(ctx->stream, src, dst, val, as, ad, av, m, bits, 2 * bits, hist, sc)) {
+    ctx->sort_work.bind(ctx->stream,
+                        static_cast<uint32_t*>(ctx->hist.get(kmst::sort_work_words(mm) * 4)), mm);
+    auto* ctr = static_cast<uint32_t*>(ctx->counters.get(256));
+    kmst::fill_u32(ctx->stream, ctr, 64, 0);
+    if (kmst::radix_sort(ctx->stream, src, dst, val, as, ad, av, m, bits, 2 * bits,
+                         ctx->sort_work, ctr + kmst::C_ERR)) {
       KMST_HIP(hipMemcpyAsync(src, as, m * 4, hipMemcpyDeviceToDevice, ctx->stream));
       KMST_HIP(hipMemcpyAsync(dst, ad, m * 4, hipMemcpyDeviceToDevice, ctx->stream));
       KMST_HIP(hipMemcpyAsync(val, av, m * 8, hipMemcpyDeviceToDevice, ctx->stream));
     }
+    uint32_t err = 0;
+    KMST_HIP(hipMemcpyAsync(&err, ctr + kmst::C_ERR, 4, hipMemcpyDeviceToHost, ctx->stream));
     KMST_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) throw kmst::Error(KMST_EDEVICE, "sort: bounded wait expired");
   });
 }
 

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "kmst_common.hpp"
+#include "kmst_kernels.hpp"
 
 namespace kmst {
 
@@ -63,9 +64,11 @@ struct kmst_context {
   // record buffers (X, Y ping-pong), strict keys of the input
   kmst::DevBuf xs, xd, xv, ys, yd, yv, val0;
   // vertex arrays
-  kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, has_edge, mst;
+  kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, seg_start, seg_end,
+      mst;
   // kernel scratch
   kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback;
+  kmst::SortWork sort_work;  // radix sort state over `hist`
   // host-API staging
   kmst::DevBuf in_s, in_d, in_w, out_s, out_d, out_w;
   // generator scratch
