@@ -427,6 +427,7 @@ __device__ __forceinline__ void seg_combine(Piece& cur, const Piece& prev) {
   }
 }
 
+template <bool VEC>
 __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ src,
                                                        const u32* __restrict__ dst,
                                                        const KeySrc keys, u64 m,
@@ -457,16 +458,59 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
   bool has_items = my_base < tile_end;
   u32 head_src = NONE32, cur_src = NONE32;
   u64 head_val = VAL_NONE, cur_val = VAL_NONE;
-  u32 head_idx = 0, cur_idx = 0;
+  u32 head_idx = 0, cur_idx = 0, cur_dst = 0;
   bool head_done = false;  // first run ended inside this thread
+  // the thread's records, loaded before use (16-byte loads when aligned)
+  u32 rs[ARG_IPT], rd[ARG_IPT];
+  u64 rv[ARG_IPT];
+  if (VEC && my_base + ARG_IPT <= m) {
+    const uint4* s4 = reinterpret_cast<const uint4*>(src + my_base);
+    const uint4* d4 = reinterpret_cast<const uint4*>(dst + my_base);
+#pragma unroll
+    for (int q = 0; q < ARG_IPT / 4; ++q) {
+      const uint4 a = s4[q], b = d4[q];
+      rs[4 * q] = a.x; rs[4 * q + 1] = a.y; rs[4 * q + 2] = a.z; rs[4 * q + 3] = a.w;
+      rd[4 * q] = b.x; rd[4 * q + 1] = b.y; rd[4 * q + 2] = b.z; rd[4 * q + 3] = b.w;
+    }
+    if (keys.v) {
+      const uint4* v4 = reinterpret_cast<const uint4*>(keys.v + my_base);
+#pragma unroll
+      for (int q = 0; q < ARG_IPT / 2; ++q) {
+        const uint4 a = v4[q];
+        rv[2 * q] = (u64(a.y) << 32) | a.x;
+        rv[2 * q + 1] = (u64(a.w) << 32) | a.z;
+      }
+    } else {
+      const uint4* w4 = reinterpret_cast<const uint4*>(keys.w + my_base);
+#pragma unroll
+      for (int q = 0; q < ARG_IPT / 4; ++q) {
+        const uint4 a = w4[q];
+        rv[4 * q] = pack_key(a.x, rs[4 * q], rd[4 * q], keys.vbits);
+        rv[4 * q + 1] = pack_key(a.y, rs[4 * q + 1], rd[4 * q + 1], keys.vbits);
+        rv[4 * q + 2] = pack_key(a.z, rs[4 * q + 2], rd[4 * q + 2], keys.vbits);
+        rv[4 * q + 3] = pack_key(a.w, rs[4 * q + 3], rd[4 * q + 3], keys.vbits);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < ARG_IPT; ++k) {
+      const u64 i = min(my_base + k, m - 1);
+      rs[k] = src[i];
+      rd[k] = dst[i];
+      rv[k] = load_key(keys, i, rs[k], rd[k]);
+    }
+  }
   if (has_items) {
-    const u64 my_end = (my_base + ARG_IPT < tile_end) ? my_base + ARG_IPT : tile_end;
-    cur_src = src[my_base];
-    cur_val = load_key(keys, my_base, cur_src, keys.v ? 0u : dst[my_base]);
+    const u32 cnt = static_cast<u32>(min<u64>(ARG_IPT, tile_end - my_base));
+    cur_src = rs[0];
+    cur_val = rv[0];
     cur_idx = static_cast<u32>(my_base);
-    for (u64 i = my_base + 1; i < my_end; ++i) {
-      const u32 s = src[i];
-      const u64 v = load_key(keys, i, s, keys.v ? 0u : dst[i]);
+    cur_dst = rd[0];
+#pragma unroll
+    for (u32 k = 1; k < ARG_IPT; ++k) {
+      if (k >= cnt) break;
+      const u32 s = rs[k];
+      const u64 v = rv[k];
       if (s != cur_src) {
         if (!head_done) {
           head_src = cur_src;
@@ -475,14 +519,16 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
           head_done = true;
         } else {  // interior run: complete inside this thread
           best_val[cur_src] = cur_val;
-          best_dst[cur_src] = dst[cur_idx];
+          best_dst[cur_src] = cur_dst;
         }
         cur_src = s;
         cur_val = v;
-        cur_idx = static_cast<u32>(i);
+        cur_idx = static_cast<u32>(my_base + k);
+        cur_dst = rd[k];
       } else if (v < cur_val) {
         cur_val = v;
-        cur_idx = static_cast<u32>(i);
+        cur_idx = static_cast<u32>(my_base + k);
+        cur_dst = rd[k];
       }
     }
   }
@@ -579,8 +625,16 @@ void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const KeyS
   const u32 tiles = ceil_div_u32(m, ARG_TILE);
   {
     KTScope t(st, K_ARGMIN, 12.0 * m + 16.0 * n);
-    argmin_kernel<<<tiles, BLOCK, 0, st>>>(src, dst, keys, m, best_val, best_dst, pieces,
-                                           piece_count);
+    const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst) |
+                       reinterpret_cast<uintptr_t>(keys.v ? static_cast<const void*>(keys.v)
+                                                          : static_cast<const void*>(keys.w))) &
+                      15) == 0;
+    if (vec)
+      argmin_kernel<true><<<tiles, BLOCK, 0, st>>>(src, dst, keys, m, best_val, best_dst, pieces,
+                                                   piece_count);
+    else
+      argmin_kernel<false><<<tiles, BLOCK, 0, st>>>(src, dst, keys, m, best_val, best_dst, pieces,
+                                                    piece_count);
     KMST_HIP(hipGetLastError());
   }
   KTScope t(st, K_ARGMIN_FIXUP, 32.0 * tiles);
