@@ -54,10 +54,14 @@ __global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(const T* __restrict_
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
+// With flagged_low, element i also gets flagged_low[i] = low 32 bits of its
+// exclusive sum when bit 0 of in[i] is set, NONE32 otherwise (the new id of
+// a root in the hook state scan, one 4-byte gather later instead of two
+// 8-byte ones).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void scan_down_kernel(const T* in, T* out, u64 n,
                                                           const T* __restrict__ partial,
-                                                          T* total_dev) {
+                                                          T* total_dev, u32* flagged_low) {
   __shared__ T lds[WAVES];
   const u64 base = u64(blockIdx.x) * SCAN_TILE + u64(threadIdx.x) * SCAN_IPT;
   T v[SCAN_IPT];
@@ -75,7 +79,10 @@ __global__ __launch_bounds__(BLOCK) void scan_down_kernel(const T* in, T* out, u
 #pragma unroll
   for (int k = 0; k < SCAN_IPT; ++k) {
     const u64 i = base + k;
-    if (i < n) out[i] = run;
+    if (i < n) {
+      out[i] = run;
+      if (flagged_low) flagged_low[i] = (v[k] & 1) ? static_cast<u32>(run) : NONE32;
+    }
     run += v[k];
   }
   if (total_dev && partial == nullptr && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1)
@@ -92,7 +99,8 @@ u64 scan_scratch_elems(u64 n) {
 }
 
 template <typename T>
-void scan_exclusive(hipStream_t st, const T* in, T* out, u64 n, T* scratch, T* total_dev) {
+void scan_exclusive(hipStream_t st, const T* in, T* out, u64 n, T* scratch, T* total_dev,
+                    u32* flagged_low = nullptr) {
   if (n == 0) {
     if (total_dev) {
       if (sizeof(T) == 8) fill_u64(st, reinterpret_cast<u64*>(total_dev), 1, 0);
@@ -103,7 +111,7 @@ void scan_exclusive(hipStream_t st, const T* in, T* out, u64 n, T* scratch, T* t
   const u32 tiles = ceil_div_u32(n, SCAN_TILE);
   if (tiles == 1) {
     KTScope t(st, K_SCAN_DOWN, 2.0 * sizeof(T) * n);
-    scan_down_kernel<T><<<1, BLOCK, 0, st>>>(in, out, n, nullptr, total_dev);
+    scan_down_kernel<T><<<1, BLOCK, 0, st>>>(in, out, n, nullptr, total_dev, flagged_low);
     KMST_HIP(hipGetLastError());
     return;
   }
@@ -115,7 +123,7 @@ void scan_exclusive(hipStream_t st, const T* in, T* out, u64 n, T* scratch, T* t
   }
   scan_exclusive<T>(st, partial, partial, tiles, scratch + tiles, total_dev);
   KTScope t(st, K_SCAN_DOWN, 2.0 * sizeof(T) * n);
-  scan_down_kernel<T><<<tiles, BLOCK, 0, st>>>(in, out, n, partial, nullptr);
+  scan_down_kernel<T><<<tiles, BLOCK, 0, st>>>(in, out, n, partial, nullptr, flagged_low);
   KMST_HIP(hipGetLastError());
 }
 
@@ -124,8 +132,8 @@ void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scr
   scan_exclusive<u32>(st, in, out, n, scratch, total_dev);
 }
 void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scratch,
-                        u64* total_dev) {
-  scan_exclusive<u64>(st, in, out, n, scratch, total_dev);
+                        u64* total_dev, u32* flagged_low) {
+  scan_exclusive<u64>(st, in, out, n, scratch, total_dev, flagged_low);
 }
 
 // ============================================================== setup
@@ -718,6 +726,7 @@ void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err) {
 // label[v] = new consecutive id of v's root (NONE32 for finished vertices);
 // emitters write their edge key into the MST list at mst_base + their slot.
 __global__ __launch_bounds__(BLOCK) void label_kernel(u64 n, const u32* __restrict__ parent,
+                                                      const u32* __restrict__ root_id,
                                                       const u64* __restrict__ state,
                                                       const u64* __restrict__ pos,
                                                       const u64* __restrict__ best_val,
@@ -725,17 +734,16 @@ __global__ __launch_bounds__(BLOCK) void label_kernel(u64 n, const u32* __restri
                                                       u64* __restrict__ mst, u64 mst_base) {
   const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
   if (v >= n) return;
-  const u32 r = parent[v];
-  label[v] = (state[r] & ST_ROOT) ? static_cast<u32>(pos[r]) : NONE32;
+  label[v] = root_id[parent[v]];
   if (state[v] & ST_EMIT) mst[mst_base + (pos[v] >> 32)] = best_val[v];
 }
 
-void make_labels(hipStream_t st, u64 n, const u32* parent, const u64* state, const u64* pos,
-                 const u64* best_val, u32* label, u64* mst, u64 mst_base) {
+void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_id, const u64* state,
+                 const u64* pos, const u64* best_val, u32* label, u64* mst, u64 mst_base) {
   if (n == 0) return;
   KTScope t(st, K_LABEL, 28.0 * n);
-  label_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, parent, state, pos, best_val, label,
-                                                          mst, mst_base);
+  label_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, parent, root_id, state, pos,
+                                                          best_val, label, mst, mst_base);
   KMST_HIP(hipGetLastError());
 }
 
@@ -1233,6 +1241,153 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
   return alt;
 }
 
+// The same sort for a single u64 array ordered by its low bits (the MST keys
+// by (min, max) at the end of a solve): 8-byte records instead of 16.
+__global__ __launch_bounds__(BLOCK) void radix_upsweep_u64_kernel(const u64* __restrict__ v,
+                                                                  u64 m, u64 mask, u32 passes,
+                                                                  u32* __restrict__ ghist) {
+  __shared__ u32 h[WAVES][MAX_PASSES][RADIX];
+  for (int i = threadIdx.x; i < WAVES * MAX_PASSES * RADIX; i += BLOCK) (&h[0][0][0])[i] = 0;
+  __syncthreads();
+  const int w = wave_id();
+  const u64 stride = u64(gridDim.x) * (BLOCK * UP_RPT);
+  for (u64 base = u64(blockIdx.x) * (BLOCK * UP_RPT); base < m; base += stride) {
+    u64 key[UP_RPT];
+#pragma unroll
+    for (int k = 0; k < UP_RPT; ++k)
+      key[k] = v[min(base + u64(k) * BLOCK + threadIdx.x, m - 1)] & mask;
+#pragma unroll
+    for (int k = 0; k < UP_RPT; ++k) {
+      if (base + u64(k) * BLOCK + threadIdx.x >= m) continue;
+      for (u32 p = 0; p < passes; ++p)
+        atomicAdd(&h[w][p][static_cast<u32>((key[k] >> (p * RADIX_BITS)) & (RADIX - 1))], 1u);
+    }
+  }
+  __syncthreads();
+  for (u32 i = threadIdx.x; i < passes * RADIX; i += BLOCK) {
+    u32 c = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) c += (&h[ww][0][0])[i];
+    if (c) atomicAdd(ghist + i, c);
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void radix_onesweep_u64_kernel(
+    const u64* __restrict__ v_in, u64* __restrict__ v_out, u64 m, u64 mask, u32 shift,
+    const u32* __restrict__ digit_base, u64* status, u32 epoch, unsigned long long* tile_counter,
+    unsigned long long tile_base_id, u32* err) {
+  __shared__ u32 wcnt[WAVES][RADIX];
+  __shared__ u32 delta[RADIX];
+  __shared__ u64 stage[SORT_TILE];
+  __shared__ u32 scan_tmp[WAVES];
+  __shared__ u32 s_ok;
+  __shared__ u32 s_tile;
+  for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&wcnt[0][0])[i] = 0;
+  if (threadIdx.x == 0) {
+    s_ok = 1;
+    s_tile = static_cast<u32>(atomicAdd(tile_counter, 1ull) - tile_base_id);
+  }
+  __syncthreads();
+  const u32 tile = s_tile;
+  const int lane = lane_id(), w = wave_id();
+  const u64 tile_base = u64(tile) * SORT_TILE;
+  const u64 base = tile_base + u64(w) * (WAVE * SORT_IPT);
+  u64 kv[SORT_IPT];
+  u32 dig[SORT_IPT], rank[SORT_IPT];
+#pragma unroll
+  for (int k = 0; k < SORT_IPT; ++k) kv[k] = v_in[min(base + u64(k) * WAVE + lane, m - 1)];
+#pragma unroll
+  for (int k = 0; k < SORT_IPT; ++k) {
+    const bool valid = base + u64(k) * WAVE + lane < m;
+    const u32 digit = valid ? static_cast<u32>(((kv[k] & mask) >> shift) & (RADIX - 1)) : u32(RADIX);
+    u64 peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RADIX_BITS; ++b) {
+      const bool bit = (digit >> b) & 1u;
+      const u64 bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    dig[k] = digit;
+    rank[k] = 0;
+    if (valid) {
+      const u32 old = wcnt[w][digit];
+      rank[k] = old + mbcnt(peers);
+      if (lane == __ffsll(static_cast<long long>(peers)) - 1)
+        wcnt[w][digit] = old + static_cast<u32>(__popcll(peers));
+    }
+  }
+  __syncthreads();
+  {
+    const u32 dg = threadIdx.x;
+    u32 tot = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) tot += wcnt[ww][dg];
+    u32 all;
+    const u32 start = block_exclusive_sum(tot, all, scan_tmp);
+    u32 run = start;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) {
+      const u32 c = wcnt[ww][dg];
+      wcnt[ww][dg] = run;
+      run += c;
+    }
+    bool ok;
+    const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
+    delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
+    if (!ok) s_ok = 0;
+  }
+  __syncthreads();
+  if (!s_ok) return;
+#pragma unroll
+  for (int k = 0; k < SORT_IPT; ++k)
+    if (dig[k] < RADIX) stage[wcnt[w][dig[k]] + rank[k]] = kv[k];
+  __syncthreads();
+  const u32 count = static_cast<u32>((m - tile_base) < SORT_TILE ? (m - tile_base) : SORT_TILE);
+  for (u32 j = threadIdx.x; j < count; j += BLOCK) {
+    const u64 x = stage[j];
+    v_out[delta[static_cast<u32>(((x & mask) >> shift) & (RADIX - 1))] + j] = x;
+  }
+}
+
+bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWork& work,
+                    u32* err) {
+  if (m <= 1 || bits == 0) return false;
+  const u32 tiles = ceil_div_u32(m, SORT_TILE);
+  if (tiles > work.tiles_cap) throw Error(KMST_EINVAL, "sort workspace too small");
+  const u32 passes = (bits + RADIX_BITS - 1) / RADIX_BITS;
+  if (passes > MAX_PASSES) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
+  const u64 mask = bits >= 64 ? ~0ull : ((u64{1} << bits) - 1);  // bits above are not key
+  u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);
+  u32* base = ghist + MAX_PASSES * RADIX;
+  fill_u32(st, ghist, u64(passes) * RADIX, 0);
+  {
+    KTScope t(st, K_SORT_HIST, 8.0 * m);
+    radix_upsweep_u64_kernel<<<grid_for((m + UP_RPT - 1) / UP_RPT, 512), BLOCK, 0, st>>>(
+        v, m, mask, passes, ghist);
+    KMST_HIP(hipGetLastError());
+    radix_digit_base_kernel<<<1, BLOCK, 0, st>>>(ghist, passes, base);
+    KMST_HIP(hipGetLastError());
+  }
+  bool alt = false;
+  u64* status = reinterpret_cast<u64*>(work.words);
+  auto* counter = reinterpret_cast<unsigned long long*>(status + work.tiles_cap * RADIX);
+  for (u32 p = 0; p < passes; ++p) {
+    if (++work.epoch == (1u << SORT_EPOCH_BITS)) {
+      fill_u64(st, status, work.tiles_cap * RADIX, 0);
+      work.epoch = 1;
+    }
+    KTScope t(st, K_SORT_SCATTER, 16.0 * m);
+    radix_onesweep_u64_kernel<<<tiles, BLOCK, 0, st>>>(alt ? v_alt : v, alt ? v : v_alt, m, mask,
+                                                       p * RADIX_BITS, base + p * RADIX, status,
+                                                       work.epoch, counter, work.tiles_issued,
+                                                       err);
+    KMST_HIP(hipGetLastError());
+    work.tiles_issued += tiles;
+    alt = !alt;
+  }
+  return alt;
+}
+
 // ============================================================== K7 dedup
 // After the (src, dst) sort, keep the lightest record of each run of equal
 // (src, dst) (EdgeProcessor::remove_duplicates, duplicate_detection.hpp:26-64)
@@ -1269,63 +1424,50 @@ void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m) {
 // ============================================================== K9 output
 // MST keys -> canonical (min, max, w): packed keys decode directly,
 // positional keys gather the twin record (GetMstEdge, misc.hpp:145-206).
-__global__ __launch_bounds__(BLOCK) void mst_triples_kernel(const u64* __restrict__ keys, u64 count,
-                                                            u32 vbits, bool packed,
-                                                            const u32* __restrict__ in_src,
-                                                            const u32* __restrict__ in_dst,
-                                                            const u32* __restrict__ in_w,
-                                                            u32* __restrict__ a, u32* __restrict__ b,
-                                                            u64* __restrict__ w) {
+__global__ __launch_bounds__(BLOCK) void mst_output_kernel(
+    const u64* __restrict__ keys, u64 count, u32 vbits, bool packed,
+    const u32* __restrict__ in_src, const u32* __restrict__ in_dst, const u32* __restrict__ in_w,
+    u32* __restrict__ out_s, u32* __restrict__ out_d, u32* __restrict__ out_w, u64* total) {
+  __shared__ u64 s_sum[WAVES];
+  u64 sum = 0;
+  const u64 mask = (u64{1} << vbits) - 1;
   for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < count; i += u64(gridDim.x) * BLOCK) {
     const u64 key = keys[i];
+    u32 a, b, x;
     if (packed) {
-      const u64 mask = (u64{1} << vbits) - 1;
-      a[i] = static_cast<u32>((key >> vbits) & mask);
-      b[i] = static_cast<u32>(key & mask);
-      w[i] = key >> (2 * vbits);
+      a = static_cast<u32>((key >> vbits) & mask);
+      b = static_cast<u32>(key & mask);
+      x = static_cast<u32>(key >> (2 * vbits));
     } else {
       const u32 uid = static_cast<u32>(key);
-      const u32 x = in_src[uid], y = in_dst[uid];
-      a[i] = x < y ? x : y;
-      b[i] = x < y ? y : x;
-      w[i] = in_w[uid];
+      const u32 s = in_src[uid], d = in_dst[uid];
+      a = s < d ? s : d;
+      b = s < d ? d : s;
+      x = in_w[uid];
     }
-  }
-}
-
-__global__ __launch_bounds__(BLOCK) void mst_output_kernel(const u32* __restrict__ a,
-                                                           const u32* __restrict__ b,
-                                                           const u64* __restrict__ w, u64 count,
-                                                           u32* __restrict__ out_s,
-                                                           u32* __restrict__ out_d,
-                                                           u32* __restrict__ out_w, u64* total) {
-  u64 sum = 0;
-  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < count; i += u64(gridDim.x) * BLOCK) {
-    out_s[i] = a[i];
-    out_d[i] = b[i];
-    const u32 x = static_cast<u32>(w[i]);
+    out_s[i] = a;
+    out_d[i] = b;
     out_w[i] = x;
     sum += x;
   }
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, WAVE);
-  if (lane_id() == 0 && sum) atomicAdd(reinterpret_cast<unsigned long long*>(total), sum);
+  if (lane_id() == 0) s_sum[wave_id()] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 t = 0;
+    for (int w = 0; w < WAVES; ++w) t += s_sum[w];
+    if (t) atomicAdd(reinterpret_cast<unsigned long long*>(total), t);
+  }
 }
 
-void mst_triples(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
-                 const u32* in_src, const u32* in_dst, const u32* in_w, u32* a, u32* b, u64* w) {
+void mst_output(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
+                const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
+                u32* out_w, u64* total_dev) {
   if (count == 0) return;
-  KTScope t(st, K_GATHER, packed ? 24.0 * count : 28.0 * count);
-  mst_triples_kernel<<<grid_for(count, 4096), BLOCK, 0, st>>>(keys, count, vbits, packed, in_src,
-                                                              in_dst, in_w, a, b, w);
-  KMST_HIP(hipGetLastError());
-}
-
-void mst_output(hipStream_t st, const u32* a, const u32* b, const u64* w, u64 count, u32* out_s,
-                u32* out_d, u32* out_w, u64* total_dev) {
-  if (count == 0) return;
-  KTScope t(st, K_GATHER, 28.0 * count);
-  mst_output_kernel<<<grid_for(count, 4096), BLOCK, 0, st>>>(a, b, w, count, out_s, out_d, out_w,
+  KTScope t(st, K_GATHER, (packed ? 20.0 : 24.0) * count);
+  mst_output_kernel<<<grid_for(count, 1024), BLOCK, 0, st>>>(keys, count, vbits, packed, in_src,
+                                                             in_dst, in_w, out_s, out_d, out_w,
                                                              total_dev);
   KMST_HIP(hipGetLastError());
 }

@@ -34,8 +34,10 @@ void fill_u32(hipStream_t st, u32* p, u64 n, u32 value);
 u64 scan_scratch_elems(u64 n);
 void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scratch,
                         u32* total_dev);
+// With flagged_low (n entries) also flagged_low[i] = (u32) out[i] where bit 0
+// of in[i] is set, NONE32 elsewhere.
 void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scratch,
-                        u64* total_dev);
+                        u64* total_dev, u32* flagged_low = nullptr);
 
 // Setup: contract checks (hsum[0..1] = forward / reverse multiset hashes),
 // CSR offsets, strict keys (packed when they fit 64 bits, positional else).
@@ -74,8 +76,9 @@ void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const KeyS
 void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* parent,
           u64* state);
 void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err);
-void make_labels(hipStream_t st, u64 n, const u32* parent, const u64* state, const u64* pos,
-                 const u64* best_val, u32* label, u64* mst, u64 mst_base);
+// root_id[r] = new id of root r (NONE32 for other vertices), from the scan
+void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_id, const u64* state,
+                 const u64* pos, const u64* best_val, u32* label, u64* mst, u64 mst_base);
 
 // K4+K5 relabel, drop self-loops and dead records, order-preserving compaction
 // in one pass.  lb needs lookback_elems(m) u64 entries; *count_dev gets the
@@ -137,11 +140,15 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
 // K7 keep the lightest record of each (s, d) run (others marked dead).
 void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m);
 
-// K9 MST keys -> (min, max, w) triples; final copy-out + u64 weight.
-void mst_triples(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
-                 const u32* in_src, const u32* in_dst, const u32* in_w, u32* a, u32* b, u64* w);
-void mst_output(hipStream_t st, const u32* a, const u32* b, const u64* w, u64 count, u32* out_s,
-                u32* out_d, u32* out_w, u64* total_dev);
+// The same sort for one u64 array by its low `bits` bits.
+bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWork& work,
+                    u32* err);
+
+// K9 MST keys (sorted by their low bits: (min, max) for packed keys, the twin
+// position for positional ones) -> canonical (min, max, w) output + u64 weight.
+void mst_output(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
+                const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
+                u32* out_w, u64* total_dev);
 
 // Filter-Boruvka helpers.
 void iota_u32(hipStream_t st, u32* p, u64 n);

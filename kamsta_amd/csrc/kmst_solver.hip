@@ -349,9 +349,10 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
                      ctr + C_PIECES);
     hook(st, n, best_val, best_dst, parent, state);
     pointer_jump(st, n, parent, ctr + C_ERR);
+    // best_dst is free after the hook: it takes the new ids of the roots
     scan_exclusive_u64(st, state, pos, n, reinterpret_cast<u64*>(scan_scratch),
-                       reinterpret_cast<u64*>(ctr + C_ROUND64));
-    make_labels(st, n, parent, state, pos, best_val, label, mst, mst_count);
+                       reinterpret_cast<u64*>(ctr + C_ROUND64), best_dst);
+    make_labels(st, n, parent, best_dst, state, pos, best_val, label, mst, mst_count);
     if (track_gpar) {
       update_parent_array(st, n, first ? nullptr : rep, parent, state, pos, gpar, rep_next);
       std::swap(rep, rep_next);
@@ -640,17 +641,18 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
   } else {
     rounds(input, X, Y, false);
   }
-  // finish: keys -> canonical triples, sort by (min, max), copy out
+  // finish: sort the MST keys by (min, max) -- the low 2 bits(n) bits of a
+  // packed key, the twin position (monotone in (min, max)) of a positional
+  // one -- then decode them into the canonical output
   const u64 cnt = mst_count;
   if (cnt > out_cap) throw Error(KMST_EINVAL, "output capacity too small");
-  mst_triples(st, mst, cnt, vbits, packed, in_s, in_d, in_w, X.s, X.d, X.v);
-  RecView res{X.s, X.d, X.v, cnt};
-  if (cnt > 1 && radix_sort(st, X.s, X.d, X.v, Y.s, Y.d, Y.v, cnt, vbits, 2 * vbits,
-                            C.sort_work, ctr + C_ERR))
-    res = RecView{Y.s, Y.d, Y.v, cnt};
+  const u64* keys = mst;
+  if (cnt > 1 && radix_sort_u64(st, mst, X.v, cnt, packed ? 2 * vbits : 32, C.sort_work,
+                                ctr + C_ERR))
+    keys = X.v;
   u64* total_dev = reinterpret_cast<u64*>(ctr + C_TOTAL64);
   fill_u64(st, total_dev, 1, 0);
-  mst_output(st, res.s, res.d, res.v, cnt, out_s, out_d, out_w, total_dev);
+  mst_output(st, keys, cnt, vbits, packed, in_s, in_d, in_w, out_s, out_d, out_w, total_dev);
   check_errors("finish");
   *out_n = cnt;
   *total = counter64(C_TOTAL64);
