@@ -175,6 +175,15 @@ __device__ __forceinline__ u64 s_red_total(const u64 (&t)[WAVES][COLS]) {
   return s_red_sum(t, 3);  // column 3 holds the record counts
 }
 
+// Records are taken in units of 256: unit u is chunk 64u .. 64u + 63 of four
+// consecutive records, one chunk per lane of a wave.  Validation writes each
+// unit's count of the two lightest weight classes, so that the class scatter
+// can place every unit without waiting on other units.
+constexpr u64 UNIT_RECORDS = 4 * WAVE;
+__device__ __forceinline__ u64 record_units_dev(u64 m) {
+  return (m + UNIT_RECORDS - 1) / UNIT_RECORDS;
+}
+
 template <bool VEC>
 __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__ src,
                                                          const u32* __restrict__ dst,
@@ -184,7 +193,9 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
                                                          const WeightPivots P,
                                                          u64* class_counts,
                                                          u32* __restrict__ seg_start,
-                                                         u32* __restrict__ seg_end) {
+                                                         u32* __restrict__ seg_end, u32 q0,
+                                                         u32 q1, u32* __restrict__ unit_c0,
+                                                         u32* __restrict__ unit_c1) {
   __shared__ u64 s_red[WAVES][MAX_PIVOTS + 4];
   const int lane = lane_id();
   u32 bad = 0, fwd = 0, mw = 0, seen = 0;
@@ -212,14 +223,18 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
     for (u32 j = 0; j < MAX_PIVOTS; ++j)
       if (j < P.K) above[j] += (w > P.q[j]) ? 1u : 0u;
   };
+  auto cls_of = [&](u32 w) {  // class 0 / class 1 as 16-bit fields
+    return (w <= q0 ? 1u : 0u) + ((w > q0 && w <= q1) ? 0x10000u : 0u);
+  };
+  const u64 full = m / 4, chunks = (m + 3) / 4;
   if (VEC) {
     // chunks of 4 consecutive records (16-byte loads); consecutive lanes take
     // consecutive chunks, so the record before a chunk is the previous lane's
     // last one.  Two chunks per thread and iteration are loaded before use.
+    // A unit is the 64 chunks of one wave in one iteration.
     constexpr int VU = 2;
-    const u64 full = m / 4;
     const u64 stride = u64(gridDim.x) * BLOCK;
-    for (u64 base = u64(blockIdx.x) * BLOCK; base < full; base += VU * stride) {
+    for (u64 base = u64(blockIdx.x) * BLOCK; base < chunks; base += VU * stride) {  // uniform
       uint4 vs[VU], vd[VU], vw[VU];
 #pragma unroll
       for (int u = 0; u < VU; ++u) {
@@ -232,6 +247,7 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
       for (int u = 0; u < VU; ++u) {
         const u64 ch = base + u * stride + threadIdx.x;
         u32 ps = __shfl_up(vs[u].w, 1, WAVE), pd = __shfl_up(vd[u].w, 1, WAVE);
+        u32 cls = 0;
         if (ch < full) {
           if (lane == 0 && ch > 0) {
             ps = src[ch * 4 - 1];
@@ -241,6 +257,25 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
           one(ch * 4 + 1, vs[u].y, vd[u].y, vw[u].y, vs[u].x, vd[u].x, true);
           one(ch * 4 + 2, vs[u].z, vd[u].z, vw[u].z, vs[u].y, vd[u].y, true);
           one(ch * 4 + 3, vs[u].w, vd[u].w, vw[u].w, vs[u].z, vd[u].z, true);
+          cls = cls_of(vw[u].x) + cls_of(vw[u].y) + cls_of(vw[u].z) + cls_of(vw[u].w);
+        } else if (ch == full && unit_c0) {
+          // the last, partial chunk: counted here, checked below
+          for (u64 i = full * 4; i < m; ++i) cls += cls_of(wt[i]);
+        }
+        if (unit_c0) {  // uniform
+          // per-lane counts are 0..4: sum them bit by bit with ballots
+          const u32 c0 = cls & 0xFFFFu, c1 = cls >> 16;
+          u32 t0 = 0, t1 = 0;
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            t0 += static_cast<u32>(__popcll(__ballot((c0 >> b) & 1u))) << b;
+            t1 += static_cast<u32>(__popcll(__ballot((c1 >> b) & 1u))) << b;
+          }
+          const u64 unit_chunk = base + u * stride + u64(wave_id()) * WAVE;
+          if (lane == 0 && unit_chunk < chunks) {
+            unit_c0[unit_chunk / WAVE] = t0;
+            unit_c1[unit_chunk / WAVE] = t1;
+          }
         }
       }
     }
@@ -249,6 +284,8 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
         one(i, src[i], dst[i], wt[i], i ? src[i - 1] : 0u, i ? dst[i - 1] : 0u, i > 0);
     }
   } else {
+    // unaligned arrays: one record per thread (no unit counts; see
+    // unit_class_counts)
     for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK)
       one(i, src[i], dst[i], wt[i], i ? src[i - 1] : 0u, i ? dst[i - 1] : 0u, i > 0);
   }
@@ -299,20 +336,64 @@ __global__ __launch_bounds__(BLOCK) void validate_kernel(const u32* __restrict__
   }
 }
 
-void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m, u64 n,
+u64 record_units(u64 m) { return (m + UNIT_RECORDS - 1) / UNIT_RECORDS; }
+
+bool validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m, u64 n,
                       u32* err, u32* fwd_count, u32* max_w, u64* hsum, const WeightPivots& P,
-                      u64* class_counts, u32* seg_start, u32* seg_end) {
-  if (m == 0) return;
-  KTScope t(st, K_VALIDATE, 12.0 * m);
-  const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst) |
-                     reinterpret_cast<uintptr_t>(w)) & 15) == 0;
+                      u64* class_counts, u32* seg_start, u32* seg_end, u32 q0, u32 q1,
+                      u32* unit_c0, u32* unit_c1) {
+  if (m == 0) return false;
+  KTScope t(st, K_VALIDATE, 12.0 * m + (unit_c0 ? 8.0 * record_units(m) : 0.0));
+  const bool vec = m >= 4 && ((reinterpret_cast<uintptr_t>(src) |
+                               reinterpret_cast<uintptr_t>(dst) |
+                               reinterpret_cast<uintptr_t>(w)) & 15) == 0;
   const u32 grid = grid_for((m + 3) / 4, 4096);
   if (vec)
     validate_kernel<true><<<grid, BLOCK, 0, st>>>(src, dst, w, m, n, err, fwd_count, max_w, hsum,
-                                                  P, class_counts, seg_start, seg_end);
+                                                  P, class_counts, seg_start, seg_end, q0, q1,
+                                                  unit_c0, unit_c1);
   else
     validate_kernel<false><<<grid, BLOCK, 0, st>>>(src, dst, w, m, n, err, fwd_count, max_w, hsum,
-                                                   P, class_counts, seg_start, seg_end);
+                                                   P, class_counts, seg_start, seg_end, q0, q1,
+                                                   nullptr, nullptr);
+  KMST_HIP(hipGetLastError());
+  return vec && unit_c0;
+}
+
+// Class 0 / class 1 counts of every 256-record unit from the weights alone
+// (for inputs whose arrays are not 16-byte aligned).
+__global__ __launch_bounds__(BLOCK) void unit_class_counts_kernel(const u32* __restrict__ wt,
+                                                                  u64 m, u32 q0, u32 q1,
+                                                                  u32* __restrict__ unit_c0,
+                                                                  u32* __restrict__ unit_c1) {
+  const int lane = lane_id();
+  const u64 units = record_units_dev(m);
+  const u64 wstride = u64(gridDim.x) * WAVES;
+  for (u64 unit = u64(blockIdx.x) * WAVES + wave_id(); unit < units; unit += wstride) {
+    u32 cls = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const u64 i = unit * UNIT_RECORDS + u64(e) * WAVE + lane;
+      if (i < m) {
+        const u32 w = wt[i];
+        cls += (w <= q0 ? 1u : 0u) + ((w > q0 && w <= q1) ? 0x10000u : 0u);
+      }
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) cls += __shfl_xor(cls, o, WAVE);
+    if (lane == 0) {
+      unit_c0[unit] = cls & 0xFFFFu;
+      unit_c1[unit] = cls >> 16;
+    }
+  }
+}
+
+void unit_class_counts(hipStream_t st, const u32* w, u64 m, u32 q0, u32 q1, u32* unit_c0,
+                       u32* unit_c1) {
+  if (m == 0) return;
+  KTScope t(st, K_OTHER, 4.0 * m + 8.0 * record_units(m));
+  const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + WAVES - 1) / WAVES, 8192));
+  unit_class_counts_kernel<<<grid, BLOCK, 0, st>>>(w, m, q0, q1, unit_c0, unit_c1);
   KMST_HIP(hipGetLastError());
 }
 
@@ -1909,95 +1990,97 @@ void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits)
   KMST_HIP(hipGetLastError());
 }
 
-// Multi-way weight-class scatter (all levels of the filter recursion's
-// light/heavy partition in one pass; the pivots are quantiles of one sample).
-// Class 0 (the lightest) keeps input order, so it stays sorted by (src, dst):
-// its offsets come from a decoupled look-back.  Classes 1..K-1 are relabelled
-// and re-sorted before use, so each tile takes one atomic slot range per
-// class.  Class K (heavier than every pivot) is not written.
+// Scatter of the two lightest weight classes: class 0 (w <= q0) to
+// out[off0[u] + ...] and class 1 (q0 < w <= q1) to out[base1 + off1[u] + ...],
+// both in input order (off0 / off1: exclusive scans of the unit counts that
+// validation wrote).  One wave per 256-record unit, no waits between units.
+template <bool VEC>
 __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 m,
-    const WeightPivots P, u32 wshift, const ClassBases B, u64 out_cap, u32* __restrict__ s_out,
-    u32* __restrict__ d_out, u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles,
-    u64* __restrict__ class_cursor, u32* err) {
-  const u32 K = P.K;
-  __shared__ u32 s_wave0[WAVES];
-  __shared__ u32 s_cnt[MAX_CLASSES];
-  __shared__ u64 s_base[MAX_CLASSES];
-  __shared__ u64 s_excl;
-  __shared__ u32 s_ok;
-  if (threadIdx.x < MAX_CLASSES) s_cnt[threadIdx.x] = 0;
-  const u32 tile = lb_tile_id(lb_counter);  // includes a barrier
-  const int lane = lane_id(), w = wave_id();
-  const u64 wbase = u64(tile) * CMP_TILE + u64(w) * (WAVE * CMP_IPT);
-  u32 ss[CMP_IPT], dd[CMP_IPT], slot[CMP_IPT];
-  u64 vv[CMP_IPT];
-  u8 cls[CMP_IPT];
-  u64 bal[CMP_IPT];
-  u32 cnt0 = 0;
-  load_records(s_in, d_in, keys, wbase, m, ss, dd, vv);
-#pragma unroll
-  for (int k = 0; k < CMP_IPT; ++k) {
-    const u64 e = wbase + u64(k) * WAVE + lane;
-    const u32 c = e < m ? weight_class(static_cast<u32>(vv[k] >> wshift), P) : K;  // K: not written
-    cls[k] = static_cast<u8>(c);
-    bal[k] = __ballot(c == 0);
-    cnt0 += static_cast<u32>(__popcll(bal[k]));
-    slot[k] = (c > 0 && c < K) ? atomicAdd(&s_cnt[c], 1u) : 0u;
-  }
-  if (lane == 0) s_wave0[w] = cnt0;
-  __syncthreads();
-  if (w == 0) {
-    u64 agg = 0;
-#pragma unroll
-    for (int ww = 0; ww < WAVES; ++ww) agg += s_wave0[ww];
-    bool ok;
-    const u64 excl = lookback_exclusive(lb_status, tile, agg, err, ok);
-    if (lane == 0) {
-      s_excl = excl;
-      s_ok = ok ? 1u : 0u;
-    }
-  } else if (w == 1 && lane > 0 && u32(lane) < K) {
-    const u32 c = s_cnt[lane];
-    s_base[lane] = B.base[lane] +
-                   (c ? atomicAdd(reinterpret_cast<unsigned long long*>(class_cursor + lane),
-                                  static_cast<unsigned long long>(c))
-                      : 0ull);
-  }
-  __syncthreads();
-  if (!s_ok) return;
-  u64 pos0 = B.base[0] + s_excl;
-  for (int ww = 0; ww < w; ++ww) pos0 += s_wave0[ww];
-#pragma unroll
-  for (int k = 0; k < CMP_IPT; ++k) {
-    const u32 c = cls[k];
-    if (c < K) {
-      const u64 p = (c == 0) ? pos0 + mbcnt(bal[k]) : s_base[c] + slot[k];
-      if (p < out_cap) {
-        s_out[p] = ss[k];
-        d_out[p] = dd[k];
-        v_out[p] = vv[k];
+    u32 wshift, u32 q0, u32 q1, const u32* __restrict__ off0, const u32* __restrict__ off1,
+    u64 base1, u64 out_cap, u32* __restrict__ s_out, u32* __restrict__ d_out,
+    u64* __restrict__ v_out, u32* err) {
+  const int lane = lane_id();
+  const u64 units = record_units_dev(m);
+  const u64 wstride = u64(gridDim.x) * WAVES;
+  for (u64 unit = u64(blockIdx.x) * WAVES + wave_id(); unit < units; unit += wstride) {
+    const u64 r0 = unit * UNIT_RECORDS + 4 * u64(lane);
+    u32 ss[4], dd[4];
+    u64 vv[4];
+    if (VEC && r0 + 3 < m) {
+      const uint4 a = *reinterpret_cast<const uint4*>(s_in + r0);
+      const uint4 b = *reinterpret_cast<const uint4*>(d_in + r0);
+      ss[0] = a.x; ss[1] = a.y; ss[2] = a.z; ss[3] = a.w;
+      dd[0] = b.x; dd[1] = b.y; dd[2] = b.z; dd[3] = b.w;
+      if (keys.v) {
+        const uint4 c = *reinterpret_cast<const uint4*>(keys.v + r0);
+        const uint4 e = *reinterpret_cast<const uint4*>(keys.v + r0 + 2);
+        vv[0] = (u64(c.y) << 32) | c.x;
+        vv[1] = (u64(c.w) << 32) | c.z;
+        vv[2] = (u64(e.y) << 32) | e.x;
+        vv[3] = (u64(e.w) << 32) | e.z;
       } else {
-        atomicOr(err, ERR_BOUNDS);
+        const uint4 c = *reinterpret_cast<const uint4*>(keys.w + r0);
+        vv[0] = pack_key(c.x, ss[0], dd[0], keys.vbits);
+        vv[1] = pack_key(c.y, ss[1], dd[1], keys.vbits);
+        vv[2] = pack_key(c.z, ss[2], dd[2], keys.vbits);
+        vv[3] = pack_key(c.w, ss[3], dd[3], keys.vbits);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const u64 i = min(r0 + e, m - 1);
+        ss[e] = s_in[i];
+        dd[e] = d_in[i];
+        vv[e] = load_key(keys, i, ss[e], dd[e]);
       }
     }
-    pos0 += static_cast<u32>(__popcll(bal[k]));
+    u32 cls[4];
+    u64 b0[4], b1[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const u32 w = static_cast<u32>(vv[e] >> wshift);
+      cls[e] = (r0 + e >= m) ? 2u : (w <= q0 ? 0u : (w <= q1 ? 1u : 2u));
+      b0[e] = __ballot(cls[e] == 0);
+      b1[e] = __ballot(cls[e] == 1);
+    }
+    // lane l's records follow those of lanes < l
+    u64 p0 = off0[unit] + mbcnt(b0[0]) + mbcnt(b0[1]) + mbcnt(b0[2]) + mbcnt(b0[3]);
+    u64 p1 = base1 + off1[unit] + mbcnt(b1[0]) + mbcnt(b1[1]) + mbcnt(b1[2]) + mbcnt(b1[3]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (cls[e] < 2) {
+        const u64 p = cls[e] == 0 ? p0++ : p1++;
+        if (p < out_cap) {
+          s_out[p] = ss[e];
+          d_out[p] = dd[e];
+          v_out[p] = vv[e];
+        } else {
+          atomicOr(err, ERR_BOUNDS);
+        }
+      }
+    }
   }
 }
 
 void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys, u64 m,
-                   const WeightPivots& P, u32 wshift, const ClassBases& B, u64 written,
-                   u32* s_out, u32* d_out, u64* v_out, u64* lb, u64* class_cursor, u32* err) {
-  if (m == 0 || P.K == 0) return;
-  const u32 tiles = ceil_div_u32(m, CMP_TILE);
-  fill_u64(st, lb, u64(tiles) + 1, 0);
-  fill_u64(st, class_cursor, MAX_CLASSES, 0);
+                   u32 wshift, u32 q0, u32 q1, const u32* off0, const u32* off1, u64 base1,
+                   u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err) {
+  if (m == 0) return;
   KTScope t(st, K_SPLIT_SCATTER, (keys.v ? 16.0 : 12.0) * m + 16.0 * double(written));
-  class_scatter_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, m, P, wshift, B, written, s_out,
-                                                d_out,
-                                                v_out, lb,
-                                                reinterpret_cast<u32*>(lb + tiles), tiles,
-                                                class_cursor, err);
+  const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + WAVES - 1) / WAVES, 8192));
+  const bool vec = ((reinterpret_cast<uintptr_t>(s_in) | reinterpret_cast<uintptr_t>(d_in) |
+                     reinterpret_cast<uintptr_t>(keys.v ? static_cast<const void*>(keys.v)
+                                                        : static_cast<const void*>(keys.w))) &
+                    15) == 0;
+  if (vec)
+    class_scatter_kernel<true><<<grid, BLOCK, 0, st>>>(s_in, d_in, keys, m, wshift, q0, q1, off0,
+                                                       off1, base1, written, s_out, d_out, v_out,
+                                                       err);
+  else
+    class_scatter_kernel<false><<<grid, BLOCK, 0, st>>>(s_in, d_in, keys, m, wshift, q0, q1, off0,
+                                                        off1, base1, written, s_out, d_out, v_out,
+                                                        err);
   KMST_HIP(hipGetLastError());
 }
 

@@ -48,18 +48,22 @@ struct WeightPivots {
   u32 q[MAX_PIVOTS];
   u32 K;
 };
-struct ClassBases {
-  u64 base[MAX_CLASSES];
-};
 // With P.K > 0 it also counts the records of every weight class into
 // class_counts[0..K] (zeroed by the caller); with seg_start / seg_end (n
 // entries each, seg_end zeroed by the caller) it records every vertex's record
 // range [seg_start[v], seg_end[v]) (seg_end stays 0 for vertices without
-// records).
-void validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
+// records).  With unit_c0 / unit_c1 (record_units(m) entries each) it writes
+// how many records of every 256-record unit fall in weight class 0 (w <= q0)
+// and class 1 (q0 < w <= q1), when the arrays are 16-byte aligned; it returns
+// whether it did (else unit_class_counts does it from the weights).
+u64 record_units(u64 m);
+void unit_class_counts(hipStream_t st, const u32* w, u64 m, u32 q0, u32 q1, u32* unit_c0,
+                       u32* unit_c1);
+bool validate_records(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                       u64 n, u32* err, u32* fwd_count, u32* max_w, u64* hsum,
                       const WeightPivots& P, u64* class_counts, u32* seg_start = nullptr,
-                      u32* seg_end = nullptr);
+                      u32* seg_end = nullptr, u32 q0 = 0, u32 q1 = 0, u32* unit_c0 = nullptr,
+                      u32* unit_c1 = nullptr);
 void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off);
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                 const u32* off, u64* val, u32* err);
@@ -166,12 +170,12 @@ void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32
 void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag);
 void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32* rep);
 
-// Weight classes 0..K-1 of the input go to out[B.base[c] ...]; class 0
-// keeps input order, the others are in no particular order.  written = total
-// size of classes 0..K-1 (positions at or past it raise ERR_BOUNDS and are
-// not written).  class_cursor needs MAX_CLASSES u64 entries.
+// Weight classes 0 (w <= q0) and 1 (q0 < w <= q1) of the input, in input
+// order, to out[0, c0) and out[base1, base1 + c1); off0 / off1 are the
+// exclusive scans of validation's unit counts.  written = c0 + c1; positions
+// at or past it raise ERR_BOUNDS and are not written.
 void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys, u64 m,
-                   const WeightPivots& P, u32 wshift, const ClassBases& B, u64 written,
-                   u32* s_out, u32* d_out, u64* v_out, u64* lb, u64* class_cursor, u32* err);
+                   u32 wshift, u32 q0, u32 q1, const u32* off0, const u32* off1, u64 base1,
+                   u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err);
 
 }  // namespace kmst

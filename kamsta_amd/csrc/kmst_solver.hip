@@ -138,7 +138,6 @@ enum Ctr : int {
   C_ERR = 0, C_FWD = 1, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6, C_MAXW = 7,
   C_TOTAL64 = 16, C_HSUM64 = 18, C_ROUND64 = 22,
   C_CLASS64 = 24,   // MAX_CLASSES u64 class counts (words 24..41)
-  C_CURSOR64 = 42,  // MAX_CLASSES u64 class cursors (words 42..59)
   C_OUTSIDE = 60    // outside_list: count (60), u64 total range length (62..63)
 };
 
@@ -264,7 +263,7 @@ void Solver::allocate() {
   lb = static_cast<u64*>(C.lookback.get(lookback_elems(m) * 8));
   const u64 sort_m = std::max(m, n);
   C.sort_work.bind(st, static_cast<u32*>(C.hist.get(sort_work_words(sort_m) * 4)), sort_m);
-  const u64 scan_max = std::max<u64>(n, (m + CMP_TILE - 1) / CMP_TILE + 1);
+  const u64 scan_max = std::max<u64>({n, (m + CMP_TILE - 1) / CMP_TILE + 1, record_units(m)});
   // u64 scans reuse the same scratch, so size it in u64 units
   scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 8 + 64));
   ctr = static_cast<u32*>(C.counters.get(256));
@@ -295,10 +294,15 @@ void Solver::setup(bool filter, bool plan_classes) {
   // init_parent_array (hybrid_boruvka_edgefilter.hpp:332-352): vertices with
   // records, marked during validation
   if (filter) fill_u32(st, seg_end, n0, 0);
-  validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW,
-                   reinterpret_cast<u64*>(ctr + C_HSUM64), pivots,
-                   reinterpret_cast<u64*>(ctr + C_CLASS64), filter ? seg_start : nullptr,
-                   filter ? seg_end : nullptr);
+  // with weight classes, the counts of the two lightest per 256-record unit
+  // go to X.s / X.d (free until the class scatter)
+  const u32 q0 = pivots.K ? pivots.q[0] : 0, q1 = pivots.K >= 2 ? pivots.q[1] : q0;
+  const bool units = validate_records(
+      st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW,
+      reinterpret_cast<u64*>(ctr + C_HSUM64), pivots, reinterpret_cast<u64*>(ctr + C_CLASS64),
+      filter ? seg_start : nullptr, filter ? seg_end : nullptr, q0, q1,
+      plan_classes ? X.s : nullptr, plan_classes ? X.d : nullptr);
+  if (plan_classes && !units) unit_class_counts(st, in_w, m0, q0, q1, X.s, X.d);
   check_errors("input");
   if (u64(C.host_counters[C_FWD]) * 2 != m0 || counter64(C_HSUM64) != counter64(C_HSUM64 + 2))
     throw Error(KMST_EASYMMETRIC, "input: records are not symmetric (src,dst,w) <-> (dst,src,w)");
@@ -458,21 +462,20 @@ void Solver::filter_classes() {
   // taken from the input when their turn comes, by then mostly through the
   // ranges of the few vertices outside the largest component
   const u32 J = std::min<u32>(K, 2);
-  WeightPivots PJ{};
-  PJ.K = J;
-  for (u32 j = 0; j < J; ++j) PJ.q[j] = pivots.q[j];
-  ClassBases B{};
-  u64 written = 0;
-  for (u32 c = 0; c < J; ++c) {
-    B.base[c] = written;
-    written += cnt[c];
-  }
+  const u32 q0 = pivots.q[0], q1 = J >= 2 ? pivots.q[1] : q0;
+  const u64 c1 = J >= 2 ? cnt[1] : 0, written = cnt[0] + c1;
   const u32 wshift = packed ? 2 * vbits : 32;
   const RecView input{const_cast<u32*>(in_s), const_cast<u32*>(in_d), packed ? nullptr : val0, m0,
                       packed ? in_w : nullptr};
-  class_scatter(st, in_s, in_d, keys_of(input), m0, PJ, wshift, B, written, Y.s, Y.d, Y.v, lb,
-                reinterpret_cast<u64*>(ctr + C_CURSOR64), ctr + C_ERR);
+  // validation left the per-unit class counts in X.s / X.d: their exclusive
+  // scans place every unit's records
+  const u64 units = record_units(m0);
+  scan_exclusive_u32(st, X.s, X.s, units, scan_scratch, nullptr);
+  scan_exclusive_u32(st, X.d, X.d, units, scan_scratch, nullptr);
+  class_scatter(st, in_s, in_d, keys_of(input), m0, wshift, q0, q1, X.s, X.d, cnt[0], written,
+                Y.s, Y.d, Y.v, ctr + C_ERR);
   check_errors("split");
+  const u64 base1 = cnt[0];
   // level l (0 = top) splits classes 0..K-l at pivot q[K-1-l]
   std::vector<FilterStat> level(K);
   u64 part = m0;
@@ -486,7 +489,7 @@ void Solver::filter_classes() {
   filter_rec(Y.at(0, cnt[0]), X.at(0, cnt[0]), stop, static_cast<int>(K));
   if (J >= 2 && cnt[1] > 0) {
     C.stats.filter.push_back(level[K - 1]);
-    RecView cls = Y.at(B.base[1], cnt[1]);
+    RecView cls = Y.at(base1, cnt[1]);
     filter_heavy(cls, X.at(0, cnt[1]), cls, static_cast<int>(K - 1));
   }
   // class c holds keys in [(q[c-1] + 1) << wshift, ((q[c] + 1) << wshift) - 1]
