@@ -404,6 +404,8 @@ def main():
             t0 = time.perf_counter()
             for h, dv in zip(host, (s, d, w)):
                 dv.copy_(h, non_blocking=pin)
+            # the library solves on its own stream: the upload must be complete
+            torch.cuda.current_stream().synchronize()
             c, _ = solve(algos[best], outs)
             for oh, od in zip(out_h, outs):
                 oh[:c].copy_(od[:c], non_blocking=pin)

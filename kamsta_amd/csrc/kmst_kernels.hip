@@ -143,13 +143,15 @@ void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scr
 // and hash sums of (min, max, w) over min<max and over max<min records:
 // the input is symmetric as a multiset iff both hash sums and counts agree
 // (up to a 2^-64 collision chance).
+// Multiplications by odd 32-bit constants are bijections mod 2^64 and cost
+// three 32-bit multiplies each; the xor-shifts carry the high bits down.
 __device__ __forceinline__ u64 sym_hash(u32 a, u32 b, u32 w) {
-  u64 x = (u64(a) << 32 | b) ^ (u64(w) * 0x9E3779B97F4A7C15ull);
-  x ^= x >> 33;
-  x *= 0xFF51AFD7ED558CCDull;
-  x ^= x >> 33;
-  x *= 0xC4CEB9FE1A85EC53ull;
-  x ^= x >> 33;
+  u64 x = (u64(a) << 32 | b) * 0x9E3779B1ull;
+  x ^= (x >> 29) ^ (u64(w) << 17) ^ w;
+  x *= 0x85EBCA77ull;
+  x ^= x >> 32;
+  x *= 0xC2B2AE3Dull;
+  x ^= x >> 29;
   return x;
 }
 
@@ -394,6 +396,118 @@ void unit_class_counts(hipStream_t st, const u32* w, u64 m, u32 q0, u32 q1, u32*
   KTScope t(st, K_OTHER, 4.0 * m + 8.0 * record_units(m));
   const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + WAVES - 1) / WAVES, 8192));
   unit_class_counts_kernel<<<grid, BLOCK, 0, st>>>(w, m, q0, q1, unit_c0, unit_c1);
+  KMST_HIP(hipGetLastError());
+}
+
+// Weights-only pass: the largest weight, the record count of every weight
+// class and the class 0 / class 1 counts of every 256-record unit.  A wave
+// takes WC_UNITS consecutive units per step (lane l: records 4l .. 4l + 3 of
+// each, one 16-byte load), loads them all before counting, and writes their
+// unit counts with one store.
+constexpr int WC_UNITS = 4;
+
+template <bool VEC>
+__global__ __launch_bounds__(BLOCK) void weight_classes_kernel(
+    const u32* __restrict__ wt, u64 m, const WeightPivots P, u32 q0, u32 q1,
+    u64* class_counts, u32* max_w, u32* __restrict__ unit_c0, u32* __restrict__ unit_c1) {
+  __shared__ u64 s_red[WAVES][MAX_PIVOTS + 4];
+  const int lane = lane_id();
+  const u64 units = record_units_dev(m);
+  const u64 steps = (units + WC_UNITS - 1) / WC_UNITS;
+  const u64 wstride = u64(gridDim.x) * WAVES;
+  u32 mw = 0, seen = 0;
+  u32 above[MAX_PIVOTS];
+#pragma unroll
+  for (int j = 0; j < MAX_PIVOTS; ++j) above[j] = 0;
+  for (u64 step = u64(blockIdx.x) * WAVES + wave_id(); step < steps; step += wstride) {
+    const u64 u0 = step * WC_UNITS;
+    u32 w4[WC_UNITS][4];
+#pragma unroll
+    for (int u = 0; u < WC_UNITS; ++u) {
+      const u64 r0 = (u0 + u) * UNIT_RECORDS + 4 * u64(lane);
+      if (VEC && r0 + 3 < m) {
+        const uint4 a = *reinterpret_cast<const uint4*>(wt + r0);
+        w4[u][0] = a.x; w4[u][1] = a.y; w4[u][2] = a.z; w4[u][3] = a.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w4[u][e] = (r0 + e < m) ? wt[r0 + e] : 0u;
+      }
+    }
+    u32 mine0 = 0, mine1 = 0;  // lane u < WC_UNITS keeps unit u0 + u's counts
+#pragma unroll
+    for (int u = 0; u < WC_UNITS; ++u) {
+      const u64 unit = u0 + u;  // wave-uniform
+      if (unit >= units) break;
+      const u64 r0 = unit * UNIT_RECORDS + 4 * u64(lane);
+      u32 c0 = 0, c1 = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (r0 + e >= m) continue;
+        const u32 w = w4[u][e];
+        mw = w > mw ? w : mw;
+        ++seen;
+        c0 += w <= q0 ? 1u : 0u;
+        c1 += (w > q0 && w <= q1) ? 1u : 0u;
+#pragma unroll
+        for (u32 j = 0; j < MAX_PIVOTS; ++j)
+          if (j < P.K) above[j] += (w > P.q[j]) ? 1u : 0u;
+      }
+      // per-lane counts are 0..4: sum them bit by bit with ballots
+      u32 t0 = 0, t1 = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        t0 += static_cast<u32>(__popcll(__ballot((c0 >> b) & 1u))) << b;
+        t1 += static_cast<u32>(__popcll(__ballot((c1 >> b) & 1u))) << b;
+      }
+      if (lane == u) {
+        mine0 = t0;
+        mine1 = t1;
+      }
+    }
+    if (lane < WC_UNITS && u0 + lane < units) {
+      unit_c0[u0 + lane] = mine0;
+      unit_c1[u0 + lane] = mine1;
+    }
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    seen += __shfl_xor(seen, o, WAVE);
+    const u32 om = __shfl_xor(mw, o, WAVE);
+    mw = om > mw ? om : mw;
+#pragma unroll
+    for (u32 j = 0; j < MAX_PIVOTS; ++j)
+      if (j < P.K) above[j] += __shfl_xor(above[j], o, WAVE);
+  }
+  const int w_ = wave_id();
+  if (lane == 0) {
+    s_red[w_][3] = seen;
+    for (int j = 0; j < MAX_PIVOTS; ++j) s_red[w_][4 + j] = above[j];
+    if (mw) atomicMax(max_w, mw);
+  }
+  __syncthreads();
+  if (threadIdx.x <= P.K) {
+    // class c counts: total - above[0], above[c-1] - above[c], above[K-1]
+    const u32 c = threadIdx.x;
+    const u64 lo = (c == 0) ? s_red_total(s_red) : s_red_sum(s_red, 4 + c - 1);
+    const u64 hi = (c == P.K) ? 0ull : s_red_sum(s_red, 4 + c);
+    const u64 v = lo - hi;
+    if (v) atomicAdd(reinterpret_cast<unsigned long long*>(class_counts + c),
+                     static_cast<unsigned long long>(v));
+  }
+}
+
+void weight_classes(hipStream_t st, const u32* w, u64 m, const WeightPivots& P, u32 q0, u32 q1,
+                    u64* class_counts, u32* max_w, u32* unit_c0, u32* unit_c1) {
+  if (m == 0) return;
+  KTScope t(st, K_VALIDATE, 4.0 * m + 8.0 * record_units(m));
+  const u64 waves = (record_units(m) + WC_UNITS - 1) / WC_UNITS;
+  const u32 grid = static_cast<u32>(std::min<u64>((waves + WAVES - 1) / WAVES, 8192));
+  if ((reinterpret_cast<uintptr_t>(w) & 15) == 0)
+    weight_classes_kernel<true><<<grid, BLOCK, 0, st>>>(w, m, P, q0, q1, class_counts, max_w,
+                                                        unit_c0, unit_c1);
+  else
+    weight_classes_kernel<false><<<grid, BLOCK, 0, st>>>(w, m, P, q0, q1, class_counts, max_w,
+                                                         unit_c0, unit_c1);
   KMST_HIP(hipGetLastError());
 }
 
@@ -1992,95 +2106,221 @@ void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits)
 
 // Scatter of the two lightest weight classes: class 0 (w <= q0) to
 // out[off0[u] + ...] and class 1 (q0 < w <= q1) to out[base1 + off1[u] + ...],
-// both in input order (off0 / off1: exclusive scans of the unit counts that
-// validation wrote).  One wave per 256-record unit, no waits between units.
-template <bool VEC>
+// both in input order (off0 / off1: exclusive scans of the unit counts of the
+// weights pass).  One wave per 256-record unit, no waits between units.  With
+// CHECK the same pass validates the input as validate_kernel does (ids, order,
+// symmetry hashes, forward count, every vertex's record range), so the input
+// is read once for both.
+// A block takes a chunk of consecutive units per step (two per wave, one
+// with the input check, which needs the registers).  The chunk's class 0 and
+// class 1 records are contiguous in the output, so they are staged in LDS and
+// leave in coalesced runs.
+template <bool CHECK>
+struct ScatterChunk {
+  static constexpr int WAVE_UNITS = CHECK ? 1 : 2;
+  static constexpr int UNITS = WAVES * WAVE_UNITS;
+  static constexpr int RECORDS = UNITS * int(UNIT_RECORDS);
+};
+
+template <bool VEC, bool CHECK>
 __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 m,
     u32 wshift, u32 q0, u32 q1, const u32* __restrict__ off0, const u32* __restrict__ off1,
     u64 base1, u64 out_cap, u32* __restrict__ s_out, u32* __restrict__ d_out,
-    u64* __restrict__ v_out, u32* err) {
-  const int lane = lane_id();
+    u64* __restrict__ v_out, u32* err, const InputCheck chk) {
+  constexpr int CS_WAVE_UNITS = ScatterChunk<CHECK>::WAVE_UNITS;
+  constexpr int CS_CHUNK = ScatterChunk<CHECK>::UNITS;
+  constexpr int CS_CHUNK_RECORDS = ScatterChunk<CHECK>::RECORDS;
+  __shared__ u32 st_s[CS_CHUNK_RECORDS];
+  __shared__ u32 st_d[CS_CHUNK_RECORDS];
+  __shared__ u64 st_v[CS_CHUNK_RECORDS];
+  __shared__ u64 s_red[WAVES][3];
+  const int lane = lane_id(), wv = wave_id();
   const u64 units = record_units_dev(m);
-  const u64 wstride = u64(gridDim.x) * WAVES;
-  for (u64 unit = u64(blockIdx.x) * WAVES + wave_id(); unit < units; unit += wstride) {
-    const u64 r0 = unit * UNIT_RECORDS + 4 * u64(lane);
-    u32 ss[4], dd[4];
-    u64 vv[4];
-    if (VEC && r0 + 3 < m) {
-      const uint4 a = *reinterpret_cast<const uint4*>(s_in + r0);
-      const uint4 b = *reinterpret_cast<const uint4*>(d_in + r0);
-      ss[0] = a.x; ss[1] = a.y; ss[2] = a.z; ss[3] = a.w;
-      dd[0] = b.x; dd[1] = b.y; dd[2] = b.z; dd[3] = b.w;
-      if (keys.v) {
-        const uint4 c = *reinterpret_cast<const uint4*>(keys.v + r0);
-        const uint4 e = *reinterpret_cast<const uint4*>(keys.v + r0 + 2);
-        vv[0] = (u64(c.y) << 32) | c.x;
-        vv[1] = (u64(c.w) << 32) | c.z;
-        vv[2] = (u64(e.y) << 32) | e.x;
-        vv[3] = (u64(e.w) << 32) | e.z;
-      } else {
-        const uint4 c = *reinterpret_cast<const uint4*>(keys.w + r0);
-        vv[0] = pack_key(c.x, ss[0], dd[0], keys.vbits);
-        vv[1] = pack_key(c.y, ss[1], dd[1], keys.vbits);
-        vv[2] = pack_key(c.z, ss[2], dd[2], keys.vbits);
-        vv[3] = pack_key(c.w, ss[3], dd[3], keys.vbits);
-      }
-    } else {
+  const u64 chunks = (units + CS_CHUNK - 1) / CS_CHUNK;
+  const u64 total0 = base1, total1 = out_cap - base1;  // class sizes
+  u32 bad = 0, fwd = 0;
+  u64 hf = 0, hr = 0;
+  for (u64 chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {  // block-uniform
+    const u64 first = chunk * CS_CHUNK;
+    const u64 end = min(first + CS_CHUNK, units);
+    // the chunk's output runs: class 0 [a0, a0 + n0), class 1 [a1, a1 + n1)
+    const u64 a0 = off0[first], a1 = off1[first];
+    const u64 n0 = (end < units ? u64(off0[end]) : total0) - a0;
+    const u64 n1 = (end < units ? u64(off1[end]) : total1) - a1;
+    u32 ss[CS_WAVE_UNITS][4], dd[CS_WAVE_UNITS][4], ww[CS_WAVE_UNITS][4];
+    u64 vv[CS_WAVE_UNITS][4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const u64 i = min(r0 + e, m - 1);
-        ss[e] = s_in[i];
-        dd[e] = d_in[i];
-        vv[e] = load_key(keys, i, ss[e], dd[e]);
-      }
-    }
-    u32 cls[4];
-    u64 b0[4], b1[4];
+    for (int u = 0; u < CS_WAVE_UNITS; ++u) {
+      const u64 r0 = (first + wv * CS_WAVE_UNITS + u) * UNIT_RECORDS + 4 * u64(lane);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const u32 w = static_cast<u32>(vv[e] >> wshift);
-      cls[e] = (r0 + e >= m) ? 2u : (w <= q0 ? 0u : (w <= q1 ? 1u : 2u));
-      b0[e] = __ballot(cls[e] == 0);
-      b1[e] = __ballot(cls[e] == 1);
-    }
-    // lane l's records follow those of lanes < l
-    u64 p0 = off0[unit] + mbcnt(b0[0]) + mbcnt(b0[1]) + mbcnt(b0[2]) + mbcnt(b0[3]);
-    u64 p1 = base1 + off1[unit] + mbcnt(b1[0]) + mbcnt(b1[1]) + mbcnt(b1[2]) + mbcnt(b1[3]);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      if (cls[e] < 2) {
-        const u64 p = cls[e] == 0 ? p0++ : p1++;
-        if (p < out_cap) {
-          s_out[p] = ss[e];
-          d_out[p] = dd[e];
-          v_out[p] = vv[e];
+      for (int e = 0; e < 4; ++e) ww[u][e] = 0;
+      if (VEC && r0 + 3 < m) {
+        const uint4 a = *reinterpret_cast<const uint4*>(s_in + r0);
+        const uint4 b = *reinterpret_cast<const uint4*>(d_in + r0);
+        ss[u][0] = a.x; ss[u][1] = a.y; ss[u][2] = a.z; ss[u][3] = a.w;
+        dd[u][0] = b.x; dd[u][1] = b.y; dd[u][2] = b.z; dd[u][3] = b.w;
+        if (keys.v) {
+          const uint4 c = *reinterpret_cast<const uint4*>(keys.v + r0);
+          const uint4 e = *reinterpret_cast<const uint4*>(keys.v + r0 + 2);
+          vv[u][0] = (u64(c.y) << 32) | c.x;
+          vv[u][1] = (u64(c.w) << 32) | c.z;
+          vv[u][2] = (u64(e.y) << 32) | e.x;
+          vv[u][3] = (u64(e.w) << 32) | e.z;
         } else {
-          atomicOr(err, ERR_BOUNDS);
+          const uint4 c = *reinterpret_cast<const uint4*>(keys.w + r0);
+          ww[u][0] = c.x; ww[u][1] = c.y; ww[u][2] = c.z; ww[u][3] = c.w;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const u64 i = min(r0 + e, m - 1);
+          ss[u][e] = s_in[i];
+          dd[u][e] = d_in[i];
+          if (keys.v) vv[u][e] = keys.v[i];
+          else ww[u][e] = keys.w[i];
         }
       }
+    }
+    // the last record of the wave's first unit (lane 63's), for the check
+    const u32 last_s = __shfl(ss[0][3], WAVE - 1, WAVE), last_d = __shfl(dd[0][3], WAVE - 1, WAVE);
+#pragma unroll
+    for (int u = 0; u < CS_WAVE_UNITS; ++u) {
+      const u64 unit = first + wv * CS_WAVE_UNITS + u;  // wave-uniform
+      if (unit >= units) break;
+      const u64 r0 = unit * UNIT_RECORDS + 4 * u64(lane);
+      if (!keys.v) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) vv[u][e] = pack_key(ww[u][e], ss[u][e], dd[u][e], keys.vbits);
+      }
+      if (CHECK) {
+        // the record before lane l's first one is lane l - 1's last
+        u32 ps = __shfl_up(ss[u][3], 1, WAVE), pd = __shfl_up(dd[u][3], 1, WAVE);
+        if (lane == 0) {
+          if (u > 0) {
+            ps = last_s;
+            pd = last_d;
+          } else if (r0 > 0 && r0 < m) {
+            ps = s_in[r0 - 1];
+            pd = d_in[r0 - 1];
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const u64 i = r0 + e;
+          if (i >= m) break;
+          const u32 s = ss[u][e], d = dd[u][e];
+          const bool has_prev = i > 0;
+          if (e > 0) {
+            ps = ss[u][e - 1];
+            pd = dd[u][e - 1];
+          }
+          if (s >= chk.n || d >= chk.n || s == d) bad |= ERR_INVALID;
+          if (has_prev && (ps > s || (ps == s && pd > d))) bad |= ERR_UNSORTED;
+          if (!has_prev || ps != s) {
+            // record i opens s's range and closes the previous vertex's
+            if (s < chk.n) chk.seg_start[s] = static_cast<u32>(i);
+            if (has_prev && ps < chk.n) chk.seg_end[ps] = static_cast<u32>(i);
+          }
+          if (i + 1 == m && s < chk.n) chk.seg_end[s] = static_cast<u32>(m);
+          const bool f = s < d;
+          // CHECK runs with packed keys only: ww holds the weights
+          const u64 h = sym_hash(f ? s : d, f ? d : s, ww[u][e]);
+          fwd += f ? 1u : 0u;
+          hf += f ? h : 0ull;
+          hr += f ? 0ull : h;
+        }
+      }
+      u32 cls[4];
+      u64 b0[4], b1[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const u32 w = keys.v ? static_cast<u32>(vv[u][e] >> wshift) : ww[u][e];
+        cls[e] = (r0 + e >= m) ? 2u : (w <= q0 ? 0u : (w <= q1 ? 1u : 2u));
+        b0[e] = __ballot(cls[e] == 0);
+        b1[e] = __ballot(cls[e] == 1);
+      }
+      // stage positions: lane l's records follow those of lanes < l
+      u64 p0 = off0[unit] - a0 + mbcnt(b0[0]) + mbcnt(b0[1]) + mbcnt(b0[2]) + mbcnt(b0[3]);
+      u64 p1 = n0 + off1[unit] - a1 + mbcnt(b1[0]) + mbcnt(b1[1]) + mbcnt(b1[2]) + mbcnt(b1[3]);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (cls[e] < 2) {
+          const u64 p = cls[e] == 0 ? p0++ : p1++;
+          if (p < u64(CS_CHUNK_RECORDS)) {
+            st_s[p] = ss[u][e];
+            st_d[p] = dd[u][e];
+            st_v[p] = vv[u][e];
+          } else {
+            atomicOr(err, ERR_BOUNDS);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // coalesced runs out of the stage
+    const u64 nst = min<u64>(n0 + n1, CS_CHUNK_RECORDS);
+    for (u32 j = threadIdx.x; j < nst; j += BLOCK) {
+      const u64 p = j < n0 ? a0 + j : base1 + a1 + (j - n0);
+      if (p < out_cap) {
+        s_out[p] = st_s[j];
+        d_out[p] = st_d[j];
+        v_out[p] = st_v[j];
+      } else {
+        atomicOr(err, ERR_BOUNDS);
+      }
+    }
+    __syncthreads();
+  }
+  if (!CHECK) return;
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    fwd += __shfl_xor(fwd, o, WAVE);
+    hf += __shfl_xor(hf, o, WAVE);
+    hr += __shfl_xor(hr, o, WAVE);
+  }
+  if (lane == 0) {
+    s_red[wv][0] = fwd;
+    s_red[wv][1] = hf;
+    s_red[wv][2] = hr;
+  }
+  if (bad) atomicOr(err, bad);
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    u64 t = 0;
+    for (int ww_ = 0; ww_ < WAVES; ++ww_) t += s_red[ww_][threadIdx.x];
+    if (threadIdx.x == 0) {
+      if (t) atomicAdd(chk.fwd_count, static_cast<u32>(t));
+    } else if (t) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(chk.hsum + threadIdx.x - 1),
+                static_cast<unsigned long long>(t));
     }
   }
 }
 
 void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys, u64 m,
                    u32 wshift, u32 q0, u32 q1, const u32* off0, const u32* off1, u64 base1,
-                   u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err) {
+                   u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err,
+                   const InputCheck* check) {
   if (m == 0) return;
+  if (check && keys.v) throw Error(KMST_EINVAL, "input check needs packed keys");
   KTScope t(st, K_SPLIT_SCATTER, (keys.v ? 16.0 : 12.0) * m + 16.0 * double(written));
-  const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + WAVES - 1) / WAVES, 8192));
+  const u64 chunk = check ? ScatterChunk<true>::UNITS : ScatterChunk<false>::UNITS;
+  const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + chunk - 1) / chunk, 8192));
   const bool vec = ((reinterpret_cast<uintptr_t>(s_in) | reinterpret_cast<uintptr_t>(d_in) |
                      reinterpret_cast<uintptr_t>(keys.v ? static_cast<const void*>(keys.v)
                                                         : static_cast<const void*>(keys.w))) &
                     15) == 0;
-  if (vec)
-    class_scatter_kernel<true><<<grid, BLOCK, 0, st>>>(s_in, d_in, keys, m, wshift, q0, q1, off0,
-                                                       off1, base1, written, s_out, d_out, v_out,
-                                                       err);
-  else
-    class_scatter_kernel<false><<<grid, BLOCK, 0, st>>>(s_in, d_in, keys, m, wshift, q0, q1, off0,
-                                                        off1, base1, written, s_out, d_out, v_out,
-                                                        err);
+  const InputCheck chk = check ? *check : InputCheck{};
+#define KMST_CLASS_SCATTER(V, C)                                                                \
+  class_scatter_kernel<V, C><<<grid, BLOCK, 0, st>>>(s_in, d_in, keys, m, wshift, q0, q1, off0, \
+                                                     off1, base1, written, s_out, d_out, v_out, \
+                                                     err, chk)
+  if (vec && check) KMST_CLASS_SCATTER(true, true);
+  else if (vec) KMST_CLASS_SCATTER(true, false);
+  else if (check) KMST_CLASS_SCATTER(false, true);
+  else KMST_CLASS_SCATTER(false, false);
+#undef KMST_CLASS_SCATTER
   KMST_HIP(hipGetLastError());
 }
 

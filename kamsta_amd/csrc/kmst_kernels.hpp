@@ -170,12 +170,32 @@ void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32
 void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag);
 void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32* rep);
 
+// Weights-only pass over the input: *max_w, the record count of every weight
+// class into class_counts[0..P.K] (zeroed by the caller), and the class 0
+// (w <= q0) / class 1 (q0 < w <= q1) counts of every 256-record unit into
+// unit_c0 / unit_c1 (record_units(m) entries each).
+void weight_classes(hipStream_t st, const u32* w, u64 m, const WeightPivots& P, u32 q0, u32 q1,
+                    u64* class_counts, u32* max_w, u32* unit_c0, u32* unit_c1);
+
+// The input checks of validate_records, done inside the class scatter: ids
+// below n and no self-loops, (src, dst) order, forward count and the two
+// symmetry hashes (hsum[0..1]), and every vertex's record range.
+struct InputCheck {
+  u64 n = 0;
+  u32* fwd_count = nullptr;
+  u64* hsum = nullptr;
+  u32* seg_start = nullptr;
+  u32* seg_end = nullptr;
+};
+
 // Weight classes 0 (w <= q0) and 1 (q0 < w <= q1) of the input, in input
 // order, to out[0, c0) and out[base1, base1 + c1); off0 / off1 are the
-// exclusive scans of validation's unit counts.  written = c0 + c1; positions
-// at or past it raise ERR_BOUNDS and are not written.
+// exclusive scans of the unit counts.  written = c0 + c1; positions at or past
+// it raise ERR_BOUNDS and are not written.  With check (packed keys only) the
+// pass also validates the input.
 void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys, u64 m,
                    u32 wshift, u32 q0, u32 q1, const u32* off0, const u32* off1, u64 base1,
-                   u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err);
+                   u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err,
+                   const InputCheck* check = nullptr);
 
 }  // namespace kmst

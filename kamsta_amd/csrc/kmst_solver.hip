@@ -198,6 +198,12 @@ class Solver {
   // validates the input; for filter-Boruvka also marks vertices with records
   // and their record ranges and, with plan_classes, counts the weight classes
   void setup(bool filter, bool plan_classes);
+  // packed keys when bits(max weight) + 2 bits(n) <= 64 (from the counters)
+  void set_key_format();
+  // errors and the symmetry test of a finished input check
+  void check_input();
+  // set by setup when the class scatter validates the input
+  bool check_in_scatter = false;
   void plan_pivots();
   void filter_classes();
   void check_errors(const char* where);
@@ -290,33 +296,57 @@ void Solver::check_errors(const char* where) {
 void Solver::setup(bool filter, bool plan_classes) {
   fill_u32(st, ctr, 64, 0);
   pivots = WeightPivots{};
+  check_in_scatter = false;
   if (plan_classes) plan_pivots();
   // init_parent_array (hybrid_boruvka_edgefilter.hpp:332-352): vertices with
   // records, marked during validation
   if (filter) fill_u32(st, seg_end, n0, 0);
-  // with weight classes, the counts of the two lightest per 256-record unit
-  // go to X.s / X.d (free until the class scatter)
-  const u32 q0 = pivots.K ? pivots.q[0] : 0, q1 = pivots.K >= 2 ? pivots.q[1] : q0;
-  const bool units = validate_records(
-      st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW,
-      reinterpret_cast<u64*>(ctr + C_HSUM64), pivots, reinterpret_cast<u64*>(ctr + C_CLASS64),
-      filter ? seg_start : nullptr, filter ? seg_end : nullptr, q0, q1,
-      plan_classes ? X.s : nullptr, plan_classes ? X.d : nullptr);
-  if (plan_classes && !units) unit_class_counts(st, in_w, m0, q0, q1, X.s, X.d);
-  check_errors("input");
-  if (u64(C.host_counters[C_FWD]) * 2 != m0 || counter64(C_HSUM64) != counter64(C_HSUM64 + 2))
-    throw Error(KMST_EASYMMETRIC, "input: records are not symmetric (src,dst,w) <-> (dst,src,w)");
   vbits = bits_for(n0);
-  C.stats.max_weight = C.host_counters[C_MAXW];
-  const u32 wbits = bits_for(u64(C.host_counters[C_MAXW]) + 1);
-  packed = wbits + 2 * vbits <= 64;
-  C.stats.packed_keys = packed;
+  u64* class64 = reinterpret_cast<u64*>(ctr + C_CLASS64);
+  u64* hsum = reinterpret_cast<u64*>(ctr + C_HSUM64);
+  if (plan_classes && pivots.K > 0) {
+    // weights-only pass: the largest weight (the key format), the class
+    // counts and the per-unit counts of the two lightest classes (into X.s /
+    // X.d, free until the class scatter)
+    const u32 q0 = pivots.q[0], q1 = pivots.K >= 2 ? pivots.q[1] : q0;
+    weight_classes(st, in_w, m0, pivots, q0, q1, class64, ctr + C_MAXW, X.s, X.d);
+    read_counters();
+    set_key_format();
+    if (packed) {
+      // the class scatter reads every record anyway: it checks the input too
+      check_in_scatter = true;
+      return;
+    }
+    // positional keys are built from the checked input: validate first
+    // (no class counting, which the weights pass did)
+    validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW, hsum,
+                     WeightPivots{}, class64, seg_start, seg_end);
+    check_input();
+  } else {
+    validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW, hsum,
+                     pivots, class64, filter ? seg_start : nullptr, filter ? seg_end : nullptr);
+    check_input();
+    set_key_format();
+  }
   if (!packed) {
     val0 = static_cast<u64*>(C.val0.get(m0 * 8));
     build_offsets(st, in_s, m0, n0, label /* reused as offsets, n0+1 */);
     build_keys(st, in_s, in_d, in_w, m0, label, val0, ctr + C_ERR);
     check_errors("input");
   }
+}
+
+void Solver::set_key_format() {
+  C.stats.max_weight = C.host_counters[C_MAXW];
+  const u32 wbits = bits_for(u64(C.host_counters[C_MAXW]) + 1);
+  packed = wbits + 2 * vbits <= 64;
+  C.stats.packed_keys = packed;
+}
+
+void Solver::check_input() {
+  check_errors("input");
+  if (u64(C.host_counters[C_FWD]) * 2 != m0 || counter64(C_HSUM64) != counter64(C_HSUM64 + 2))
+    throw Error(KMST_EASYMMETRIC, "input: records are not symmetric (src,dst,w) <-> (dst,src,w)");
 }
 
 RecView Solver::sort_by_src(RecView cur, RecView alt, u32 bits) {
@@ -472,9 +502,18 @@ void Solver::filter_classes() {
   const u64 units = record_units(m0);
   scan_exclusive_u32(st, X.s, X.s, units, scan_scratch, nullptr);
   scan_exclusive_u32(st, X.d, X.d, units, scan_scratch, nullptr);
+  InputCheck chk;
+  chk.n = n0;
+  chk.fwd_count = ctr + C_FWD;
+  chk.hsum = reinterpret_cast<u64*>(ctr + C_HSUM64);
+  chk.seg_start = seg_start;
+  chk.seg_end = seg_end;
   class_scatter(st, in_s, in_d, keys_of(input), m0, wshift, q0, q1, X.s, X.d, cnt[0], written,
-                Y.s, Y.d, Y.v, ctr + C_ERR);
-  check_errors("split");
+                Y.s, Y.d, Y.v, ctr + C_ERR, check_in_scatter ? &chk : nullptr);
+  if (check_in_scatter)
+    check_input();
+  else
+    check_errors("split");
   const u64 base1 = cnt[0];
   // level l (0 = top) splits classes 0..K-l at pivot q[K-1-l]
   std::vector<FilterStat> level(K);
