@@ -1176,9 +1176,13 @@ __device__ __forceinline__ u64 sort_status(u64 flag, u32 epoch, u64 value) {
   return (flag << 62) | (u64(epoch) << 48) | (value & SORT_VAL);
 }
 
-// Serial look-back of one digit over the tiles before `tile`; returns the
-// number of records with this digit in them and publishes the inclusive
-// count.  Bounded like lookback_exclusive.
+// Look-back of one digit over the tiles before `tile`: reads SORT_LB_WIN
+// predecessors at a time (their loads in flight together), sums aggregates
+// back to the nearest inclusive prefix, returns the number of records with
+// this digit in those tiles and publishes the inclusive count.  Bounded like
+// lookback_exclusive.
+constexpr int SORT_LB_WIN = 4;
+
 __device__ __forceinline__ u64 digit_lookback(u64* status, u32 epoch, u32 tile, u32 digit,
                                               u64 count, u32* err, bool& ok) {
   ok = true;
@@ -1189,41 +1193,59 @@ __device__ __forceinline__ u64 digit_lookback(u64* status, u32 epoch, u32 tile, 
   }
   lb_store(me, sort_status(1, epoch, count));
   u64 excl = 0;
-  i64 t = static_cast<i64>(tile) - 1;
+  i64 t = static_cast<i64>(tile) - 1;  // next tile to read
   u32 spins = 0;
-  while (t >= 0) {
-    const u64 w = lb_load(status + u64(t) * RADIX + digit);
-    const bool current = ((w >> 48) & ((1u << SORT_EPOCH_BITS) - 1)) == epoch;
-    const u64 flag = current ? (w >> 62) : 0;
-    if (flag == 0) {
+  while (true) {
+    u64 win[SORT_LB_WIN];
+#pragma unroll
+    for (int j = 0; j < SORT_LB_WIN; ++j)
+      win[j] = (t - j >= 0) ? lb_load(status + u64(t - j) * RADIX + digit)
+                            : sort_status(2, epoch, 0);  // before tile 0: an empty prefix
+    int used = 0;
+    bool done = false;
+#pragma unroll
+    for (int j = 0; j < SORT_LB_WIN; ++j) {
+      if (done || used < j) continue;  // stop at the first inclusive or unpublished word
+      const u64 w = win[j];
+      const bool current = ((w >> 48) & ((1u << SORT_EPOCH_BITS) - 1)) == epoch;
+      const u64 flag = current ? (w >> 62) : 0;
+      if (flag == 0) continue;
+      excl += w & SORT_VAL;
+      ++used;
+      done = flag == 2;
+    }
+    if (done) break;
+    t -= used;
+    if (used == 0) {
       if (++spins > LB_SPIN_LIMIT) {
         atomicOr(err, ERR_SPIN);
         ok = false;
         return 0;
       }
       __builtin_amdgcn_s_sleep(1);
-      continue;
     }
-    excl += w & SORT_VAL;
-    if (flag == 2) break;
-    --t;
   }
   lb_store(me, sort_status(2, epoch, excl + count));
   return excl;
 }
 
-template <bool HAS_D, bool HAS_V, bool DEDUP>
-__global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
+// THREADS = 256 or 512 threads per tile of THREADS * SORT_IPT records; the
+// first RADIX threads own one digit each for the scan and the look-back.
+template <int THREADS, bool HAS_D, bool HAS_V, bool DEDUP>
+__global__ __launch_bounds__(THREADS) void radix_onesweep_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in,
     u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out, u64 m, u32 dbits,
     u32 shift, const u32* __restrict__ digit_base, u64* status, u32 epoch,
     unsigned long long* tile_counter, unsigned long long tile_base_id, u32* err) {
-  __shared__ u32 wcnt[WAVES][RADIX];
+  constexpr int NW = THREADS / WAVE;
+  constexpr int TILE = THREADS * SORT_IPT;
+  static_assert(THREADS >= RADIX, "one thread per digit");
+  __shared__ u32 wcnt[NW][RADIX];
   __shared__ u32 delta[RADIX];  // global position - tile position, per digit
-  __shared__ u64 stage[SORT_TILE];
-  __shared__ u32 scan_tmp[WAVES];
+  __shared__ u64 stage[TILE];
+  __shared__ u32 scan_tmp[NW];
   __shared__ u32 s_ok;
-  for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&wcnt[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < NW * RADIX; i += THREADS) (&wcnt[0][0])[i] = 0;
   if (threadIdx.x == 0) s_ok = 1;
   // dynamic tile ids in dispatch order from a counter that only grows
   __shared__ u32 s_tile;
@@ -1231,7 +1253,7 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
   __syncthreads();
   const u32 tile = s_tile;
   const int lane = lane_id(), w = wave_id();
-  const u64 tile_base = u64(tile) * SORT_TILE;
+  const u64 tile_base = u64(tile) * TILE;
   const u64 base = tile_base + u64(w) * (WAVE * SORT_IPT);
   u32 ks[SORT_IPT], kd[SORT_IPT], dig[SORT_IPT], rank[SORT_IPT];
   u64 kv[SORT_IPT];
@@ -1265,30 +1287,36 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
   }
   __syncthreads();
   {
-    // thread t owns digit t: tile-local start, per-wave offsets, global base
+    // thread t < RADIX owns digit t: tile-local start, per-wave offsets,
+    // global base (the exclusive sum spans the first RADIX threads; the other
+    // waves add zeros after them)
     const u32 dg = threadIdx.x;
     u32 tot = 0;
+    if (dg < RADIX) {
 #pragma unroll
-    for (int ww = 0; ww < WAVES; ++ww) tot += wcnt[ww][dg];
+      for (int ww = 0; ww < NW; ++ww) tot += wcnt[ww][dg];
+    }
     u32 all;
     const u32 start = block_exclusive_sum(tot, all, scan_tmp);
-    u32 run = start;
+    if (dg < RADIX) {
+      u32 run = start;
 #pragma unroll
-    for (int ww = 0; ww < WAVES; ++ww) {
-      const u32 c = wcnt[ww][dg];
-      wcnt[ww][dg] = run;
-      run += c;
+      for (int ww = 0; ww < NW; ++ww) {
+        const u32 c = wcnt[ww][dg];
+        wcnt[ww][dg] = run;
+        run += c;
+      }
+      bool ok;
+      const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
+      delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
+      if (!ok) s_ok = 0;
     }
-    bool ok;
-    const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
-    delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
-    if (!ok) s_ok = 0;
   }
   __syncthreads();
   if (!s_ok) return;
-  const u32 count = static_cast<u32>((m - tile_base) < SORT_TILE ? (m - tile_base) : SORT_TILE);
+  const u32 count = static_cast<u32>((m - tile_base) < TILE ? (m - tile_base) : TILE);
   u32* st_s = reinterpret_cast<u32*>(stage);
-  u32* st_d = st_s + SORT_TILE;
+  u32* st_d = st_s + TILE;
 #pragma unroll
   for (int k = 0; k < SORT_IPT; ++k) {
     if (dig[k] < RADIX) {
@@ -1299,12 +1327,12 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
   }
   __syncthreads();
   // Position j of the sorted tile is handled by lane j % 64 of wave
-  // (j / 64) % WAVES, so each wave owns aligned groups of 64 positions.
+  // (j / 64) % NW, so each wave owns aligned groups of 64 positions.
   u32 pos[SORT_IPT];
   u64 heads[SORT_IPT];  // with DEDUP: lanes that start a run of equal (src, dst)
 #pragma unroll
   for (int r = 0; r < SORT_IPT; ++r) {
-    const u32 j = r * BLOCK + threadIdx.x;
+    const u32 j = r * THREADS + threadIdx.x;
     pos[r] = 0;
     bool head = true;
     if (j < count) {
@@ -1326,7 +1354,7 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_kernel(
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < SORT_IPT; ++r) {
-    const u32 j = r * BLOCK + threadIdx.x;
+    const u32 j = r * THREADS + threadIdx.x;
     u64 vj = (j < count) ? stage[j] : VAL_NONE;
     if (DEDUP) {
       // the head of each run keeps the run's lightest key, the others are
@@ -1365,21 +1393,32 @@ void SortWork::bind(hipStream_t st, u32* p, u64 m) {
   }
 }
 
+u32 sort_threads(const SortWork& w, u64 m) {
+  if (w.threads == 256 || w.threads == 512) return w.threads;
+  return m >= SORT_WIDE_MIN ? 512u : 256u;
+}
+
 template <bool HAS_D, bool HAS_V, bool DEDUP>
-static void launch_onesweep(hipStream_t st, u32 tiles, const u32* si, const u32* di,
+static void launch_onesweep(hipStream_t st, u32 threads, u32 tiles, const u32* si, const u32* di,
                             const u64* vi, u32* so, u32* dout, u64* vo, u64 m, u32 dbits, u32 shift,
                             const u32* base, SortWork& w, u32* err) {
   u64* status = reinterpret_cast<u64*>(w.words);
   auto* counter = reinterpret_cast<unsigned long long*>(status + w.tiles_cap * RADIX);
-  radix_onesweep_kernel<HAS_D, HAS_V, DEDUP><<<tiles, BLOCK, 0, st>>>(
-      si, di, vi, so, dout, vo, m, dbits, shift, base, status, w.epoch, counter, w.tiles_issued,
-      err);
+  if (threads == 512)
+    radix_onesweep_kernel<512, HAS_D, HAS_V, DEDUP><<<tiles, 512, 0, st>>>(
+        si, di, vi, so, dout, vo, m, dbits, shift, base, status, w.epoch, counter, w.tiles_issued,
+        err);
+  else
+    radix_onesweep_kernel<256, HAS_D, HAS_V, DEDUP><<<tiles, 256, 0, st>>>(
+        si, di, vi, so, dout, vo, m, dbits, shift, base, status, w.epoch, counter, w.tiles_issued,
+        err);
 }
 
 bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, u64* v_alt, u64 m,
                 u32 dbits, u32 total_bits, SortWork& work, u32* err, bool dedup_last) {
   if (m <= 1 || total_bits == 0) return false;
-  const u32 tiles = ceil_div_u32(m, SORT_TILE);
+  const u32 threads = sort_threads(work, m);
+  const u32 tiles = ceil_div_u32(m, u64(threads) * SORT_IPT);
   if (tiles > work.tiles_cap) throw Error(KMST_EINVAL, "sort workspace too small");
   const u32 passes = (total_bits + RADIX_BITS - 1) / RADIX_BITS;
   if (passes > MAX_PASSES) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
@@ -1415,20 +1454,20 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
     const u32* b = base + p * RADIX;
     const bool dd = can_dedup && p + 1 == passes;
     if (dd)
-      launch_onesweep<true, true, true>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
-                                        work, err);
+      launch_onesweep<true, true, true>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
+                                        shift, b, work, err);
     else if (d && v)
-      launch_onesweep<true, true, false>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
-                                         work, err);
+      launch_onesweep<true, true, false>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
+                                         shift, b, work, err);
     else if (d)
-      launch_onesweep<true, false, false>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
-                                          work, err);
+      launch_onesweep<true, false, false>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
+                                          shift, b, work, err);
     else if (v)
-      launch_onesweep<false, true, false>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
-                                          work, err);
+      launch_onesweep<false, true, false>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
+                                          shift, b, work, err);
     else
-      launch_onesweep<false, false, false>(st, tiles, si, di, vi, so, dout, vo, m, dbits, shift, b,
-                                           work, err);
+      launch_onesweep<false, false, false>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
+                                           shift, b, work, err);
     KMST_HIP(hipGetLastError());
     work.tiles_issued += tiles;
     alt = !alt;
@@ -1467,17 +1506,21 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_u64_kernel(const u64* __r
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void radix_onesweep_u64_kernel(
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void radix_onesweep_u64_kernel(
     const u64* __restrict__ v_in, u64* __restrict__ v_out, u64 m, u64 mask, u32 shift,
     const u32* __restrict__ digit_base, u64* status, u32 epoch, unsigned long long* tile_counter,
     unsigned long long tile_base_id, u32* err) {
-  __shared__ u32 wcnt[WAVES][RADIX];
+  constexpr int NW = THREADS / WAVE;
+  constexpr int TILE = THREADS * SORT_IPT;
+  static_assert(THREADS >= RADIX, "one thread per digit");
+  __shared__ u32 wcnt[NW][RADIX];
   __shared__ u32 delta[RADIX];
-  __shared__ u64 stage[SORT_TILE];
-  __shared__ u32 scan_tmp[WAVES];
+  __shared__ u64 stage[TILE];
+  __shared__ u32 scan_tmp[NW];
   __shared__ u32 s_ok;
   __shared__ u32 s_tile;
-  for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&wcnt[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < NW * RADIX; i += THREADS) (&wcnt[0][0])[i] = 0;
   if (threadIdx.x == 0) {
     s_ok = 1;
     s_tile = static_cast<u32>(atomicAdd(tile_counter, 1ull) - tile_base_id);
@@ -1485,7 +1528,7 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_u64_kernel(
   __syncthreads();
   const u32 tile = s_tile;
   const int lane = lane_id(), w = wave_id();
-  const u64 tile_base = u64(tile) * SORT_TILE;
+  const u64 tile_base = u64(tile) * TILE;
   const u64 base = tile_base + u64(w) * (WAVE * SORT_IPT);
   u64 kv[SORT_IPT];
   u32 dig[SORT_IPT], rank[SORT_IPT];
@@ -1515,21 +1558,25 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_u64_kernel(
   {
     const u32 dg = threadIdx.x;
     u32 tot = 0;
+    if (dg < RADIX) {
 #pragma unroll
-    for (int ww = 0; ww < WAVES; ++ww) tot += wcnt[ww][dg];
+      for (int ww = 0; ww < NW; ++ww) tot += wcnt[ww][dg];
+    }
     u32 all;
     const u32 start = block_exclusive_sum(tot, all, scan_tmp);
-    u32 run = start;
+    if (dg < RADIX) {
+      u32 run = start;
 #pragma unroll
-    for (int ww = 0; ww < WAVES; ++ww) {
-      const u32 c = wcnt[ww][dg];
-      wcnt[ww][dg] = run;
-      run += c;
+      for (int ww = 0; ww < NW; ++ww) {
+        const u32 c = wcnt[ww][dg];
+        wcnt[ww][dg] = run;
+        run += c;
+      }
+      bool ok;
+      const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
+      delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
+      if (!ok) s_ok = 0;
     }
-    bool ok;
-    const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
-    delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
-    if (!ok) s_ok = 0;
   }
   __syncthreads();
   if (!s_ok) return;
@@ -1537,8 +1584,8 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_u64_kernel(
   for (int k = 0; k < SORT_IPT; ++k)
     if (dig[k] < RADIX) stage[wcnt[w][dig[k]] + rank[k]] = kv[k];
   __syncthreads();
-  const u32 count = static_cast<u32>((m - tile_base) < SORT_TILE ? (m - tile_base) : SORT_TILE);
-  for (u32 j = threadIdx.x; j < count; j += BLOCK) {
+  const u32 count = static_cast<u32>((m - tile_base) < TILE ? (m - tile_base) : TILE);
+  for (u32 j = threadIdx.x; j < count; j += THREADS) {
     const u64 x = stage[j];
     v_out[delta[static_cast<u32>(((x & mask) >> shift) & (RADIX - 1))] + j] = x;
   }
@@ -1547,7 +1594,8 @@ __global__ __launch_bounds__(BLOCK) void radix_onesweep_u64_kernel(
 bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWork& work,
                     u32* err) {
   if (m <= 1 || bits == 0) return false;
-  const u32 tiles = ceil_div_u32(m, SORT_TILE);
+  const u32 threads = sort_threads(work, m);
+  const u32 tiles = ceil_div_u32(m, u64(threads) * SORT_IPT);
   if (tiles > work.tiles_cap) throw Error(KMST_EINVAL, "sort workspace too small");
   const u32 passes = (bits + RADIX_BITS - 1) / RADIX_BITS;
   if (passes > MAX_PASSES) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
@@ -1572,10 +1620,14 @@ bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWor
       work.epoch = 1;
     }
     KTScope t(st, K_SORT_SCATTER, 16.0 * m);
-    radix_onesweep_u64_kernel<<<tiles, BLOCK, 0, st>>>(alt ? v_alt : v, alt ? v : v_alt, m, mask,
-                                                       p * RADIX_BITS, base + p * RADIX, status,
-                                                       work.epoch, counter, work.tiles_issued,
-                                                       err);
+    if (threads == 512)
+      radix_onesweep_u64_kernel<512><<<tiles, 512, 0, st>>>(
+          alt ? v_alt : v, alt ? v : v_alt, m, mask, p * RADIX_BITS, base + p * RADIX, status,
+          work.epoch, counter, work.tiles_issued, err);
+    else
+      radix_onesweep_u64_kernel<256><<<tiles, 256, 0, st>>>(
+          alt ? v_alt : v, alt ? v : v_alt, m, mask, p * RADIX_BITS, base + p * RADIX, status,
+          work.epoch, counter, work.tiles_issued, err);
     KMST_HIP(hipGetLastError());
     work.tiles_issued += tiles;
     alt = !alt;

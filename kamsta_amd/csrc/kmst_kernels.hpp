@@ -132,8 +132,13 @@ struct SortWork {
   u64 tiles_cap = 0;     // tiles the status area holds
   u32 epoch = 0;         // epoch of the last pass
   u64 tiles_issued = 0;  // value of the tile counter
+  // threads per scatter tile (256 or 512; tiles of threads * SORT_IPT
+  // records), 0 = by size (512 from SORT_WIDE_MIN records on)
+  u32 threads = 0;
   void bind(hipStream_t st, u32* p, u64 m);
 };
+constexpr u64 SORT_WIDE_MIN = u64{1} << 20;
+u32 sort_threads(const SortWork& w, u64 m);
 // With dedup_last (needs d, v and dbits > 0) the last pass also keeps only
 // the lightest record of each run of equal (s, d) inside an aligned group of
 // 64 positions; the rest are marked dead.
