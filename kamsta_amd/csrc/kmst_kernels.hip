@@ -37,22 +37,27 @@ void fill_u32(hipStream_t st, u32* p, u64 n, u32 value) {
 
 // ============================================================== scan
 // Three-phase exclusive scan: per-tile reduce, scan of tile sums (recursive),
-// per-tile downsweep.  Each thread owns SCAN_IPT consecutive elements.
+// per-tile downsweep.  Loads and stores are striped (element k * BLOCK + t of
+// a tile for thread t, coalesced); the downsweep turns them into runs of
+// SCAN_IPT consecutive elements per thread through LDS, padded by one word
+// every SCAN_IPT elements so that both orders are free of bank conflicts.
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(const T* __restrict__ in, u64 n,
                                                             T* __restrict__ partial) {
   __shared__ T lds[WAVES];
-  const u64 base = u64(blockIdx.x) * SCAN_TILE + u64(threadIdx.x) * SCAN_IPT;
+  const u64 base = u64(blockIdx.x) * SCAN_TILE + threadIdx.x;
   T sum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_IPT; ++k) {
-    const u64 i = base + k;
+    const u64 i = base + u64(k) * BLOCK;
     sum += (i < n) ? in[i] : T(0);
   }
   T total;
   block_exclusive_sum(sum, total, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
+
+__device__ __forceinline__ u32 scan_pad(u32 i) { return i + i / SCAN_IPT; }
 
 // With flagged_low, element i also gets flagged_low[i] = low 32 bits of its
 // exclusive sum when bit 0 of in[i] is set, NONE32 otherwise (the new id of
@@ -63,27 +68,41 @@ __global__ __launch_bounds__(BLOCK) void scan_down_kernel(const T* in, T* out, u
                                                           const T* __restrict__ partial,
                                                           T* total_dev, u32* flagged_low) {
   __shared__ T lds[WAVES];
-  const u64 base = u64(blockIdx.x) * SCAN_TILE + u64(threadIdx.x) * SCAN_IPT;
+  __shared__ T tile[SCAN_TILE + SCAN_TILE / SCAN_IPT];
+  const u64 base = u64(blockIdx.x) * SCAN_TILE + threadIdx.x;
   T v[SCAN_IPT];
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; ++k) {
+    const u64 i = base + u64(k) * BLOCK;
+    v[k] = (i < n) ? in[i] : T(0);
+    tile[scan_pad(k * BLOCK + threadIdx.x)] = v[k];
+  }
+  __syncthreads();
+  T b[SCAN_IPT];
   T sum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_IPT; ++k) {
-    const u64 i = base + k;
-    v[k] = (i < n) ? in[i] : T(0);
-    sum += v[k];
+    b[k] = tile[scan_pad(threadIdx.x * SCAN_IPT + k)];
+    sum += b[k];
   }
   T total;
-  T run = block_exclusive_sum(sum, total, lds);
+  T run = block_exclusive_sum(sum, total, lds);  // also orders the reads above before the writes
   const T tile_off = partial ? partial[blockIdx.x] : T(0);
   run += tile_off;
 #pragma unroll
   for (int k = 0; k < SCAN_IPT; ++k) {
-    const u64 i = base + k;
+    tile[scan_pad(threadIdx.x * SCAN_IPT + k)] = run;
+    run += b[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SCAN_IPT; ++k) {
+    const u64 i = base + u64(k) * BLOCK;
     if (i < n) {
-      out[i] = run;
-      if (flagged_low) flagged_low[i] = (v[k] & 1) ? static_cast<u32>(run) : NONE32;
+      const T o = tile[scan_pad(k * BLOCK + threadIdx.x)];
+      out[i] = o;
+      if (flagged_low) flagged_low[i] = (v[k] & 1) ? static_cast<u32>(o) : NONE32;
     }
-    run += v[k];
   }
   if (total_dev && partial == nullptr && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1)
     *total_dev = tile_off + total;
