@@ -1114,7 +1114,6 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
 // digit matching, so a pass is stable; the tile is staged in LDS in (digit,
 // input order) and leaves in runs of consecutive addresses: (src, dst) first,
 // then the keys through the same LDS buffer.  No host synchronisation.
-constexpr u32 MAX_PASSES = 8;
 static_assert(RADIX == BLOCK, "one thread per digit");
 
 // Sort key: (s << dbits) | d, or s alone when dbits is 0 (d, if present, is
@@ -1124,6 +1123,29 @@ __device__ __forceinline__ u64 sort_key_of(u32 s, u32 d, u32 dbits) {
   return (HAS_D && dbits) ? ((u64(s) << dbits) | d) : u64(s);
 }
 
+// Digit of pass p: width[p] key bits from bit shift[p] (widths of at most
+// RADIX_BITS).  With balanced digits a key of b bits takes ceil(b / 8) passes
+// of nearly equal widths (20 bits: 7 + 7 + 6), so narrower keys get fewer
+// buckets per pass and longer runs per bucket in every scatter tile.
+DigitPlan plan_digits(u32 total_bits, bool balanced) {
+  DigitPlan pl{};
+  pl.passes = (total_bits + RADIX_BITS - 1) / RADIX_BITS;
+  u32 shift = 0;
+  for (u32 p = 0; p < pl.passes; ++p) {
+    u32 w = RADIX_BITS;
+    if (balanced) w = total_bits / pl.passes + (p < total_bits % pl.passes ? 1u : 0u);
+    else if (shift + w > total_bits) w = total_bits - shift;
+    pl.shift[p] = static_cast<u8>(shift);
+    pl.width[p] = static_cast<u8>(w);
+    shift += w;
+  }
+  return pl;
+}
+
+__device__ __forceinline__ u32 plan_digit(u64 key, const DigitPlan& pl, u32 p) {
+  return static_cast<u32>((key >> pl.shift[p]) & ((1u << pl.width[p]) - 1));
+}
+
 // ghist[p * RADIX + x] = number of records whose digit p is x.  Per-wave
 // LDS histograms; each thread loads UP_RPT keys before counting them.
 constexpr int UP_RPT = 8;
@@ -1131,8 +1153,9 @@ constexpr int UP_RPT = 8;
 template <bool HAS_D>
 __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const u32* __restrict__ s,
                                                               const u32* __restrict__ d, u64 m,
-                                                              u32 dbits, u32 passes,
+                                                              u32 dbits, const DigitPlan pl,
                                                               u32* __restrict__ ghist) {
+  const u32 passes = pl.passes;
   __shared__ u32 h[WAVES][MAX_PASSES][RADIX];
   for (int i = threadIdx.x; i < WAVES * MAX_PASSES * RADIX; i += BLOCK) (&h[0][0][0])[i] = 0;
   __syncthreads();
@@ -1150,7 +1173,7 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const u32* __restr
       const bool valid = base + u64(k) * BLOCK + threadIdx.x < m;
       const u64 vmask = __ballot(valid);
       for (u32 p = 0; p < passes; ++p) {
-        const u32 digit = static_cast<u32>((key[k] >> (p * RADIX_BITS)) & (RADIX - 1));
+        const u32 digit = plan_digit(key[k], pl, p);
         // a wave whose records share the digit adds once (the usual case for
         // the high digits); lane 0 is valid whenever any lane is
         const u32 first = __builtin_amdgcn_readfirstlane(digit);
@@ -1254,7 +1277,7 @@ template <int THREADS, bool HAS_D, bool HAS_V, bool DEDUP>
 __global__ __launch_bounds__(THREADS) void radix_onesweep_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const u64* __restrict__ v_in,
     u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out, u64 m, u32 dbits,
-    u32 shift, const u32* __restrict__ digit_base, u64* status, u32 epoch,
+    u32 shift, u32 mask, const u32* __restrict__ digit_base, u64* status, u32 epoch,
     unsigned long long* tile_counter, unsigned long long tile_base_id, u32* err) {
   constexpr int NW = THREADS / WAVE;
   constexpr int TILE = THREADS * SORT_IPT;
@@ -1287,7 +1310,7 @@ __global__ __launch_bounds__(THREADS) void radix_onesweep_kernel(
   for (int k = 0; k < SORT_IPT; ++k) {
     const bool valid = base + u64(k) * WAVE + lane < m;
     const u64 key = sort_key_of<HAS_D>(ks[k], kd[k], dbits);
-    const u32 digit = valid ? static_cast<u32>((key >> shift) & (RADIX - 1)) : u32(RADIX);
+    const u32 digit = valid ? static_cast<u32>((key >> shift) & mask) : u32(RADIX);
     u64 peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < RADIX_BITS; ++b) {
@@ -1325,10 +1348,12 @@ __global__ __launch_bounds__(THREADS) void radix_onesweep_kernel(
         wcnt[ww][dg] = run;
         run += c;
       }
-      bool ok;
-      const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
-      delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
-      if (!ok) s_ok = 0;
+      if (dg <= mask) {  // digits beyond the pass's width hold no records
+        bool ok;
+        const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
+        delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
+        if (!ok) s_ok = 0;
+      }
     }
   }
   __syncthreads();
@@ -1357,7 +1382,7 @@ __global__ __launch_bounds__(THREADS) void radix_onesweep_kernel(
     if (j < count) {
       const u32 sj = st_s[j];
       const u32 dj = HAS_D ? st_d[j] : 0u;
-      const u32 digit = static_cast<u32>((sort_key_of<HAS_D>(sj, dj, dbits) >> shift) & (RADIX - 1));
+      const u32 digit = static_cast<u32>((sort_key_of<HAS_D>(sj, dj, dbits) >> shift) & mask);
       pos[r] = delta[digit] + j;
       s_out[pos[r]] = sj;
       if (HAS_D) d_out[pos[r]] = dj;
@@ -1420,17 +1445,17 @@ u32 sort_threads(const SortWork& w, u64 m) {
 template <bool HAS_D, bool HAS_V, bool DEDUP>
 static void launch_onesweep(hipStream_t st, u32 threads, u32 tiles, const u32* si, const u32* di,
                             const u64* vi, u32* so, u32* dout, u64* vo, u64 m, u32 dbits, u32 shift,
-                            const u32* base, SortWork& w, u32* err) {
+                            u32 mask, const u32* base, SortWork& w, u32* err) {
   u64* status = reinterpret_cast<u64*>(w.words);
   auto* counter = reinterpret_cast<unsigned long long*>(status + w.tiles_cap * RADIX);
   if (threads == 512)
     radix_onesweep_kernel<512, HAS_D, HAS_V, DEDUP><<<tiles, 512, 0, st>>>(
-        si, di, vi, so, dout, vo, m, dbits, shift, base, status, w.epoch, counter, w.tiles_issued,
-        err);
+        si, di, vi, so, dout, vo, m, dbits, shift, mask, base, status, w.epoch, counter,
+        w.tiles_issued, err);
   else
     radix_onesweep_kernel<256, HAS_D, HAS_V, DEDUP><<<tiles, 256, 0, st>>>(
-        si, di, vi, so, dout, vo, m, dbits, shift, base, status, w.epoch, counter, w.tiles_issued,
-        err);
+        si, di, vi, so, dout, vo, m, dbits, shift, mask, base, status, w.epoch, counter,
+        w.tiles_issued, err);
 }
 
 bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, u64* v_alt, u64 m,
@@ -1439,8 +1464,9 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
   const u32 threads = sort_threads(work, m);
   const u32 tiles = ceil_div_u32(m, u64(threads) * SORT_IPT);
   if (tiles > work.tiles_cap) throw Error(KMST_EINVAL, "sort workspace too small");
-  const u32 passes = (total_bits + RADIX_BITS - 1) / RADIX_BITS;
-  if (passes > MAX_PASSES) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
+  if (total_bits > 64) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
+  const DigitPlan pl = plan_digits(total_bits, work.balanced);
+  const u32 passes = pl.passes;
   u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);
   u32* base = ghist + MAX_PASSES * RADIX;
   fill_u32(st, ghist, u64(passes) * RADIX, 0);
@@ -1448,8 +1474,8 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
     KTScope t(st, K_SORT_HIST, (d ? 8.0 : 4.0) * m);
     // few, fat blocks: every block adds its histograms to the same global bins
     const u32 grid = grid_for((m + UP_RPT - 1) / UP_RPT, 512);
-    if (d) radix_upsweep_kernel<true><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, passes, ghist);
-    else radix_upsweep_kernel<false><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, passes, ghist);
+    if (d) radix_upsweep_kernel<true><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, pl, ghist);
+    else radix_upsweep_kernel<false><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, pl, ghist);
     KMST_HIP(hipGetLastError());
     radix_digit_base_kernel<<<1, BLOCK, 0, st>>>(ghist, passes, base);
     KMST_HIP(hipGetLastError());
@@ -1469,24 +1495,24 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
       work.epoch = 1;
     }
     KTScope t(st, K_SORT_SCATTER, 2.0 * m * (4.0 + (d ? 4.0 : 0.0) + (v ? 8.0 : 0.0)));
-    const u32 shift = p * RADIX_BITS;
+    const u32 shift = pl.shift[p], mask = (1u << pl.width[p]) - 1;
     const u32* b = base + p * RADIX;
     const bool dd = can_dedup && p + 1 == passes;
     if (dd)
       launch_onesweep<true, true, true>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
-                                        shift, b, work, err);
+                                        shift, mask, b, work, err);
     else if (d && v)
       launch_onesweep<true, true, false>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
-                                         shift, b, work, err);
+                                         shift, mask, b, work, err);
     else if (d)
       launch_onesweep<true, false, false>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
-                                          shift, b, work, err);
+                                          shift, mask, b, work, err);
     else if (v)
       launch_onesweep<false, true, false>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
-                                          shift, b, work, err);
+                                          shift, mask, b, work, err);
     else
       launch_onesweep<false, false, false>(st, threads, tiles, si, di, vi, so, dout, vo, m, dbits,
-                                           shift, b, work, err);
+                                           shift, mask, b, work, err);
     KMST_HIP(hipGetLastError());
     work.tiles_issued += tiles;
     alt = !alt;
@@ -1497,8 +1523,10 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
 // The same sort for a single u64 array ordered by its low bits (the MST keys
 // by (min, max) at the end of a solve): 8-byte records instead of 16.
 __global__ __launch_bounds__(BLOCK) void radix_upsweep_u64_kernel(const u64* __restrict__ v,
-                                                                  u64 m, u64 mask, u32 passes,
+                                                                  u64 m, u64 mask,
+                                                                  const DigitPlan pl,
                                                                   u32* __restrict__ ghist) {
+  const u32 passes = pl.passes;
   __shared__ u32 h[WAVES][MAX_PASSES][RADIX];
   for (int i = threadIdx.x; i < WAVES * MAX_PASSES * RADIX; i += BLOCK) (&h[0][0][0])[i] = 0;
   __syncthreads();
@@ -1513,7 +1541,7 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_u64_kernel(const u64* __r
     for (int k = 0; k < UP_RPT; ++k) {
       if (base + u64(k) * BLOCK + threadIdx.x >= m) continue;
       for (u32 p = 0; p < passes; ++p)
-        atomicAdd(&h[w][p][static_cast<u32>((key[k] >> (p * RADIX_BITS)) & (RADIX - 1))], 1u);
+        atomicAdd(&h[w][p][plan_digit(key[k], pl, p)], 1u);
     }
   }
   __syncthreads();
@@ -1528,8 +1556,8 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_u64_kernel(const u64* __r
 template <int THREADS>
 __global__ __launch_bounds__(THREADS) void radix_onesweep_u64_kernel(
     const u64* __restrict__ v_in, u64* __restrict__ v_out, u64 m, u64 mask, u32 shift,
-    const u32* __restrict__ digit_base, u64* status, u32 epoch, unsigned long long* tile_counter,
-    unsigned long long tile_base_id, u32* err) {
+    u32 dmask, const u32* __restrict__ digit_base, u64* status, u32 epoch,
+    unsigned long long* tile_counter, unsigned long long tile_base_id, u32* err) {
   constexpr int NW = THREADS / WAVE;
   constexpr int TILE = THREADS * SORT_IPT;
   static_assert(THREADS >= RADIX, "one thread per digit");
@@ -1556,7 +1584,7 @@ __global__ __launch_bounds__(THREADS) void radix_onesweep_u64_kernel(
 #pragma unroll
   for (int k = 0; k < SORT_IPT; ++k) {
     const bool valid = base + u64(k) * WAVE + lane < m;
-    const u32 digit = valid ? static_cast<u32>(((kv[k] & mask) >> shift) & (RADIX - 1)) : u32(RADIX);
+    const u32 digit = valid ? static_cast<u32>(((kv[k] & mask) >> shift) & dmask) : u32(RADIX);
     u64 peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < RADIX_BITS; ++b) {
@@ -1591,10 +1619,12 @@ __global__ __launch_bounds__(THREADS) void radix_onesweep_u64_kernel(
         wcnt[ww][dg] = run;
         run += c;
       }
-      bool ok;
-      const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
-      delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
-      if (!ok) s_ok = 0;
+      if (dg <= dmask) {  // digits beyond the pass's width hold no keys
+        bool ok;
+        const u64 before = digit_lookback(status, epoch, tile, dg, tot, err, ok);
+        delta[dg] = digit_base[dg] + static_cast<u32>(before) - start;
+        if (!ok) s_ok = 0;
+      }
     }
   }
   __syncthreads();
@@ -1606,7 +1636,7 @@ __global__ __launch_bounds__(THREADS) void radix_onesweep_u64_kernel(
   const u32 count = static_cast<u32>((m - tile_base) < TILE ? (m - tile_base) : TILE);
   for (u32 j = threadIdx.x; j < count; j += THREADS) {
     const u64 x = stage[j];
-    v_out[delta[static_cast<u32>(((x & mask) >> shift) & (RADIX - 1))] + j] = x;
+    v_out[delta[static_cast<u32>(((x & mask) >> shift) & dmask)] + j] = x;
   }
 }
 
@@ -1616,8 +1646,9 @@ bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWor
   const u32 threads = sort_threads(work, m);
   const u32 tiles = ceil_div_u32(m, u64(threads) * SORT_IPT);
   if (tiles > work.tiles_cap) throw Error(KMST_EINVAL, "sort workspace too small");
-  const u32 passes = (bits + RADIX_BITS - 1) / RADIX_BITS;
-  if (passes > MAX_PASSES) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
+  if (bits > 64) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
+  const DigitPlan pl = plan_digits(bits, work.balanced);
+  const u32 passes = pl.passes;
   const u64 mask = bits >= 64 ? ~0ull : ((u64{1} << bits) - 1);  // bits above are not key
   u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);
   u32* base = ghist + MAX_PASSES * RADIX;
@@ -1625,7 +1656,7 @@ bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWor
   {
     KTScope t(st, K_SORT_HIST, 8.0 * m);
     radix_upsweep_u64_kernel<<<grid_for((m + UP_RPT - 1) / UP_RPT, 512), BLOCK, 0, st>>>(
-        v, m, mask, passes, ghist);
+        v, m, mask, pl, ghist);
     KMST_HIP(hipGetLastError());
     radix_digit_base_kernel<<<1, BLOCK, 0, st>>>(ghist, passes, base);
     KMST_HIP(hipGetLastError());
@@ -1639,13 +1670,14 @@ bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWor
       work.epoch = 1;
     }
     KTScope t(st, K_SORT_SCATTER, 16.0 * m);
+    const u32 shift = pl.shift[p], dmask = (1u << pl.width[p]) - 1;
     if (threads == 512)
       radix_onesweep_u64_kernel<512><<<tiles, 512, 0, st>>>(
-          alt ? v_alt : v, alt ? v : v_alt, m, mask, p * RADIX_BITS, base + p * RADIX, status,
+          alt ? v_alt : v, alt ? v : v_alt, m, mask, shift, dmask, base + p * RADIX, status,
           work.epoch, counter, work.tiles_issued, err);
     else
       radix_onesweep_u64_kernel<256><<<tiles, 256, 0, st>>>(
-          alt ? v_alt : v, alt ? v : v_alt, m, mask, p * RADIX_BITS, base + p * RADIX, status,
+          alt ? v_alt : v, alt ? v : v_alt, m, mask, shift, dmask, base + p * RADIX, status,
           work.epoch, counter, work.tiles_issued, err);
     KMST_HIP(hipGetLastError());
     work.tiles_issued += tiles;

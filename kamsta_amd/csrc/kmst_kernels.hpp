@@ -135,10 +135,23 @@ struct SortWork {
   // threads per scatter tile (256 or 512; tiles of threads * SORT_IPT
   // records), 0 = by size (512 from SORT_WIDE_MIN records on)
   u32 threads = 0;
+  // digit widths: balanced over the ceil(bits / 8) passes, else 8 bits
+  // (the last pass takes the rest)
+  bool balanced = true;
   void bind(hipStream_t st, u32* p, u64 m);
 };
 constexpr u64 SORT_WIDE_MIN = u64{1} << 20;
 u32 sort_threads(const SortWork& w, u64 m);
+
+// Per-pass digits of a radix sort: pass p sorts by key bits
+// [shift[p], shift[p] + width[p]), width[p] <= RADIX_BITS.
+constexpr u32 MAX_PASSES = 8;
+struct DigitPlan {
+  u32 passes;
+  u8 shift[MAX_PASSES];
+  u8 width[MAX_PASSES];
+};
+DigitPlan plan_digits(u32 total_bits, bool balanced);
 // With dedup_last (needs d, v and dbits > 0) the last pass also keeps only
 // the lightest record of each run of equal (s, d) inside an aligned group of
 // 64 positions; the rest are marked dead.
