@@ -35,7 +35,11 @@ def parse():
     p.add_argument("--seed", type=int, default=None,
                    help="generator seed (default: gnm 2, rgg2d 3, rhg 4)")
     p.add_argument("--algorithm", choices=["boruvka", "filter_boruvka", "best"], default="best")
-    p.add_argument("--filter-threshold", type=int, default=4)
+    p.add_argument("--filter-threshold", type=int, default=4,
+                   help="starting point of the autotuner (or the threshold with --no-autotune)")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="skip kmst_autotune (filter threshold and sort launch shapes) before "
+                        "the timed region")
     p.add_argument("--cpu-log-n", type=int, default=23)
     p.add_argument("--cpu-log-m", type=int, default=27)
     p.add_argument("--no-cpu", action="store_true")
@@ -280,6 +284,21 @@ def main():
             print(json.dumps({"error": "plain and filter MSTs differ"}))
             sys.exit(1)
 
+    # autotuner: filter threshold and sort launch shapes, timed on this input
+    # (every candidate is checked to return the same MST)
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        res = ctx.autotune(s, d, w, N, n, algorithm=k.FILTER_BORUVKA, config=cfg, repeats=3)
+        cfg = k.AlgorithmConfig(filter_threshold=res["filter_threshold"])
+        tune = {"filter_threshold": res["filter_threshold"],
+                "sort_threads": res["tuning"].sort_threads,
+                "balanced_digits": res["tuning"].balanced_digits,
+                "sort_wide_min": res["tuning"].sort_wide_min,
+                "best_ms": round(res["best_ms"], 3), "candidates": res["candidates"],
+                "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"[bench] autotune: {tune}")
+
     def timed(algo, steps, warmup):
         """The timed region runs without per-launch instrumentation; a second
         pass over the same number of steps records HIP events around every
@@ -449,7 +468,8 @@ def main():
         "data": ("synthetic (device GNM generator, seed %d)" if args.graph == "gnm"
                  else "synthetic (host generator, seed %d)") % args.seed,
         "config": {"workload": workload,
-                   "algorithm": best, "filter_threshold": args.filter_threshold,
+                   "algorithm": best, "filter_threshold": cfg.filter_threshold,
+                   "autotune": tune,
                    "directed_records": N,
                    "mst_edges": cnt_p, "mst_weight": tw_p,
                    "ms_per_step": {a: round(results[a][0], 3) for a in results},

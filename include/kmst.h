@@ -116,6 +116,39 @@ kmst_status kmst_solve_device(kmst_context* ctx, const kmst_dev_edges* in,
  * hook; off by default).  Totals appear in kmst_context_stats_json. */
 void kmst_context_set_timing(kmst_context* ctx, int on);
 
+/* ---- Tuning (no counterpart in the reference, whose only knob on this path
+ * is AlgorithmConfig::filter_threshold). ------------------------------------
+ * Launch-shape choices of a context's solves.  They change speed only: every
+ * tuning returns the same MST. */
+typedef struct kmst_tuning {
+  uint32_t sort_threads;     /* threads per radix scatter tile (tiles of 16 records per
+                                thread): 256, 512, or 0 = by size (see sort_wide_min) */
+  uint32_t balanced_digits;  /* 1: a b-bit key sorts in ceil(b/8) passes of nearly equal
+                                digit widths; 0: 8-bit digits, the last pass takes the rest */
+  uint64_t sort_wide_min;    /* with sort_threads 0: sorts of at least this many records
+                                use 512-thread tiles */
+} kmst_tuning;
+
+/* The defaults (0, 1, 2^20). */
+void kmst_tuning_default(kmst_tuning* t);
+kmst_status kmst_context_set_tuning(kmst_context* ctx, const kmst_tuning* t);
+kmst_status kmst_context_get_tuning(const kmst_context* ctx, kmst_tuning* t);
+
+/* Autotuner: solves the device-resident input `in` with cfg->algorithm under
+ * candidate settings -- filter thresholds {2, 3, 4, 6} (filter-Boruvka only),
+ * then sort tile threads {by size, 256, 512}, then digit widths {balanced,
+ * 8-bit}, each stage keeping the best so far -- timing each candidate by the
+ * median of `repeats` solves (at least 1) after one warm-up solve.  Every
+ * candidate must return the same MST (count, total weight and an
+ * order-independent hash of the edges), else KMST_EDEVICE.  Leaves the
+ * fastest tuning set on the context and copied to *tuning, its filter
+ * threshold in cfg->filter_threshold, its median time (ms) in *best_ms and
+ * the number of candidates timed in *candidates.  Uses the context's own
+ * output staging; `in` must satisfy the kmst_solve_device contract. */
+kmst_status kmst_autotune(kmst_context* ctx, const kmst_dev_edges* in, kmst_config* cfg,
+                          int32_t repeats, kmst_tuning* tuning, double* best_ms,
+                          uint32_t* candidates);
+
 /* Thread-local message for the last non-OK status. */
 const char* kmst_last_error(void);
 

@@ -42,6 +42,7 @@ EXPORTED_SYMBOLS = [
     "kmst_config_default", "kmst_context_create", "kmst_context_destroy", "kmst_context_trim",
     "kmst_context_reserve",
     "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_context_set_timing",
+    "kmst_tuning_default", "kmst_context_set_tuning", "kmst_context_get_tuning", "kmst_autotune",
     "kmst_last_error", "kmst_context_stats_json", "kmst_gen_gnm_device", "kmst_gen_gnm_host",
     "kmst_k_segmented_argmin", "kmst_k_sort_pairs", "kmst_k_pointer_jump",
     "kmst_server_config_default", "kmst_server_create", "kmst_server_submit", "kmst_server_poll",
@@ -79,6 +80,27 @@ class _RequestResult(ctypes.Structure):
                 ("batch_requests", ctypes.c_uint64), ("submit_ns", ctypes.c_uint64),
                 ("admit_ns", ctypes.c_uint64), ("start_ns", ctypes.c_uint64),
                 ("solved_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64)]
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("sort_threads", ctypes.c_uint32), ("balanced_digits", ctypes.c_uint32),
+                ("sort_wide_min", ctypes.c_uint64)]
+
+
+@dataclasses.dataclass
+class Tuning:
+    """Mirror of kmst_tuning: launch shapes of a context's solves (speed only;
+    every tuning returns the same MST)."""
+    sort_threads: int = 0          # 256, 512, or 0 = by size
+    balanced_digits: int = 1       # digit widths balanced over the passes
+    sort_wide_min: int = 1 << 20   # with sort_threads 0: 512-thread tiles from here on
+
+    def _c(self):
+        return _Tuning(self.sort_threads, self.balanced_digits, self.sort_wide_min)
+
+    @staticmethod
+    def _from(t):
+        return Tuning(t.sort_threads, t.balanced_digits, t.sort_wide_min)
 
 
 class _DevMst(ctypes.Structure):
@@ -123,6 +145,11 @@ def load():
         "kmst_filter_boruvka": (i32, [vp, u64, vp, vp, pu64, pu64]),
         "kmst_solve_device": (i32, [vp, vp, vp, vp, pu64, pu64]),
         "kmst_context_set_timing": (None, [vp, i32]),
+        "kmst_tuning_default": (None, [vp]),
+        "kmst_context_set_tuning": (i32, [vp, vp]),
+        "kmst_context_get_tuning": (i32, [vp, vp]),
+        "kmst_autotune": (i32, [vp, vp, vp, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_uint32)]),
         "kmst_last_error": (ctypes.c_char_p, []),
         "kmst_context_stats_json": (ctypes.c_size_t, [vp, vp, ctypes.c_size_t]),
         "kmst_gen_gnm_device": (i32, [vp, u64, u64, u32, u64, vp, vp, vp]),
@@ -273,6 +300,30 @@ class Context:
                                            ctypes.byref(o), ctypes.byref(n_out),
                                            ctypes.byref(total)))
         return n_out.value, total.value
+
+    def set_tuning(self, tuning):
+        t = tuning._c()
+        _check(self._lib.kmst_context_set_tuning(self._h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = _Tuning()
+        _check(self._lib.kmst_context_get_tuning(self._h, ctypes.byref(t)))
+        return Tuning._from(t)
+
+    def autotune(self, src, dst, weight, m, n, algorithm=FILTER_BORUVKA, config=None, repeats=3):
+        """Times candidate filter thresholds and sort launch shapes on this
+        device-resident input (kmst_autotune); the fastest tuning stays set on
+        the context.  Returns {"tuning", "filter_threshold", "best_ms",
+        "candidates"}."""
+        cfg = (config or AlgorithmConfig(device=self.device))._c(algorithm)
+        e = _DevEdges(_ptr(src).value, _ptr(dst).value, _ptr(weight).value, m, n)
+        t = _Tuning()
+        best = ctypes.c_double(0.0)
+        cand = ctypes.c_uint32(0)
+        _check(self._lib.kmst_autotune(self._h, ctypes.byref(e), ctypes.byref(cfg), repeats,
+                                       ctypes.byref(t), ctypes.byref(best), ctypes.byref(cand)))
+        return {"tuning": Tuning._from(t), "filter_threshold": int(cfg.filter_threshold),
+                "best_ms": best.value, "candidates": int(cand.value)}
 
     def gen_gnm_device(self, n, m, max_weight, seed, src, dst, weight):
         _check(self._lib.kmst_gen_gnm_device(self._h, n, m, max_weight, seed, _ptr(src),

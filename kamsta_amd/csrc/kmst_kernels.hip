@@ -174,6 +174,28 @@ __device__ __forceinline__ u64 sym_hash(u32 a, u32 b, u32 w) {
   return x;
 }
 
+__global__ __launch_bounds__(BLOCK) void fingerprint_kernel(const u32* __restrict__ s,
+                                                          const u32* __restrict__ d,
+                                                          const u32* __restrict__ w, u64 count,
+                                                          u64* out) {
+  __shared__ u64 lds[WAVES];
+  u64 h = 0;
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < count; i += u64(gridDim.x) * BLOCK)
+    h += sym_hash(s[i], d[i], w[i]);
+  u64 total;
+  block_exclusive_sum(h, total, lds);
+  if (threadIdx.x == 0 && total)
+    atomicAdd(reinterpret_cast<unsigned long long*>(out), static_cast<unsigned long long>(total));
+}
+
+void edge_fingerprint(hipStream_t st, const u32* s, const u32* d, const u32* w, u64 count,
+                      u64* out) {
+  if (count == 0) return;
+  KTScope t(st, K_OTHER, 12.0 * count);
+  fingerprint_kernel<<<grid_for(count, 1024), BLOCK, 0, st>>>(s, d, w, count, out);
+  KMST_HIP(hipGetLastError());
+}
+
 // Weight class of w for ascending pivots q[0..K): the number of pivots below
 // w (0 = lightest class, K = heavier than every pivot).
 __device__ __forceinline__ u32 weight_class(u32 w, const WeightPivots& P) {
@@ -1439,7 +1461,7 @@ void SortWork::bind(hipStream_t st, u32* p, u64 m) {
 
 u32 sort_threads(const SortWork& w, u64 m) {
   if (w.threads == 256 || w.threads == 512) return w.threads;
-  return m >= SORT_WIDE_MIN ? 512u : 256u;
+  return m >= w.wide_min ? 512u : 256u;
 }
 
 template <bool HAS_D, bool HAS_V, bool DEDUP>

@@ -138,10 +138,16 @@ struct SortWork {
   // digit widths: balanced over the ceil(bits / 8) passes, else 8 bits
   // (the last pass takes the rest)
   bool balanced = true;
+  u64 wide_min = u64{1} << 20;  // with threads 0: 512-thread tiles from here on
   void bind(hipStream_t st, u32* p, u64 m);
 };
 constexpr u64 SORT_WIDE_MIN = u64{1} << 20;
 u32 sort_threads(const SortWork& w, u64 m);
+
+// *out += sum over i < count of a hash of (s[i], d[i], w[i]) (order-free
+// fingerprint of an MST).
+void edge_fingerprint(hipStream_t st, const u32* s, const u32* d, const u32* w, u64 count,
+                      u64* out);
 
 // Per-pass digits of a radix sort: pass p sorts by key bits
 // [shift[p], shift[p] + width[p]), width[p] <= RADIX_BITS.

@@ -128,10 +128,13 @@ std::vector<DevBuf*> kmst_context::all_bufs() {
           &rep,      &rep_next,   &gpar,    &seg_start, &seg_end, &mst,   &pieces,
           &tiles,    &scan_scratch, &hist,  &counters,  &lookback, &in_s, &in_d,
           &in_w,     &out_s,      &out_d,   &out_w,     &samples, &gen_a, &gen_b,
-          &gen_c,    &gen_d};
+          &gen_c,    &gen_d,      &tune};
 }
 
 namespace kmst {
+
+// The context's tuning onto its sort state (before every solve).
+void apply_tuning(kmst_context& C);
 
 // Device counter slots (u32 words; u64 values at even word offsets >= 16).
 enum Ctr : int {
@@ -269,11 +272,18 @@ void Solver::allocate() {
   lb = static_cast<u64*>(C.lookback.get(lookback_elems(m) * 8));
   const u64 sort_m = std::max(m, n);
   C.sort_work.bind(st, static_cast<u32*>(C.hist.get(sort_work_words(sort_m) * 4)), sort_m);
+  apply_tuning(C);
   const u64 scan_max = std::max<u64>({n, (m + CMP_TILE - 1) / CMP_TILE + 1, record_units(m)});
   // u64 scans reuse the same scratch, so size it in u64 units
   scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 8 + 64));
   ctr = static_cast<u32*>(C.counters.get(256));
   samples = static_cast<u32*>(C.samples.get(4096 * 4));
+}
+
+void apply_tuning(kmst_context& C) {
+  C.sort_work.threads = C.tuning.sort_threads;
+  C.sort_work.balanced = C.tuning.balanced_digits != 0;
+  C.sort_work.wide_min = C.tuning.sort_wide_min;
 }
 
 void Solver::read_counters() {
@@ -828,6 +838,100 @@ kmst_status host_solve(const kmst_wedge* in, uint64_t m, const kmst_config* cfg,
   });
 }
 
+struct TuneResult {
+  uint64_t n = 0, weight = 0, fingerprint = 0;
+  bool operator!=(const TuneResult& o) const {
+    return n != o.n || weight != o.weight || fingerprint != o.fingerprint;
+  }
+};
+
+// Coordinate search over the filter threshold and the sort launch shapes;
+// each candidate is checked against the first one's MST.
+void autotune_impl(kmst_context* ctx, const kmst_dev_edges* in, kmst_config* cfg,
+                   int32_t repeats, kmst_tuning* tuning, double* best_ms, uint32_t* candidates) {
+  check_config(cfg);
+  if (!ctx || !in || !tuning || !best_ms || !candidates)
+    throw kmst::Error(KMST_EINVAL, "null argument");
+  KMST_HIP(hipSetDevice(ctx->device));
+  const int reps = repeats < 1 ? 1 : repeats;
+  const uint64_t cap = in->n ? in->n : 1;
+  kmst_dev_mst out{static_cast<uint32_t*>(ctx->out_s.get(cap * 4)),
+                   static_cast<uint32_t*>(ctx->out_d.get(cap * 4)),
+                   static_cast<uint32_t*>(ctx->out_w.get(cap * 4)), cap};
+  auto* fp = static_cast<uint64_t*>(ctx->tune.get(64));
+  const kmst_tuning entry = ctx->tuning;
+  auto solve_once = [&](const kmst_config& c, const kmst_tuning& t, TuneResult* r) {
+    ctx->tuning = t;
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t on = 0, tw = 0;
+    solve_device_impl(ctx, in, &c, &out, &on, &tw);
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (r) {
+      kmst::fill_u64(ctx->stream, fp, 1, 0);
+      kmst::edge_fingerprint(ctx->stream, out.src, out.dst, out.weight, on, fp);
+      KMST_HIP(hipMemcpyAsync(&r->fingerprint, fp, 8, hipMemcpyDeviceToHost, ctx->stream));
+      KMST_HIP(hipStreamSynchronize(ctx->stream));
+      r->n = on;
+      r->weight = tw;
+    }
+    return ms;
+  };
+  TuneResult ref;
+  bool have_ref = false;
+  uint32_t count = 0;
+  auto measure = [&](const kmst_config& c, const kmst_tuning& t) {
+    TuneResult r;
+    solve_once(c, t, &r);  // warm-up and result check
+    if (!have_ref) {
+      ref = r;
+      have_ref = true;
+    } else if (r != ref) {
+      throw kmst::Error(KMST_EDEVICE, "autotune: candidates returned different MSTs");
+    }
+    std::vector<double> ts;
+    for (int i = 0; i < reps; ++i) ts.push_back(solve_once(c, t, nullptr));
+    std::sort(ts.begin(), ts.end());
+    ++count;
+    return ts[ts.size() / 2];
+  };
+  try {
+    kmst_config best_cfg = *cfg;
+    kmst_tuning best_t = entry;
+    double best = measure(best_cfg, best_t);
+    if (cfg->algorithm == KMST_ALGO_FILTER_BORUVKA) {
+      for (uint64_t thr : {2, 3, 4, 6}) {
+        if (thr == cfg->filter_threshold) continue;  // the entry point, timed above
+        kmst_config c = best_cfg;
+        c.filter_threshold = thr;
+        const double ms = measure(c, best_t);
+        if (ms < best) best = ms, best_cfg = c;
+      }
+    }
+    for (uint32_t threads : {0u, 256u, 512u}) {
+      if (threads == entry.sort_threads) continue;
+      kmst_tuning t = best_t;
+      t.sort_threads = threads;
+      const double ms = measure(best_cfg, t);
+      if (ms < best) best = ms, best_t = t;
+    }
+    {
+      kmst_tuning t = best_t;
+      t.balanced_digits = best_t.balanced_digits ? 0 : 1;
+      const double ms = measure(best_cfg, t);
+      if (ms < best) best = ms, best_t = t;
+    }
+    ctx->tuning = best_t;
+    *tuning = best_t;
+    cfg->filter_threshold = best_cfg.filter_threshold;
+    *best_ms = best;
+    *candidates = count;
+  } catch (...) {
+    ctx->tuning = entry;
+    throw;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -893,6 +997,37 @@ void kmst_context_set_timing(kmst_context* ctx, int on) {
   ctx->timer.reset();
 }
 
+void kmst_tuning_default(kmst_tuning* t) {
+  if (!t) return;
+  t->sort_threads = 0;
+  t->balanced_digits = 1;
+  t->sort_wide_min = kmst::SORT_WIDE_MIN;
+}
+
+kmst_status kmst_context_set_tuning(kmst_context* ctx, const kmst_tuning* t) {
+  return guarded([&] {
+    if (!ctx || !t) throw kmst::Error(KMST_EINVAL, "null argument");
+    if (t->sort_threads != 0 && t->sort_threads != 256 && t->sort_threads != 512)
+      throw kmst::Error(KMST_EINVAL, "sort_threads must be 0, 256 or 512");
+    if (t->balanced_digits > 1) throw kmst::Error(KMST_EINVAL, "balanced_digits must be 0 or 1");
+    if (t->sort_wide_min == 0) throw kmst::Error(KMST_EINVAL, "sort_wide_min must be positive");
+    ctx->tuning = *t;
+  });
+}
+
+kmst_status kmst_context_get_tuning(const kmst_context* ctx, kmst_tuning* t) {
+  return guarded([&] {
+    if (!ctx || !t) throw kmst::Error(KMST_EINVAL, "null argument");
+    *t = ctx->tuning;
+  });
+}
+
+kmst_status kmst_autotune(kmst_context* ctx, const kmst_dev_edges* in, kmst_config* cfg,
+                          int32_t repeats, kmst_tuning* tuning, double* best_ms,
+                          uint32_t* candidates) {
+  return guarded([&] { autotune_impl(ctx, in, cfg, repeats, tuning, best_ms, candidates); });
+}
+
 kmst_status kmst_gen_gnm_device(kmst_context* ctx, uint64_t n, uint64_t m, uint32_t max_weight,
                                 uint64_t seed, uint32_t* src, uint32_t* dst, uint32_t* weight) {
   return guarded([&] {
@@ -939,6 +1074,7 @@ kmst_status kmst_k_sort_pairs(kmst_context* ctx, uint32_t* src, uint32_t* dst, u
     auto* av = static_cast<uint64_t*>(ctx->xv.get(mm * 8));
     ctx->sort_work.bind(ctx->stream,
                         static_cast<uint32_t*>(ctx->hist.get(kmst::sort_work_words(mm) * 4)), mm);
+    kmst::apply_tuning(*ctx);
     auto* ctr = static_cast<uint32_t*>(ctx->counters.get(256));
     kmst::fill_u32(ctx->stream, ctr, 64, 0);
     if (kmst::radix_sort(ctx->stream, src, dst, val, as, ad, av, m, bits, 2 * bits,

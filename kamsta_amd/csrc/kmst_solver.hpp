@@ -69,10 +69,13 @@ struct kmst_context {
   // kernel scratch
   kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback;
   kmst::SortWork sort_work;  // radix sort state over `hist`
+  kmst_tuning tuning{0, 1, kmst::SORT_WIDE_MIN};  // launch shapes of later solves
   // host-API staging
   kmst::DevBuf in_s, in_d, in_w, out_s, out_d, out_w;
   // generator scratch
   kmst::DevBuf gen_a, gen_b, gen_c, gen_d;
+  // autotuner result fingerprint
+  kmst::DevBuf tune;
 
   std::vector<kmst::DevBuf*> all_bufs();
 };

@@ -115,3 +115,10 @@ def test_geometric_generators_contract(gen):
     for lp in (0, 1):
         b = oracle.boruvka(e, lp)
         assert b[1] == kr[1] and np.array_equal(b[0], kr[0])
+
+
+def test_tuning_default_matches_python_mirror():
+    t = k._Tuning()
+    k.load().kmst_tuning_default(ctypes.byref(t))
+    assert k.Tuning._from(t) == k.Tuning()
+    assert (t.sort_threads, t.balanced_digits, t.sort_wide_min) == (0, 1, 1 << 20)
