@@ -600,50 +600,115 @@ void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off) {
 // record (u, v) with u > v finds its twin by binary search in v's segment;
 // within runs of equal (src, dst) (multigraph inputs) the k-th record of
 // weight w pairs with the k-th twin of weight w.
+// With CHECK the same pass validates the input as validate_kernel does (the
+// twin search then runs on unchecked records, so its row bounds are clamped
+// to the arrays and records with ids out of range are skipped).
+template <bool CHECK>
 __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src,
                                                      const u32* __restrict__ dst,
                                                      const u32* __restrict__ w, u64 m,
                                                      const u32* __restrict__ off,
-                                                     u64* __restrict__ val, u32* err) {
-  const u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x;
-  if (i >= m) return;
-  const u32 s = src[i], d = dst[i], wi = w[i];
-  if (s < d) {
-    val[i] = (u64(wi) << 32) | i;
-    return;
-  }
-  // within runs of equal (src, dst) (multigraph inputs) the k-th record of
-  // weight w pairs with the k-th twin of weight w
-  u32 before = 0;
-  for (u64 j = i; j > 0; --j) {
-    if (src[j - 1] != s || dst[j - 1] != d) break;
-    before += (w[j - 1] == wi) ? 1u : 0u;
-  }
-  u64 lo = off[d], hi = off[d + 1];
-  const u64 end = hi;
-  while (lo < hi) {
-    const u64 mid = (lo + hi) >> 1;
-    if (dst[mid] < s) lo = mid + 1;
-    else hi = mid;
-  }
-  for (u64 j = lo; j < end && dst[j] == s; ++j) {
-    if (w[j] == wi) {
-      if (before == 0) {
-        val[i] = (u64(wi) << 32) | j;
-        return;
+                                                     u64* __restrict__ val, u32* err,
+                                                     const InputCheck chk) {
+  __shared__ u64 s_red[WAVES][3];
+  u32 bad = 0, fwd = 0;
+  u64 hf = 0, hr = 0;
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
+    const u32 s = src[i], d = dst[i], wi = w[i];
+    if (CHECK) {
+      const bool has_prev = i > 0;
+      const u32 ps = has_prev ? src[i - 1] : 0u, pd = has_prev ? dst[i - 1] : 0u;
+      if (s >= chk.n || d >= chk.n || s == d) bad |= ERR_INVALID;
+      if (has_prev && (ps > s || (ps == s && pd > d))) bad |= ERR_UNSORTED;
+      if (chk.seg_end && (!has_prev || ps != s)) {
+        // record i opens s's range and closes the previous vertex's
+        if (s < chk.n) chk.seg_start[s] = static_cast<u32>(i);
+        if (has_prev && ps < chk.n) chk.seg_end[ps] = static_cast<u32>(i);
       }
-      --before;
+      if (chk.seg_end && i + 1 == m && s < chk.n) chk.seg_end[s] = static_cast<u32>(m);
+      const bool f = s < d;
+      const u64 h = sym_hash(f ? s : d, f ? d : s, wi);
+      fwd += f ? 1u : 0u;
+      hf += f ? h : 0ull;
+      hr += f ? 0ull : h;
+      if (s >= chk.n || d >= chk.n) {
+        val[i] = VAL_NONE;
+        continue;
+      }
+    }
+    if (s < d) {
+      val[i] = (u64(wi) << 32) | i;
+      continue;
+    }
+    // within runs of equal (src, dst) (multigraph inputs) the k-th record of
+    // weight w pairs with the k-th twin of weight w
+    u32 before = 0;
+    for (u64 j = i; j > 0; --j) {
+      if (src[j - 1] != s || dst[j - 1] != d) break;
+      before += (w[j - 1] == wi) ? 1u : 0u;
+    }
+    u64 lo = off[d], hi = off[d + 1];
+    if (CHECK) {  // unchecked offsets: keep the row inside the arrays
+      lo = min(lo, m);
+      hi = min(max(hi, lo), m);
+    }
+    const u64 end = hi;
+    while (lo < hi) {
+      const u64 mid = (lo + hi) >> 1;
+      if (dst[mid] < s) lo = mid + 1;
+      else hi = mid;
+    }
+    bool found = false;
+    for (u64 j = lo; j < end && dst[j] == s; ++j) {
+      if (w[j] == wi) {
+        if (before == 0) {
+          val[i] = (u64(wi) << 32) | j;
+          found = true;
+          break;
+        }
+        --before;
+      }
+    }
+    if (!found) {
+      val[i] = VAL_NONE;
+      bad |= ERR_ASYMMETRIC;
     }
   }
-  val[i] = VAL_NONE;
-  atomicOr(err, ERR_ASYMMETRIC);
+  if (bad) atomicOr(err, bad);
+  if (!CHECK) return;
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    fwd += __shfl_xor(fwd, o, WAVE);
+    hf += __shfl_xor(hf, o, WAVE);
+    hr += __shfl_xor(hr, o, WAVE);
+  }
+  if (lane_id() == 0) {
+    s_red[wave_id()][0] = fwd;
+    s_red[wave_id()][1] = hf;
+    s_red[wave_id()][2] = hr;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    u64 t = 0;
+    for (int ww = 0; ww < WAVES; ++ww) t += s_red[ww][threadIdx.x];
+    if (threadIdx.x == 0) {
+      if (t) atomicAdd(chk.fwd_count, static_cast<u32>(t));
+    } else if (t) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(chk.hsum + threadIdx.x - 1),
+                static_cast<unsigned long long>(t));
+    }
+  }
 }
 
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
-                const u32* off, u64* val, u32* err) {
+                const u32* off, u64* val, u32* err, const InputCheck* check) {
   if (m == 0) return;
   KTScope t(st, K_KEYS, 20.0 * m);
-  keys_kernel<<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, dst, w, m, off, val, err);
+  const u32 grid = grid_for(m, 4096);
+  if (check)
+    keys_kernel<true><<<grid, BLOCK, 0, st>>>(src, dst, w, m, off, val, err, *check);
+  else
+    keys_kernel<false><<<grid, BLOCK, 0, st>>>(src, dst, w, m, off, val, err, InputCheck{});
   KMST_HIP(hipGetLastError());
 }
 

@@ -65,8 +65,11 @@ bool validate_records(hipStream_t st, const u32* src, const u32* dst, const u32*
                       u32* seg_end = nullptr, u32 q0 = 0, u32 q1 = 0, u32* unit_c0 = nullptr,
                       u32* unit_c1 = nullptr);
 void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off);
+struct InputCheck;
+// Positional keys (twin search in the CSR rows given by off); with check the
+// same pass also validates the (then unchecked) input.
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
-                const u32* off, u64* val, u32* err);
+                const u32* off, u64* val, u32* err, const InputCheck* check = nullptr);
 void build_keys_packed(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                        u32 vbits, u64* val);
 

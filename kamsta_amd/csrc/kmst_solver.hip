@@ -327,11 +327,19 @@ void Solver::setup(bool filter, bool plan_classes) {
       check_in_scatter = true;
       return;
     }
-    // positional keys are built from the checked input: validate first
-    // (no class counting, which the weights pass did)
-    validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW, hsum,
-                     WeightPivots{}, class64, seg_start, seg_end);
+    // positional keys: the key pass validates the input too (its twin search
+    // is bounded for unchecked records)
+    InputCheck chk;
+    chk.n = n0;
+    chk.fwd_count = ctr + C_FWD;
+    chk.hsum = hsum;
+    chk.seg_start = seg_start;
+    chk.seg_end = seg_end;
+    val0 = static_cast<u64*>(C.val0.get(m0 * 8));
+    build_offsets(st, in_s, m0, n0, label /* reused as offsets, n0+1 */);
+    build_keys(st, in_s, in_d, in_w, m0, label, val0, ctr + C_ERR, &chk);
     check_input();
+    return;
   } else {
     validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW, hsum,
                      pivots, class64, filter ? seg_start : nullptr, filter ? seg_end : nullptr);

@@ -270,3 +270,34 @@ def test_rhg_hubs(gpu, n):
     """Config 4 shape at reduced n: power-law degrees (hubs across many tiles)."""
     s, d, w = k.gen_rhg_host(n, 32.0, 2.8, 254, 4)
     check_equal(k.to_wedges(s, d, w), threshold=4)
+
+
+@pytest.mark.parametrize("n,m,mw", [(4096, 1 << 16, 254),              # packed keys
+                                    (70000, 400000, (1 << 31) - 1)])   # positional keys
+def test_contract_errors_with_weight_classes(gpu, n, m, mw):
+    """Inputs large enough for filter-Boruvka's weight classes: the input
+    checks then run inside the class scatter (packed keys) or the key pass
+    (positional keys) instead of validate_kernel."""
+    s, d, w = k.gen_gnm_host(n, m, mw, 13)
+    good = k.to_wedges(s, d, w)
+    exp, exp_w = oracle.kruskal(good)
+    got, got_w = k.filter_boruvka(good)
+    assert got_w == exp_w and np.array_equal(strip(got), strip(exp))
+    i = len(good) // 2
+    unsorted = good.copy()
+    unsorted[[i, i + 1]] = unsorted[[i + 1, i]]
+    asym = good.copy()
+    asym["weight"][i] = asym["weight"][i] % mw + 1
+    loop = good.copy()
+    loop["dst"][i] = loop["src"][i]
+    wide = good.copy()
+    wide["dst"][-1] = n + 5
+    for bad, ok in ((unsorted, (k.KMST_EUNSORTED,)), (asym, (k.KMST_EASYMMETRIC,)),
+                    (loop, (k.KMST_EINVAL, k.KMST_EUNSORTED)),
+                    (wide, (k.KMST_EINVAL, k.KMST_EUNSORTED, k.KMST_EASYMMETRIC))):
+        with pytest.raises(k.KmstError) as e:
+            k.filter_boruvka(bad)
+        assert e.value.status in ok, (e.value.status, ok)
+    # the context still works after the errors
+    got, got_w = k.filter_boruvka(good)
+    assert got_w == exp_w and np.array_equal(strip(got), strip(exp))
