@@ -152,13 +152,16 @@ __device__ __forceinline__ T block_exclusive_sum(T x, T& total, T* lds) {
 
 // ------------------------------------------------ decoupled look-back
 // Tile status word: bits 63..62 flag (0 none, 1 aggregate, 2 inclusive
-// prefix), bits 61..0 value.  Tiles take dynamic ids in dispatch order, so a
-// tile only ever waits for tiles that are already running, and every wait is
-// bounded: on expiry the kernel raises ERR_SPIN and the tile writes nothing,
-// which the host turns into an error.
+// prefix), bits 61..48 the call's epoch, bits 47..0 value.  A word of another
+// epoch reads as "none", so the status array is cleared only when it is
+// (re)allocated or the epochs wrap.  Tiles take dynamic ids in dispatch
+// order, so a tile only ever waits for tiles that are already running, and
+// every wait is bounded: on expiry the kernel raises ERR_SPIN and the tile
+// writes nothing, which the host turns into an error.
 constexpr u64 LB_AGG = u64{1} << 62;
 constexpr u64 LB_INC = u64{2} << 62;
-constexpr u64 LB_VAL = (u64{1} << 62) - 1;
+constexpr u32 LB_EPOCH_BITS = 14;
+constexpr u64 LB_VAL = (u64{1} << 48) - 1;
 constexpr u32 LB_SPIN_LIMIT = 1u << 22;
 
 __device__ __forceinline__ u64 lb_load(const u64* p) {
@@ -167,26 +170,30 @@ __device__ __forceinline__ u64 lb_load(const u64* p) {
 __device__ __forceinline__ void lb_store(u64* p, u64 v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ u64 lb_word(u64 flag, u32 epoch, u64 value) {
+  return flag | (u64(epoch) << 48) | (value & LB_VAL);
+}
 
 // Called by all 64 lanes of one wave; returns the exclusive prefix of `tile`
 // (wave-uniform) and publishes the inclusive one.  `ok` turns false when the
 // bounded wait expired; the caller must then write nothing.
-__device__ __forceinline__ u64 lookback_exclusive(u64* status, u32 tile, u64 aggregate, u32* err,
-                                                  bool& ok) {
+__device__ __forceinline__ u64 lookback_exclusive(u64* status, u32 epoch, u32 tile, u64 aggregate,
+                                                  u32* err, bool& ok) {
   ok = true;
   const int lane = threadIdx.x & (WAVE - 1);
   if (tile == 0) {
-    if (lane == 0) lb_store(status, LB_INC | aggregate);
+    if (lane == 0) lb_store(status, lb_word(LB_INC, epoch, aggregate));
     return 0;
   }
-  if (lane == 0) lb_store(status + tile, LB_AGG | aggregate);
+  if (lane == 0) lb_store(status + tile, lb_word(LB_AGG, epoch, aggregate));
   u64 excl = 0;
   i64 end = static_cast<i64>(tile);  // window covers tiles [end-64, end)
   u32 spins = 0;
   while (true) {
     const i64 idx = end - 1 - lane;
-    const u64 w = (idx >= 0) ? lb_load(status + idx) : LB_INC;
-    const u64 flag = w >> 62;
+    const u64 w = (idx >= 0) ? lb_load(status + idx) : lb_word(LB_INC, epoch, 0);
+    const bool current = ((w >> 48) & ((1u << LB_EPOCH_BITS) - 1)) == epoch;
+    const u64 flag = current ? (w >> 62) : 0;
     const u64 inc_mask = __ballot(flag == 2);
     const u64 none_mask = __ballot(flag == 0);
     const int first_inc = inc_mask ? (__ffsll(static_cast<long long>(inc_mask)) - 1) : WAVE;
@@ -207,7 +214,7 @@ __device__ __forceinline__ u64 lookback_exclusive(u64* status, u32 tile, u64 agg
     if (first_inc < WAVE) break;
     end -= WAVE;
   }
-  if (lane == 0) lb_store(status + tile, LB_INC | ((excl + aggregate) & LB_VAL));
+  if (lane == 0) lb_store(status + tile, lb_word(LB_INC, epoch, excl + aggregate));
   return excl;
 }
 

@@ -926,10 +926,20 @@ __global__ __launch_bounds__(BLOCK) void argmin_fixup_kernel(const ArgPiece* __r
   }
 }
 
+__global__ __launch_bounds__(BLOCK) void argmin_init_kernel(u64* __restrict__ best_val, u64 n,
+                                                           u32* __restrict__ piece_count) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *piece_count = 0;
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += u64(gridDim.x) * BLOCK)
+    best_val[i] = VAL_NONE;
+}
+
 void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const KeySrc& keys, u64 m,
                       u64 n, u64* best_val, u32* best_dst, ArgPiece* pieces, u32* piece_count) {
-  fill_u64(st, best_val, n, VAL_NONE);
-  fill_u32(st, piece_count, 1, 0);
+  {
+    KTScope t(st, K_FILL, 8.0 * n);
+    argmin_init_kernel<<<grid_for(n), BLOCK, 0, st>>>(best_val, n, piece_count);
+    KMST_HIP(hipGetLastError());
+  }
   if (m == 0) return;
   const u32 tiles = ceil_div_u32(m, ARG_TILE);
   {
@@ -1052,9 +1062,10 @@ void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_id, c
 // Relabel both endpoints, drop self-loops and dead records, keep order; one
 // pass: each wave owns a contiguous sub-tile (coalesced rows of 64 records),
 // ballots give the in-wave order, a decoupled look-back gives the tile offset.
-__device__ __forceinline__ u32 lb_tile_id(u32* counter) {
+// Dynamic tile id in dispatch order from a counter that only grows.
+__device__ __forceinline__ u32 lb_tile_id(unsigned long long* counter, unsigned long long base) {
   __shared__ u32 s_tile;
-  if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+  if (threadIdx.x == 0) s_tile = static_cast<u32>(atomicAdd(counter, 1ull) - base);
   __syncthreads();
   return s_tile;
 }
@@ -1096,12 +1107,13 @@ __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 key_min,
     u64 key_max, u64 m, const u32* __restrict__ label, const u64* __restrict__ giant,
     u32* __restrict__ s_out,
-    u32* __restrict__ d_out, u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles,
-    u32* count_dev, u32* err) {
+    u32* __restrict__ d_out, u64* __restrict__ v_out, u64* lb_status, u32 lb_epoch,
+    unsigned long long* lb_counter, unsigned long long lb_base, u32 tiles, u32* count_dev,
+    u32* err) {
   __shared__ u32 s_wave_cnt[WAVES];
   __shared__ u64 s_excl;
   __shared__ u32 s_ok;
-  const u32 tile = lb_tile_id(lb_counter);
+  const u32 tile = lb_tile_id(lb_counter, lb_base);
   const int lane = lane_id(), w = wave_id();
   const u64 wbase = u64(tile) * CMP_TILE + u64(w) * (WAVE * CMP_IPT);
   u32 ls[CMP_IPT], ld[CMP_IPT];
@@ -1145,7 +1157,7 @@ __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
 #pragma unroll
     for (int ww = 0; ww < WAVES; ++ww) agg += s_wave_cnt[ww];
     bool ok;
-    const u64 excl = lookback_exclusive(lb_status, tile, agg, err, ok);
+    const u64 excl = lookback_exclusive(lb_status, lb_epoch, tile, agg, err, ok);
     if (lane == 0) {
       s_excl = excl;
       s_ok = ok ? 1u : 0u;
@@ -1168,27 +1180,50 @@ __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
   }
 }
 
+void LookbackWork::bind(hipStream_t st, u64* p, u64 m) {
+  const u64 tiles = ceil_div_u32(m ? m : 1, CMP_TILE);
+  if (p != words || tiles > tiles_cap) {
+    // fresh memory: zero the status words and the tile counter once
+    words = p;
+    tiles_cap = tiles;
+    fill_u64(st, p, tiles + 1, 0);
+    epoch = 0;
+    tiles_issued = 0;
+  }
+}
+
+u32 LookbackWork::next_call(hipStream_t st, u32 tiles) {
+  if (tiles > tiles_cap) throw Error(KMST_EINVAL, "look-back workspace too small");
+  if (++epoch == (1u << LB_EPOCH_BITS)) {
+    // epochs wrapped: clear the status words (the counter keeps growing)
+    fill_u64(st, words, tiles_cap, 0);
+    epoch = 1;
+  }
+  return epoch;
+}
+
 void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                      u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
-                     u64* lb, u32* count_dev, u32* err, const u64* giant, u64 n, u64 key_max) {
+                     LookbackWork& lb, u32* count_dev, u32* err, const u64* giant, u64 n,
+                     u64 key_max) {
   if (m == 0) {
     fill_u32(st, count_dev, 1, 0);
     return;
   }
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
-  fill_u64(st, lb, u64(tiles) + 1, 0);  // tile states + dynamic tile counter
+  const u32 epoch = lb.next_call(st, tiles);
   // reads: record (12 B with computed keys, 16 B stored) + two 4-byte label
   // gathers; with the component bitmap only (src, dst) and the bitmap are
   // counted (a lower bound).  The 16-byte writes of kept records are added by
   // the caller.
   KTScope t(st, K_RELABEL_SCATTER,
             giant ? 8.0 * m + double(n) / 8.0 : (keys.v ? 16.0 : 12.0) * m + 8.0 * m);
-  relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, key_min, key_max, m, label,
-                                                   giant,
-                                                   s_out, d_out, v_out,
-                                                   lb, reinterpret_cast<u32*>(lb + tiles), tiles,
-                                                   count_dev, err);
+  relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(
+      s_in, d_in, keys, key_min, key_max, m, label, giant, s_out, d_out, v_out, lb.words, epoch,
+      reinterpret_cast<unsigned long long*>(lb.words + lb.tiles_cap), lb.tiles_issued, tiles,
+      count_dev, err);
   KMST_HIP(hipGetLastError());
+  lb.tiles_issued += tiles;
 }
 
 // ============================================================== K6 sort
@@ -1282,8 +1317,9 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const u32* __restr
   }
 }
 
-// base[p * RADIX + x] = number of records whose digit p is below x.
-__global__ __launch_bounds__(BLOCK) void radix_digit_base_kernel(const u32* __restrict__ ghist,
+// base[p * RADIX + x] = number of records whose digit p is below x; the
+// histograms are cleared for the next sort (they start zeroed at bind).
+__global__ __launch_bounds__(BLOCK) void radix_digit_base_kernel(u32* __restrict__ ghist,
                                                                  u32 passes,
                                                                  u32* __restrict__ base) {
   __shared__ u32 tmp[WAVES];
@@ -1291,6 +1327,7 @@ __global__ __launch_bounds__(BLOCK) void radix_digit_base_kernel(const u32* __re
     u32 total;
     const u32 x = ghist[p * RADIX + threadIdx.x];
     base[p * RADIX + threadIdx.x] = block_exclusive_sum(x, total, tmp);
+    ghist[p * RADIX + threadIdx.x] = 0;
   }
 }
 
@@ -1515,10 +1552,12 @@ u64 sort_work_words(u64 m) {
 void SortWork::bind(hipStream_t st, u32* p, u64 m) {
   const u64 tiles = ceil_div_u32(m ? m : 1, SORT_TILE);
   if (p != words || tiles > tiles_cap) {
-    // fresh memory: zero the status words and the tile counter once
+    // fresh memory: zero the status words, the tile counter and the digit
+    // histograms once
     words = p;
     tiles_cap = tiles;
     fill_u64(st, reinterpret_cast<u64*>(p), tiles * RADIX + 1, 0);
+    fill_u32(st, p + 2 * (tiles * RADIX + 1), u64(MAX_PASSES) * RADIX, 0);
     epoch = 0;
     tiles_issued = 0;
   }
@@ -1554,9 +1593,8 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
   if (total_bits > 64) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
   const DigitPlan pl = plan_digits(total_bits, work.balanced);
   const u32 passes = pl.passes;
-  u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);
+  u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);  // zero between sorts
   u32* base = ghist + MAX_PASSES * RADIX;
-  fill_u32(st, ghist, u64(passes) * RADIX, 0);
   {
     KTScope t(st, K_SORT_HIST, (d ? 8.0 : 4.0) * m);
     // few, fat blocks: every block adds its histograms to the same global bins
@@ -1737,9 +1775,8 @@ bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWor
   const DigitPlan pl = plan_digits(bits, work.balanced);
   const u32 passes = pl.passes;
   const u64 mask = bits >= 64 ? ~0ull : ((u64{1} << bits) - 1);  // bits above are not key
-  u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);
+  u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);  // zero between sorts
   u32* base = ghist + MAX_PASSES * RADIX;
-  fill_u32(st, ghist, u64(passes) * RADIX, 0);
   {
     KTScope t(st, K_SORT_HIST, 8.0 * m);
     radix_upsweep_u64_kernel<<<grid_for((m + UP_RPT - 1) / UP_RPT, 512), BLOCK, 0, st>>>(
@@ -1933,12 +1970,12 @@ void sample_weights(hipStream_t st, const u64* v, const u32* wt, u64 m, u32 k, u
 __global__ __launch_bounds__(BLOCK) void split_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 m,
     u32 pivot, u32 wshift, bool write_heavy, u32* __restrict__ s_out, u32* __restrict__ d_out,
-    u64* __restrict__ v_out, u64* lb_status, u32* lb_counter, u32 tiles, u32* light_dev,
-    u32* err) {
+    u64* __restrict__ v_out, u64* lb_status, u32 lb_epoch, unsigned long long* lb_counter,
+    unsigned long long lb_base, u32 tiles, u32* light_dev, u32* err) {
   __shared__ u32 s_wave_light[WAVES];
   __shared__ u64 s_excl;
   __shared__ u32 s_ok;
-  const u32 tile = lb_tile_id(lb_counter);
+  const u32 tile = lb_tile_id(lb_counter, lb_base);
   const int lane = lane_id(), w = wave_id();
   const u64 tile_base = u64(tile) * CMP_TILE;
   const u64 wbase = tile_base + u64(w) * (WAVE * CMP_IPT);
@@ -1963,7 +2000,7 @@ __global__ __launch_bounds__(BLOCK) void split_kernel(
 #pragma unroll
     for (int ww = 0; ww < WAVES; ++ww) agg += s_wave_light[ww];
     bool ok;
-    const u64 excl = lookback_exclusive(lb_status, tile, agg, err, ok);
+    const u64 excl = lookback_exclusive(lb_status, lb_epoch, tile, agg, err, ok);
     if (lane == 0) {
       s_excl = excl;
       s_ok = ok ? 1u : 0u;
@@ -1996,21 +2033,22 @@ __global__ __launch_bounds__(BLOCK) void split_kernel(
 
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                        u64 m, u32 pivot, u32 wshift, bool write_heavy, u32* s_out, u32* d_out,
-                       u64* v_out, u64* lb, u32* light_count_dev, u32* err) {
+                       u64* v_out, LookbackWork& lb, u32* light_count_dev, u32* err) {
   if (m == 0) {
     fill_u32(st, light_count_dev, 1, 0);
     return;
   }
   const u32 tiles = ceil_div_u32(m, CMP_TILE);
-  fill_u64(st, lb, u64(tiles) + 1, 0);
+  const u32 epoch = lb.next_call(st, tiles);
   // reads the records; writes (light, plus heavy when kept) are added by
   // the caller once the light count is known
   KTScope t(st, K_SPLIT_SCATTER, (keys.v ? 16.0 : 12.0) * m);
-  split_kernel<<<tiles, BLOCK, 0, st>>>(s_in, d_in, keys, m, pivot, wshift, write_heavy, s_out,
-                                         d_out, v_out, lb,
-                                         reinterpret_cast<u32*>(lb + tiles), tiles,
-                                         light_count_dev, err);
+  split_kernel<<<tiles, BLOCK, 0, st>>>(
+      s_in, d_in, keys, m, pivot, wshift, write_heavy, s_out, d_out, v_out, lb.words, epoch,
+      reinterpret_cast<unsigned long long*>(lb.words + lb.tiles_cap), lb.tiles_issued, tiles,
+      light_count_dev, err);
   KMST_HIP(hipGetLastError());
+  lb.tiles_issued += tiles;
 }
 
 // Present-vertex compaction before a sparse base case: flag every src, scan,

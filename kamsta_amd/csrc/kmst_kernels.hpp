@@ -87,18 +87,30 @@ void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err);
 void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_id, const u64* state,
                  const u64* pos, const u64* best_val, u32* label, u64* mst, u64 mst_base);
 
-// K4+K5 relabel, drop self-loops and dead records, order-preserving compaction
-// in one pass.  lb needs lookback_elems(m) u64 entries; *count_dev gets the
-// kept count.
+// Look-back state of the single-pass compactions (relabel, split): one
+// status word per tile and a tile counter (lookback_elems(m) u64 words).
+// Status words carry a call epoch, so they are cleared only when the memory
+// is (re)bound; bind() with the buffer for the largest m before the calls.
 inline u64 lookback_elems(u64 m) { return (m + CMP_TILE - 1) / CMP_TILE + 1; }
+struct LookbackWork {
+  u64* words = nullptr;
+  u64 tiles_cap = 0;     // tiles the status area holds
+  u32 epoch = 0;         // epoch of the last call
+  u64 tiles_issued = 0;  // value of the tile counter
+  void bind(hipStream_t st, u64* p, u64 m);
+  u32 next_call(hipStream_t st, u32 tiles);  // epoch for a call over `tiles` tiles
+};
+
+// K4+K5 relabel, drop self-loops and dead records, order-preserving compaction
+// in one pass; *count_dev gets the kept count.
 // Records whose key is below key_min are dropped too (0 keeps all).  With
 // giant (a bitmap over the n vertex ids of one component of label, see
 // component_bits) records with both ends inside it are dropped without reading
 // their key or labels.
 void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                      u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
-                     u64* lb, u32* count_dev, u32* err, const u64* giant = nullptr, u64 n = 0,
-                     u64 key_max = ~0ull);
+                     LookbackWork& lb, u32* count_dev, u32* err, const u64* giant = nullptr,
+                     u64 n = 0, u64 key_max = ~0ull);
 // bits[v / 64] bit v % 64 = (gpar[v] == root) for v < n; bits needs
 // ceil(n / 64) entries.
 void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits);
@@ -190,7 +202,7 @@ void sample_weights(hipStream_t st, const u64* v, const u32* wt, u64 m, u32 k, u
                     u32* out);
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                        u64 m, u32 pivot, u32 wshift, bool write_heavy, u32* s_out, u32* d_out,
-                       u64* v_out, u64* lb, u32* light_count_dev, u32* err);
+                       u64* v_out, LookbackWork& lb, u32* light_count_dev, u32* err);
 void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32* count_dev);
 
 // Present-vertex compaction helpers (sorted records; flag must be zeroed).

@@ -191,7 +191,6 @@ class Solver {
   u64* mst = nullptr;
   u64 mst_count = 0;
   u32* scan_scratch = nullptr;
-  u64* lb = nullptr;
   ArgPiece* pieces = nullptr;
   u32* samples = nullptr;
 
@@ -269,7 +268,7 @@ void Solver::allocate() {
   mst = static_cast<u64*>(C.mst.get(n * 8));
   const u64 arg_tiles = (m + ARG_TILE - 1) / ARG_TILE;
   pieces = static_cast<ArgPiece*>(C.pieces.get(2 * arg_tiles * sizeof(ArgPiece) + 64));
-  lb = static_cast<u64*>(C.lookback.get(lookback_elems(m) * 8));
+  C.lb_work.bind(st, static_cast<u64*>(C.lookback.get(lookback_elems(m) * 8)), m);
   const u64 sort_m = std::max(m, n);
   C.sort_work.bind(st, static_cast<u32*>(C.hist.get(sort_work_words(sort_m) * 4)), sort_m);
   apply_tuning(C);
@@ -410,7 +409,7 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
       std::swap(rep, rep_next);
     }
     RecView out = (cur.s == A.s) ? B : A;
-    relabel_compact(st, cur.s, cur.d, keys_of(cur), 0, m, label, out.s, out.d, out.v, lb,
+    relabel_compact(st, cur.s, cur.d, keys_of(cur), 0, m, label, out.s, out.d, out.v, C.lb_work,
                     ctr + C_MNEXT, ctr + C_ERR);
     check_errors("round");
     if (g_ktimer) g_ktimer->adjust_last(K_RELABEL_SCATTER, 16.0 * C.host_counters[C_MNEXT]);
@@ -442,7 +441,7 @@ void Solver::base_case(RecView P, RecView Q) {
     mark_present(st, P.s, P.m, flag);
     scan_exclusive_u32(st, flag, newid, n0, scan_scratch, ctr + C_NNEXT);
     compact_rep(st, flag, newid, n0, rep);
-    relabel_compact(st, P.s, P.d, keys_of(P), 0, P.m, newid, Q.s, Q.d, Q.v, lb, ctr + C_MNEXT,
+    relabel_compact(st, P.s, P.d, keys_of(P), 0, P.m, newid, Q.s, Q.d, Q.v, C.lb_work, ctr + C_MNEXT,
                     ctr + C_ERR);
     check_errors("compact");
     if (g_ktimer) g_ktimer->adjust_last(K_RELABEL_SCATTER, 16.0 * C.host_counters[C_MNEXT]);
@@ -564,7 +563,7 @@ void Solver::filter_classes() {
 
 u64 Solver::partition(RecView in, RecView out, u32 pivot, bool write_heavy) {
   split_light_heavy(st, in.s, in.d, keys_of(in), in.m, pivot, packed ? 2 * vbits : 32,
-                    write_heavy, out.s, out.d, out.v, lb, ctr + C_LIGHT, ctr + C_ERR);
+                    write_heavy, out.s, out.d, out.v, C.lb_work, ctr + C_LIGHT, ctr + C_ERR);
   check_errors("split");
   const u64 L = C.host_counters[C_LIGHT];
   if (g_ktimer) g_ktimer->adjust_last(K_SPLIT_SCATTER, 16.0 * (write_heavy ? in.m : L));
@@ -638,7 +637,7 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
       giant = best_val;
     }
     relabel_compact(st, heavy.s, heavy.d, keys_of(heavy), key_min, heavy.m, gpar, scratch.s,
-                    scratch.d, scratch.v, lb, ctr + C_MNEXT, ctr + C_ERR, giant, n0, key_max);
+                    scratch.d, scratch.v, C.lb_work, ctr + C_MNEXT, ctr + C_ERR, giant, n0, key_max);
   }
   check_errors("filter");
   const u64 Hk = C.host_counters[C_MNEXT];
@@ -969,6 +968,7 @@ void kmst_context_trim(kmst_context* ctx) {
   for (kmst::DevBuf* b : ctx->all_bufs())
     if (b != &ctx->counters) b->release();
   ctx->sort_work = kmst::SortWork{};  // its status words are gone with `hist`
+  ctx->lb_work = kmst::LookbackWork{};  // and these with `lookback`
 }
 
 kmst_status kmst_boruvka(const kmst_wedge* in, uint64_t m, const kmst_config* cfg, kmst_wedge* out,

@@ -69,6 +69,7 @@ struct kmst_context {
   // kernel scratch
   kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback;
   kmst::SortWork sort_work;  // radix sort state over `hist`
+  kmst::LookbackWork lb_work;  // compaction look-back state over `lookback`
   kmst_tuning tuning{0, 1, kmst::SORT_WIDE_MIN};  // launch shapes of later solves
   // host-API staging
   kmst::DevBuf in_s, in_d, in_w, out_s, out_d, out_w;
