@@ -1269,7 +1269,9 @@ __device__ __forceinline__ u32 plan_digit(u64 key, const DigitPlan& pl, u32 p) {
 }
 
 // ghist[p * RADIX + x] = number of records whose digit p is x.  Per-wave
-// LDS histograms; each thread loads UP_RPT keys before counting them.
+// LDS histograms.  Each thread counts UP_RPT consecutive keys and adds runs
+// of equal digits at once: records grouped by src arrive in runs (a vertex's
+// records, relabelled alike), so most LDS atomics fold away.
 constexpr int UP_RPT = 8;
 
 template <bool HAS_D>
@@ -1281,31 +1283,32 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const u32* __restr
   __shared__ u32 h[WAVES][MAX_PASSES][RADIX];
   for (int i = threadIdx.x; i < WAVES * MAX_PASSES * RADIX; i += BLOCK) (&h[0][0][0])[i] = 0;
   __syncthreads();
-  const int w = wave_id(), lane = lane_id();
+  const int w = wave_id();
   const u64 stride = u64(gridDim.x) * (BLOCK * UP_RPT);
-  for (u64 base = u64(blockIdx.x) * (BLOCK * UP_RPT); base < m; base += stride) {  // uniform
+  for (u64 base = u64(blockIdx.x) * (BLOCK * UP_RPT) + u64(threadIdx.x) * UP_RPT; base < m;
+       base += stride) {
     u64 key[UP_RPT];
 #pragma unroll
     for (int k = 0; k < UP_RPT; ++k) {
-      const u64 e = min(base + u64(k) * BLOCK + threadIdx.x, m - 1);
+      const u64 e = min(base + k, m - 1);
       key[k] = sort_key_of<HAS_D>(s[e], HAS_D ? d[e] : 0u, dbits);
     }
+    const u32 valid = static_cast<u32>(min<u64>(UP_RPT, m - base));
+    for (u32 p = 0; p < passes; ++p) {
+      u32 cur = plan_digit(key[0], pl, p), run = 1;
 #pragma unroll
-    for (int k = 0; k < UP_RPT; ++k) {
-      const bool valid = base + u64(k) * BLOCK + threadIdx.x < m;
-      const u64 vmask = __ballot(valid);
-      for (u32 p = 0; p < passes; ++p) {
-        const u32 digit = plan_digit(key[k], pl, p);
-        // a wave whose records share the digit adds once (the usual case for
-        // the high digits); lane 0 is valid whenever any lane is
-        const u32 first = __builtin_amdgcn_readfirstlane(digit);
-        const u64 same = __ballot(valid && digit == first);
-        if (same == vmask) {
-          if (lane == 0 && vmask) h[w][p][first] += static_cast<u32>(__popcll(vmask));
-        } else if (valid) {
-          atomicAdd(&h[w][p][digit], 1u);
+      for (int k = 1; k < UP_RPT; ++k) {
+        if (u32(k) >= valid) break;
+        const u32 dg = plan_digit(key[k], pl, p);
+        if (dg == cur) {
+          ++run;
+        } else {
+          atomicAdd(&h[w][p][cur], run);
+          cur = dg;
+          run = 1;
         }
       }
+      atomicAdd(&h[w][p][cur], run);
     }
   }
   __syncthreads();
