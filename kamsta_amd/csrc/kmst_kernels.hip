@@ -1034,6 +1034,51 @@ void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err) {
   KMST_HIP(hipGetLastError());
 }
 
+// Pointer jumping with the round's bookkeeping folded in (the state scan only
+// needs the hook, so it runs first): once v has reached its root r,
+// label[v] = root_id[r] (NONE32 for finished vertices), an emitter writes its
+// edge key into the MST list at mst_base + its slot, and with gpar the
+// parent array over the original ids is updated as gpar_kernel does.
+__global__ __launch_bounds__(BLOCK) void jump_label_kernel(
+    u64 n, u32* parent, const u32* __restrict__ root_id, const u64* __restrict__ state,
+    const u64* __restrict__ pos, const u64* __restrict__ best_val, u32* __restrict__ label,
+    u64* __restrict__ mst, u64 mst_base, const u32* __restrict__ rep, u32* __restrict__ gpar,
+    u32* __restrict__ rep_next, u32* err) {
+  const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  if (v >= n) return;
+  u32 x = parent[v];
+  u32 y = parent[x];
+  u32 guard = 0;
+  while (x != y) {
+    parent[v] = y;
+    x = y;
+    y = parent[x];
+    if (++guard > (1u << 26)) {
+      atomicOr(err, ERR_SPIN);
+      return;
+    }
+  }
+  parent[v] = x;
+  label[v] = root_id[x];
+  const u64 sv = state[v];
+  if (sv & ST_EMIT) mst[mst_base + (pos[v] >> 32)] = best_val[v];
+  if (gpar && sv != 0) {
+    const u32 name = rep ? rep[v] : static_cast<u32>(v);
+    gpar[name] = rep ? rep[x] : x;
+    if (sv & ST_ROOT) rep_next[static_cast<u32>(pos[v])] = name;
+  }
+}
+
+void jump_and_label(hipStream_t st, u64 n, u32* parent, const u32* root_id, const u64* state,
+                    const u64* pos, const u64* best_val, u32* label, u64* mst, u64 mst_base,
+                    const u32* rep, u32* gpar, u32* rep_next, u32* err) {
+  if (n == 0) return;
+  KTScope t(st, K_JUMP, 40.0 * n + (gpar ? 16.0 * n : 0.0));
+  jump_label_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(
+      n, parent, root_id, state, pos, best_val, label, mst, mst_base, rep, gpar, rep_next, err);
+  KMST_HIP(hipGetLastError());
+}
+
 // label[v] = new consecutive id of v's root (NONE32 for finished vertices);
 // emitters write their edge key into the MST list at mst_base + their slot.
 __global__ __launch_bounds__(BLOCK) void label_kernel(u64 n, const u32* __restrict__ parent,

@@ -83,6 +83,11 @@ void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const KeyS
 void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* parent,
           u64* state);
 void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err);
+// Pointer jumping, labels, MST emission and (with gpar) the parent-array
+// update of one round in one pass; the state scan must have run.
+void jump_and_label(hipStream_t st, u64 n, u32* parent, const u32* root_id, const u64* state,
+                    const u64* pos, const u64* best_val, u32* label, u64* mst, u64 mst_base,
+                    const u32* rep, u32* gpar, u32* rep_next, u32* err);
 // root_id[r] = new id of root r (NONE32 for other vertices), from the scan
 void make_labels(hipStream_t st, u64 n, const u32* parent, const u32* root_id, const u64* state,
                  const u64* pos, const u64* best_val, u32* label, u64* mst, u64 mst_base);

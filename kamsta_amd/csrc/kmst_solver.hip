@@ -399,15 +399,13 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     segmented_argmin(st, cur.s, cur.d, keys_of(cur), m, n, best_val, best_dst, pieces,
                      ctr + C_PIECES);
     hook(st, n, best_val, best_dst, parent, state);
-    pointer_jump(st, n, parent, ctr + C_ERR);
     // best_dst is free after the hook: it takes the new ids of the roots
     scan_exclusive_u64(st, state, pos, n, reinterpret_cast<u64*>(scan_scratch),
                        reinterpret_cast<u64*>(ctr + C_ROUND64), best_dst);
-    make_labels(st, n, parent, best_dst, state, pos, best_val, label, mst, mst_count);
-    if (track_gpar) {
-      update_parent_array(st, n, first ? nullptr : rep, parent, state, pos, gpar, rep_next);
-      std::swap(rep, rep_next);
-    }
+    jump_and_label(st, n, parent, best_dst, state, pos, best_val, label, mst, mst_count,
+                   (track_gpar && !first) ? rep : nullptr, track_gpar ? gpar : nullptr,
+                   rep_next, ctr + C_ERR);
+    if (track_gpar) std::swap(rep, rep_next);
     RecView out = (cur.s == A.s) ? B : A;
     relabel_compact(st, cur.s, cur.d, keys_of(cur), 0, m, label, out.s, out.d, out.v, C.lb_work,
                     ctr + C_MNEXT, ctr + C_ERR);
