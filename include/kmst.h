@@ -51,7 +51,10 @@ typedef struct kmst_config {
   int32_t algorithm;            /* kmst_algorithm */
   int32_t local_preprocessing;  /* LocalPreprocessing: accepted, no effect on one GPU
                                    (the whole graph is local) */
-  int32_t compression;          /* Compression: only 0 (NO_COMPRESSION) */
+  int32_t compression;          /* Compression: 0 none; 1 the difference codec
+                                   (DifferenceEncodedGraph): the host API sends the
+                                   records difference-encoded and decodes them on the
+                                   device; no effect on kmst_solve_device */
   int32_t device;               /* HIP device ordinal */
   uint64_t filter_threshold;    /* filter-Boruvka base case when m/n0 <= this (default 4) */
 } kmst_config;
@@ -115,6 +118,19 @@ kmst_status kmst_solve_device(kmst_context* ctx, const kmst_dev_edges* in,
 /* Per-kernel HIP-event timing of later solves on this context (measurement
  * hook; off by default).  Totals appear in kmst_context_stats_json. */
 void kmst_context_set_timing(kmst_context* ctx, int on);
+
+/* Difference encoding of compression level 1 on the host
+ * (DifferenceEncodedGraph::encode_edges / decode_edge,
+ * include/datastructures/compression/difference_encoded_graph.hpp:166-212, with the
+ * varints of include/datastructures/variable_length_encoding.hpp:6-160): records in
+ * chunks of chunk_records; chunk k keeps its first record raw in
+ * first[3k..3k+2] = (src, dst, weight), the others as 7-bit varints of
+ * (src - previous src mod 2^32, dst, weight); offsets[k] is where chunk k's
+ * varints start, offsets[chunks] the length.  With bytes == NULL only
+ * *bytes_used is set. */
+kmst_status kmst_encode_host(const uint32_t* src, const uint32_t* dst, const uint32_t* weight,
+                             uint64_t m, uint32_t chunk_records, uint8_t* bytes, uint64_t bytes_cap,
+                             uint64_t* offsets, uint32_t* first, uint64_t* bytes_used);
 
 /* ---- Tuning (no counterpart in the reference, whose only knob on this path
  * is AlgorithmConfig::filter_threshold). ------------------------------------

@@ -43,6 +43,7 @@ EXPORTED_SYMBOLS = [
     "kmst_context_reserve",
     "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_context_set_timing",
     "kmst_tuning_default", "kmst_context_set_tuning", "kmst_context_get_tuning", "kmst_autotune",
+    "kmst_encode_host",
     "kmst_last_error", "kmst_context_stats_json", "kmst_gen_gnm_device", "kmst_gen_gnm_host",
     "kmst_k_segmented_argmin", "kmst_k_sort_pairs", "kmst_k_pointer_jump",
     "kmst_server_config_default", "kmst_server_create", "kmst_server_submit", "kmst_server_poll",
@@ -146,6 +147,7 @@ def load():
         "kmst_solve_device": (i32, [vp, vp, vp, vp, pu64, pu64]),
         "kmst_context_set_timing": (None, [vp, i32]),
         "kmst_tuning_default": (None, [vp]),
+        "kmst_encode_host": (i32, [vp, vp, vp, u64, u32, vp, u64, vp, vp, pu64]),
         "kmst_context_set_tuning": (i32, [vp, vp]),
         "kmst_context_get_tuning": (i32, [vp, vp]),
         "kmst_autotune": (i32, [vp, vp, vp, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double),
@@ -240,6 +242,26 @@ def gen_rhg_host(n, avg_degree=32.0, gamma=2.8, max_weight=254, seed=4):
     return _gen_sized(lambda s, d, w, cap, m: L.kmst_gen_rhg_host(
         n, float(avg_degree), float(gamma), max_weight, seed, s, d, w, cap, m),
         n * avg_degree * 1.5)
+
+
+def encode_host(src, dst, weight, chunk_records=32):
+    """Difference encoding of compression level 1 (kmst_encode_host): returns
+    (bytes uint8, offsets uint64 with chunks + 1 entries, first uint32 of
+    3 * chunks entries)."""
+    L = load()
+    s, d, w = (np.ascontiguousarray(a, dtype=np.uint32) for a in (src, dst, weight))
+    m = len(s)
+    used = ctypes.c_uint64(0)
+    _check(L.kmst_encode_host(s.ctypes.data, d.ctypes.data, w.ctypes.data, m, chunk_records,
+                              None, 0, None, None, ctypes.byref(used)))
+    chunks = (m + chunk_records - 1) // chunk_records
+    b = np.zeros(max(used.value, 1), dtype=np.uint8)
+    off = np.zeros(chunks + 1, dtype=np.uint64)
+    first = np.zeros(max(3 * chunks, 1), dtype=np.uint32)
+    _check(L.kmst_encode_host(s.ctypes.data, d.ctypes.data, w.ctypes.data, m, chunk_records,
+                              b.ctypes.data, len(b), off.ctypes.data, first.ctypes.data,
+                              ctypes.byref(used)))
+    return b[:used.value], off, first[:3 * chunks]
 
 
 def to_wedges(src, dst, weight):

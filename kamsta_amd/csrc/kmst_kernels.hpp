@@ -2,6 +2,8 @@
 // Every launcher enqueues work on `st` and returns immediately.
 #pragma once
 
+#include <vector>
+
 #include "kmst_common.hpp"
 
 namespace kmst {
@@ -241,5 +243,17 @@ void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySr
                    u32 wshift, u32 q0, u32 q1, const u32* off0, const u32* off1, u64 base1,
                    u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err,
                    const InputCheck* check = nullptr);
+
+// Difference codec of compression level 1 (kmst_codec.hip): chunks of
+// chunk_records records, the first raw, the rest as 7-bit varints of
+// (src delta, dst, weight).
+constexpr u32 CODEC_CHUNK = 32;
+u64 encoded_chunks(u64 m, u32 chunk_records);
+void encode_host(const u32* src, const u32* dst, const u32* w, u64 m, u32 chunk_records,
+                 std::vector<u8>& bytes, std::vector<u64>& offset, std::vector<u32>& first);
+// Decodes into src / dst / w (m entries each); a malformed stream raises
+// ERR_INVALID in *err.
+void decode_device(hipStream_t st, const u8* bytes, const u64* offset, const u32* first, u64 m,
+                   u32 chunk_records, u32* src, u32* dst, u32* w, u32* err);
 
 }  // namespace kmst

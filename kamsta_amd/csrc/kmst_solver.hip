@@ -128,7 +128,7 @@ std::vector<DevBuf*> kmst_context::all_bufs() {
           &rep,      &rep_next,   &gpar,    &seg_start, &seg_end, &mst,   &pieces,
           &tiles,    &scan_scratch, &hist,  &counters,  &lookback, &in_s, &in_d,
           &in_w,     &out_s,      &out_d,   &out_w,     &samples, &gen_a, &gen_b,
-          &gen_c,    &gen_d,      &tune};
+          &gen_c,    &gen_d,      &tune,    &codec_bytes, &codec_off, &codec_first};
 }
 
 namespace kmst {
@@ -757,8 +757,8 @@ void check_config(const kmst_config* cfg) {
   if (!cfg) throw kmst::Error(KMST_EINVAL, "null config");
   if (cfg->algorithm != KMST_ALGO_BORUVKA && cfg->algorithm != KMST_ALGO_FILTER_BORUVKA)
     throw kmst::Error(KMST_EINVAL, "unknown algorithm");
-  if (cfg->compression != 0)
-    throw kmst::Error(KMST_EINVAL, "only NO_COMPRESSION is supported");
+  if (cfg->compression != 0 && cfg->compression != 1)
+    throw kmst::Error(KMST_EINVAL, "compression must be 0 (none) or 1 (difference codec)");
 }
 
 void solve_device_impl(kmst_context* ctx, const kmst_dev_edges* in, const kmst_config* cfg,
@@ -824,7 +824,30 @@ kmst_status host_solve(const kmst_wedge* in, uint64_t m, const kmst_config* cfg,
     kmst_dev_mst o{static_cast<uint32_t*>(ctx->out_s.get(cap * 4)),
                    static_cast<uint32_t*>(ctx->out_d.get(cap * 4)),
                    static_cast<uint32_t*>(ctx->out_w.get(cap * 4)), cap};
-    if (m) {
+    if (m && c.compression == 1) {
+      // compression level 1: the records travel difference-encoded and are
+      // decoded on the device (DifferenceEncodedGraph::get_WEdges)
+      std::vector<uint8_t> bytes;
+      std::vector<uint64_t> offset;
+      std::vector<uint32_t> first;
+      kmst::encode_host(s.data(), d.data(), w.data(), m, kmst::CODEC_CHUNK, bytes, offset, first);
+      auto* db = static_cast<uint8_t*>(ctx->codec_bytes.get(bytes.size() + 16));
+      auto* dof = static_cast<uint64_t*>(ctx->codec_off.get(offset.size() * 8));
+      auto* dfi = static_cast<uint32_t*>(ctx->codec_first.get(first.size() * 4));
+      auto* derr = static_cast<uint32_t*>(ctx->tune.get(64));
+      kmst::fill_u32(ctx->stream, derr, 1, 0);
+      if (!bytes.empty())
+        KMST_HIP(hipMemcpyAsync(db, bytes.data(), bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+      KMST_HIP(hipMemcpyAsync(dof, offset.data(), offset.size() * 8, hipMemcpyHostToDevice,
+                              ctx->stream));
+      KMST_HIP(hipMemcpyAsync(dfi, first.data(), first.size() * 4, hipMemcpyHostToDevice,
+                              ctx->stream));
+      kmst::decode_device(ctx->stream, db, dof, dfi, m, kmst::CODEC_CHUNK, ds, dd, dw, derr);
+      uint32_t err = 0;
+      KMST_HIP(hipMemcpyAsync(&err, derr, 4, hipMemcpyDeviceToHost, ctx->stream));
+      KMST_HIP(hipStreamSynchronize(ctx->stream));
+      if (err) throw kmst::Error(KMST_EDEVICE, "difference-coded input failed to decode");
+    } else if (m) {
       KMST_HIP(hipMemcpyAsync(ds, s.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
       KMST_HIP(hipMemcpyAsync(dd, d.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
       KMST_HIP(hipMemcpyAsync(dw, w.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
@@ -1001,6 +1024,26 @@ void kmst_context_set_timing(kmst_context* ctx, int on) {
   if (!ctx) return;
   ctx->timing = on != 0;
   ctx->timer.reset();
+}
+
+kmst_status kmst_encode_host(const uint32_t* src, const uint32_t* dst, const uint32_t* weight,
+                             uint64_t m, uint32_t chunk_records, uint8_t* bytes, uint64_t bytes_cap,
+                             uint64_t* offsets, uint32_t* first, uint64_t* bytes_used) {
+  return guarded([&] {
+    if (!bytes_used || chunk_records == 0 || (m && (!src || !dst || !weight)))
+      throw kmst::Error(KMST_EINVAL, "null argument or zero chunk length");
+    std::vector<uint8_t> b;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> fi;
+    kmst::encode_host(src, dst, weight, m, chunk_records, b, off, fi);
+    *bytes_used = b.size();
+    if (!bytes) return;  // size query
+    if (bytes_cap < b.size() || !offsets || !first)
+      throw kmst::Error(KMST_EINVAL, "output buffers too small");
+    std::copy(b.begin(), b.end(), bytes);
+    std::copy(off.begin(), off.end(), offsets);
+    std::copy(fi.begin(), fi.end(), first);
+  });
 }
 
 void kmst_tuning_default(kmst_tuning* t) {

@@ -75,8 +75,10 @@ struct kmst_context {
   kmst::DevBuf in_s, in_d, in_w, out_s, out_d, out_w;
   // generator scratch
   kmst::DevBuf gen_a, gen_b, gen_c, gen_d;
-  // autotuner result fingerprint
+  // autotuner result fingerprint (and the codec's error word)
   kmst::DevBuf tune;
+  // difference-coded host input (compression level 1)
+  kmst::DevBuf codec_bytes, codec_off, codec_first;
 
   std::vector<kmst::DevBuf*> all_bufs();
 };

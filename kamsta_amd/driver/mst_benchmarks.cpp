@@ -65,7 +65,7 @@ void usage() {
       "mst_benchmarks [-a hybridBoruvka|filter_hybridBoruvka] [-g GNM|INFILE_WEIGHTED] [-f file]\n"
       "               [--log_n N] [--log_m M] [-i iterations] [--filter_threshold T]\n"
       "               [--max_edge_weight W] [--local_kernelization_level L]\n"
-      "               [--compression_level 0] [--seed S] [--symmetrize] [-c|--checks]");
+      "               [--compression_level 0|1] [--seed S] [--symmetrize] [-c|--checks]");
 }
 
 bool parse(int argc, char** argv, Params& p) {

@@ -44,6 +44,13 @@ def test_host_api_rejects_wide_ids_without_gpu():
     with pytest.raises(k.KmstError) as err:
         k.boruvka(e, k.AlgorithmConfig(compression=1))
     assert err.value.status == k.KMST_EINVAL
+    # compression levels beyond the difference codec are rejected before any
+    # device work
+    ok = k.to_wedges(np.array([0, 1], np.uint32), np.array([1, 0], np.uint32),
+                     np.array([5, 5], np.uint32))
+    with pytest.raises(k.KmstError) as err:
+        k.boruvka(ok, k.AlgorithmConfig(compression=2))
+    assert err.value.status == k.KMST_EINVAL
 
 
 @pytest.mark.parametrize("n,m,mw,seed", [(2, 1, 254, 0), (16, 120, 3, 1), (300, 2000, 254, 2),

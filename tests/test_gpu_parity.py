@@ -301,3 +301,16 @@ def test_contract_errors_with_weight_classes(gpu, n, m, mw):
     # the context still works after the errors
     got, got_w = k.filter_boruvka(good)
     assert got_w == exp_w and np.array_equal(strip(got), strip(exp))
+
+
+@pytest.mark.parametrize("n,m,mw,seed", [(1 << 12, 1 << 16, 254, 21), (70000, 300000, (1 << 31) - 1, 22),
+                                         (2, 1, 254, 23)])
+def test_compression_level_1(gpu, n, m, mw, seed):
+    """compression=1: the host API sends the records difference-encoded and
+    decodes them on the device; the MST is the same as without it."""
+    s, d, w = k.gen_gnm_host(n, m, mw, seed)
+    rec = k.to_wedges(s, d, w)
+    exp, exp_w = oracle.kruskal(rec)
+    for fn in (k.boruvka, k.filter_boruvka):
+        got, got_w = fn(rec, k.AlgorithmConfig(compression=1))
+        assert got_w == exp_w and np.array_equal(strip(got), strip(exp)), fn.__name__
