@@ -17,8 +17,8 @@ constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int CMP_IPT = 16;
 constexpr int CMP_TILE = BLOCK * CMP_IPT;        // 4096
-constexpr int ARG_IPT = 8;
-constexpr int ARG_TILE = BLOCK * ARG_IPT;        // 2048
+constexpr int ARG_IPT = 16;
+constexpr int ARG_TILE = BLOCK * ARG_IPT;        // 4096
 
 // Boundary piece of the segmented argmin: a vertex whose records cross a tile
 // boundary; the per-tile partial minimum is resolved in a fix-up pass.
