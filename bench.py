@@ -9,6 +9,7 @@ solve (input already in HBM, MST left in HBM): the window of the reference's
 m / T.  Prints one JSON line.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -40,6 +41,9 @@ def parse():
     p.add_argument("--no-autotune", action="store_true",
                    help="skip kmst_autotune (filter threshold and sort launch shapes) before "
                         "the timed region")
+    p.add_argument("--persistent-max", type=int, default=None,
+                   help="rounds on at most this many records run in one persistent launch "
+                        "(kmst_tuning.persistent_max; 0 = never; default: the library's)")
     p.add_argument("--cpu-log-n", type=int, default=23)
     p.add_argument("--cpu-log-m", type=int, default=27)
     p.add_argument("--no-cpu", action="store_true")
@@ -242,6 +246,8 @@ def main():
         log(f"[bench] calibration copy of {args.copy_calibration_mib} MiB done")
         del cal, cal2
     ctx = k.Context(0)
+    if args.persistent_max is not None:
+        ctx.set_tuning(dataclasses.replace(ctx.get_tuning(), persistent_max=args.persistent_max))
     t0 = time.time()
     if args.graph == "gnm":
         log(f"[bench] generating GNM n=2^{args.log_n} m=2^{args.log_m} on device")
@@ -295,6 +301,7 @@ def main():
                 "sort_threads": res["tuning"].sort_threads,
                 "balanced_digits": res["tuning"].balanced_digits,
                 "sort_wide_min": res["tuning"].sort_wide_min,
+                "persistent_max": res["tuning"].persistent_max,
                 "best_ms": round(res["best_ms"], 3), "candidates": res["candidates"],
                 "seconds": round(time.perf_counter() - t0, 2)}
         log(f"[bench] autotune: {tune}")
@@ -469,6 +476,7 @@ def main():
                  else "synthetic (host generator, seed %d)") % args.seed,
         "config": {"workload": workload,
                    "algorithm": best, "filter_threshold": cfg.filter_threshold,
+                   "persistent_max": ctx.get_tuning().persistent_max,
                    "autotune": tune,
                    "directed_records": N,
                    "mst_edges": cnt_p, "mst_weight": tw_p,

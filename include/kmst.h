@@ -143,9 +143,13 @@ typedef struct kmst_tuning {
                                 digit widths; 0: 8-bit digits, the last pass takes the rest */
   uint64_t sort_wide_min;    /* with sort_threads 0: sorts of at least this many records
                                 use 512-thread tiles */
+  uint64_t persistent_max;   /* rounds on at most this many live records (with at most
+                                twice as many vertex ids, plus 1024, and at most 32 records
+                                per vertex id or at most 2^14 records) run to the end in
+                                one persistent cooperative launch; 0 = never */
 } kmst_tuning;
 
-/* The defaults (0, 1, 2^20). */
+/* The defaults (0, 1, 2^20, 2^18). */
 void kmst_tuning_default(kmst_tuning* t);
 kmst_status kmst_context_set_tuning(kmst_context* ctx, const kmst_tuning* t);
 kmst_status kmst_context_get_tuning(const kmst_context* ctx, kmst_tuning* t);
@@ -153,7 +157,8 @@ kmst_status kmst_context_get_tuning(const kmst_context* ctx, kmst_tuning* t);
 /* Autotuner: solves the device-resident input `in` with cfg->algorithm under
  * candidate settings -- filter thresholds {2, 3, 4, 6} (filter-Boruvka only),
  * then sort tile threads {by size, 256, 512}, then digit widths {balanced,
- * 8-bit}, each stage keeping the best so far -- timing each candidate by the
+ * 8-bit}, then persistent_max {0, 2^16, 2^18, 2^20}, each stage keeping the
+ * best so far -- timing each candidate by the
  * median of `repeats` solves (at least 1) after one warm-up solve.  Every
  * candidate must return the same MST (count, total weight and an
  * order-independent hash of the edges), else KMST_EDEVICE.  Leaves the

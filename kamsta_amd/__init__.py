@@ -85,7 +85,7 @@ class _RequestResult(ctypes.Structure):
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("sort_threads", ctypes.c_uint32), ("balanced_digits", ctypes.c_uint32),
-                ("sort_wide_min", ctypes.c_uint64)]
+                ("sort_wide_min", ctypes.c_uint64), ("persistent_max", ctypes.c_uint64)]
 
 
 @dataclasses.dataclass
@@ -95,13 +95,15 @@ class Tuning:
     sort_threads: int = 0          # 256, 512, or 0 = by size
     balanced_digits: int = 1       # digit widths balanced over the passes
     sort_wide_min: int = 1 << 20   # with sort_threads 0: 512-thread tiles from here on
+    persistent_max: int = 1 << 18  # rounds on at most this many records: one persistent launch
 
     def _c(self):
-        return _Tuning(self.sort_threads, self.balanced_digits, self.sort_wide_min)
+        return _Tuning(self.sort_threads, self.balanced_digits, self.sort_wide_min,
+                       self.persistent_max)
 
     @staticmethod
     def _from(t):
-        return Tuning(t.sort_threads, t.balanced_digits, t.sort_wide_min)
+        return Tuning(t.sort_threads, t.balanced_digits, t.sort_wide_min, t.persistent_max)
 
 
 class _DevMst(ctypes.Structure):

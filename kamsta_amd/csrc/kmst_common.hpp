@@ -62,7 +62,7 @@ enum KernelKind : int {
   K_FILL, K_SCAN_REDUCE, K_SCAN_DOWN, K_VALIDATE, K_OFFSETS, K_KEYS, K_ARGMIN, K_ARGMIN_FIXUP,
   K_HOOK, K_JUMP, K_LABEL, K_GPAR, K_RELABEL_COUNT, K_RELABEL_SCATTER, K_SORT_HIST,
   K_SORT_SCATTER, K_DEDUP, K_SAMPLE, K_SPLIT_COUNT, K_SPLIT_SCATTER, K_GATHER, K_GEN,
-  K_OTHER, K_COUNT
+  K_PERSIST, K_OTHER, K_COUNT
 };
 const char* kernel_kind_name(int k);
 

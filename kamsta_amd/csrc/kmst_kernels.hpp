@@ -256,4 +256,53 @@ void encode_host(const u32* src, const u32* dst, const u32* w, u64 m, u32 chunk_
 void decode_device(hipStream_t st, const u8* bytes, const u64* offset, const u32* first, u64 m,
                    u32 chunk_records, u32* src, u32* dst, u32* w, u32* err);
 
+// Every remaining Boruvka round of a small graph in one cooperative launch
+// (kmst_persist.hip).  Round 0 reads (s0, d0, k0); round r writes its kept,
+// relabelled records to s[r & 1], d[r & 1], v[r & 1] (cap records each) and
+// the next round reads them.  Ids keep their values: a vertex whose
+// component merged points at the component's root in parent[], live roots
+// keep their own id.  The MST keys go to mst[mst_base + slot].  With gpar,
+// gpar[rep[v]] (gpar[v] without rep) takes the name of v's root every round
+// v has records.  ctl (PERSIST_CTL_WORDS, directly followed by log; both
+// zeroed by the launcher) ends with the MST edges added in ctl[4] and the
+// rounds in ctl[5]; log[2r] holds round r's records and log[2r + 1] its live
+// vertices (low half) and MST edges (high half).  One workgroup (graphs up to
+// PERSIST_ONE_BLOCK records and vertices) synchronises with its own barrier.
+constexpr u32 PERSIST_CTL_WORDS = 8;
+constexpr u64 PERSIST_MAX_DEFAULT = u64{1} << 18;
+// The persistent rounds keep parallel records (no dedup), so they are used
+// only up to this average degree, or on rounds this small; denser rounds
+// shrink faster by sorting.
+constexpr u64 PERSIST_MAX_DEGREE = 32;
+constexpr u64 PERSIST_SMALL = u64{1} << 14;
+constexpr u64 PERSIST_ONE_BLOCK = u64{1} << 12;
+constexpr u32 PERSIST_MAX_ROUNDS = 64;
+struct PersistArgs {
+  const u32* s0 = nullptr;
+  const u32* d0 = nullptr;
+  KeySrc k0{};
+  u64 m0 = 0;
+  u32* s[2] = {nullptr, nullptr};
+  u32* d[2] = {nullptr, nullptr};
+  u64* v[2] = {nullptr, nullptr};
+  u64 cap = 0;
+  u64 n = 0;
+  u64* best = nullptr;
+  u32* bdst = nullptr;
+  u32* parent = nullptr;
+  u64* mst = nullptr;
+  u64 mst_base = 0;
+  u64 mst_cap = 0;
+  const u32* rep = nullptr;
+  u32* gpar = nullptr;
+  u32* ctl = nullptr;
+  u64* log = nullptr;
+  u32* err = nullptr;
+};
+// Workgroups the persistent launch uses on `device` (one per CU), 0 where
+// cooperative launches are not supported.
+u32 persistent_blocks_max(int device);
+u32 persistent_threads();
+void persistent_rounds(hipStream_t st, const PersistArgs& a, u32 blocks);
+
 }  // namespace kmst

@@ -75,7 +75,8 @@ const char* kernel_kind_name(int k) {
                                 "keys", "argmin", "argmin_fixup", "hook", "jump", "label",
                                 "gpar", "relabel_count", "relabel_scatter", "sort_hist",
                                 "sort_scatter", "dedup", "sample", "split_count",
-                                "split_scatter", "gather", "gen", "other"};
+                                "split_scatter", "gather", "gen", "persistent_rounds",
+                                "other"};
   return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -128,7 +129,8 @@ std::vector<DevBuf*> kmst_context::all_bufs() {
           &rep,      &rep_next,   &gpar,    &seg_start, &seg_end, &mst,   &pieces,
           &tiles,    &scan_scratch, &hist,  &counters,  &lookback, &in_s, &in_d,
           &in_w,     &out_s,      &out_d,   &out_w,     &samples, &gen_a, &gen_b,
-          &gen_c,    &gen_d,      &tune,    &codec_bytes, &codec_off, &codec_first};
+          &gen_c,    &gen_d,      &tune,    &codec_bytes, &codec_off, &codec_first,
+          &persist_log};
 }
 
 namespace kmst {
@@ -220,6 +222,13 @@ class Solver {
   // alias recs.  With track_gpar every round updates the parent array.
   void rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_start = 0,
               bool rep_given = false);
+  // true when the remaining rounds of a graph with m live records and ids
+  // below n run as one persistent launch
+  bool use_persistent(u64 m, u64 n) const;
+  // every remaining round of `cur` in the persistent kernel (same contract
+  // as one pass of the rounds loop, run to the end)
+  void persistent(RecView cur, RecView A, RecView B, u64 n, bool track_gpar, bool first);
+  u64* persist_log = nullptr;
   // base case: compact sparse vertex ids first, then run the rounds
   void base_case(RecView P, RecView Q);
   void filter_rec(RecView P, RecView Q, bool stop, int depth);
@@ -277,6 +286,7 @@ void Solver::allocate() {
   scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 8 + 64));
   ctr = static_cast<u32*>(C.counters.get(256));
   samples = static_cast<u32*>(C.samples.get(4096 * 4));
+  persist_log = static_cast<u64*>(C.persist_log.get(PERSIST_CTL_WORDS * 4 + 2 * PERSIST_MAX_ROUNDS * 8));
 }
 
 void apply_tuning(kmst_context& C) {
@@ -392,7 +402,12 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
   u64 m = recs.m;
   u64 n = n_start ? n_start : n0;  // by default the first round runs in the original id space
   bool first = !rep_given;
+  bool deduped = false;  // cur holds at most n^2 live records, the rest marked dead
   while (m > 0 && n > 1) {
+    if (use_persistent(deduped ? std::min(m, n * n) : m, n)) {
+      persistent(cur, A, B, n, track_gpar, first);
+      break;
+    }
     RoundStat rs;
     rs.n = n;
     rs.m = m;
@@ -424,10 +439,96 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     if (m == 0 || n <= 1) break;
     RecView alt = (cur.s == A.s) ? B : A;
     // dense rounds (more records than n^2 / 8) carry many parallel records
-    cur = (m * 8 > n * n) ? sort_dedup(cur, alt, bits_for(n)) : sort_by_src(cur, alt, bits_for(n));
+    deduped = m * 8 > n * n;
+    cur = deduped ? sort_dedup(cur, alt, bits_for(n)) : sort_by_src(cur, alt, bits_for(n));
     first = false;
   }
   if (track_gpar) pointer_jump(st, n0, gpar, ctr + C_ERR);
+}
+
+bool Solver::use_persistent(u64 m, u64 n) const {
+  return C.tuning.persistent_max != 0 && m <= C.tuning.persistent_max && n <= 2 * m + 1024 &&
+         (m <= PERSIST_MAX_DEGREE * n || m <= PERSIST_SMALL) &&
+         persistent_blocks_max(C.device) != 0;
+}
+
+void Solver::persistent(RecView cur, RecView A, RecView B, u64 n, bool track_gpar, bool first) {
+  // outputs alternate between the two buffers that are not `cur` (B may be
+  // the records of the first round, which are consumed by then)
+  const RecView o0 = (cur.s == A.s) ? B : A;
+  const RecView o1 = (o0.s == A.s) ? B : A;
+  PersistArgs a;
+  a.s0 = cur.s;
+  a.d0 = cur.d;
+  a.k0 = keys_of(cur);
+  a.m0 = cur.m;
+  a.s[0] = o0.s;
+  a.d[0] = o0.d;
+  a.v[0] = o0.v;
+  a.s[1] = o1.s;
+  a.d[1] = o1.d;
+  a.v[1] = o1.v;
+  a.cap = cur.m;
+  a.n = n;
+  a.best = best_val;
+  a.bdst = best_dst;
+  a.parent = parent;
+  a.mst = mst;
+  a.mst_base = mst_count;
+  a.mst_cap = u64(n0) + 1;
+  a.rep = (track_gpar && !first) ? rep : nullptr;
+  a.gpar = track_gpar ? gpar : nullptr;
+  a.ctl = reinterpret_cast<u32*>(persist_log);
+  a.log = persist_log + PERSIST_CTL_WORDS / 2;
+  a.err = ctr + C_ERR;
+  // about four records or vertices per thread (one chunk per wave), at most
+  // one workgroup per CU: fewer workgroups make each grid barrier cheaper,
+  // and small graphs run in one workgroup without grid barriers
+  const u64 per_block = 4 * u64(persistent_threads());
+  const u64 size = std::max(cur.m, n);
+  const u64 want = size <= PERSIST_ONE_BLOCK ? 1 : (size + per_block - 1) / per_block;
+  const u32 blocks =
+      static_cast<u32>(std::max<u64>(1, std::min<u64>(want, persistent_blocks_max(C.device))));
+  persistent_rounds(st, a, blocks);
+  // control words and log in one copy, into the host staging words
+  KMST_HIP(hipMemcpyAsync(C.host_counters + HOST_SAMPLE_WORD, persist_log,
+                          PERSIST_CTL_WORDS * 4 + 2 * PERSIST_MAX_ROUNDS * 8,
+                          hipMemcpyDeviceToHost, st));
+  read_counters();
+  const u32* pc = C.host_counters + HOST_SAMPLE_WORD;
+  const u64* plog = reinterpret_cast<const u64*>(pc + PERSIST_CTL_WORDS);
+  if (C.host_counters[C_ERR]) {
+    // which guard stopped the launch (cycle 1, rounds 2, barrier timeout 4,
+    // a peer's error 8, unbound minimum 16, output full 32, error on entry
+    // 64) and in which round
+    std::ostringstream w;
+    w << "persistent rounds (stop bits " << pc[6] << " round " << pc[7] << ", n " << n
+      << ", m " << cur.m << ", blocks " << blocks << ", records per round";
+    for (u32 i = 0; i < std::min<u32>(pc[5] ? pc[5] : PERSIST_MAX_ROUNDS, 8); ++i)
+      w << " " << plog[2 * i] << "/" << (plog[2 * i + 1] & 0xFFFFFFFFull) << "/"
+        << (plog[2 * i + 1] >> 32);
+    w << ")";
+    check_errors(w.str().c_str());
+  }
+  const u64 added = pc[4];
+  const u32 r = std::min<u32>(pc[5], PERSIST_MAX_ROUNDS);
+  u64 log[2 * PERSIST_MAX_ROUNDS];
+  std::memcpy(log, plog, sizeof(log));
+  double bytes = 0;
+  for (u32 i = 0; i < r; ++i) {
+    RoundStat rs;
+    rs.m = log[2 * i];
+    rs.n = log[2 * i + 1] & 0xFFFFFFFFull;
+    rs.n_next = rs.n - (log[2 * i + 1] >> 32);
+    rs.m_next = (i + 1 < r) ? log[2 * i + 2] : 0;
+    rs.persistent = true;
+    C.stats.rounds.push_back(rs);
+    // per round: bind (12 B per record + 12 per source), hook and jump
+    // (about 30 B per vertex), relabel (24 B read + 16 B written per record)
+    bytes += 12.0 * double(rs.m) + 40.0 * double(rs.m) + 30.0 * double(n);
+  }
+  if (g_ktimer) g_ktimer->adjust_last(K_PERSIST, bytes);
+  mst_count += added;
 }
 
 void Solver::base_case(RecView P, RecView Q) {
@@ -949,6 +1050,13 @@ void autotune_impl(kmst_context* ctx, const kmst_dev_edges* in, kmst_config* cfg
       const double ms = measure(best_cfg, t);
       if (ms < best) best = ms, best_t = t;
     }
+    for (uint64_t pm : {uint64_t{0}, uint64_t{1} << 16, uint64_t{1} << 18, uint64_t{1} << 20}) {
+      if (pm == entry.persistent_max) continue;
+      kmst_tuning t = best_t;
+      t.persistent_max = pm;
+      const double ms = measure(best_cfg, t);
+      if (ms < best) best = ms, best_t = t;
+    }
     ctx->tuning = best_t;
     *tuning = best_t;
     cfg->filter_threshold = best_cfg.filter_threshold;
@@ -1051,6 +1159,7 @@ void kmst_tuning_default(kmst_tuning* t) {
   t->sort_threads = 0;
   t->balanced_digits = 1;
   t->sort_wide_min = kmst::SORT_WIDE_MIN;
+  t->persistent_max = kmst::PERSIST_MAX_DEFAULT;
 }
 
 kmst_status kmst_context_set_tuning(kmst_context* ctx, const kmst_tuning* t) {
@@ -1060,6 +1169,8 @@ kmst_status kmst_context_set_tuning(kmst_context* ctx, const kmst_tuning* t) {
       throw kmst::Error(KMST_EINVAL, "sort_threads must be 0, 256 or 512");
     if (t->balanced_digits > 1) throw kmst::Error(KMST_EINVAL, "balanced_digits must be 0 or 1");
     if (t->sort_wide_min == 0) throw kmst::Error(KMST_EINVAL, "sort_wide_min must be positive");
+    if (t->persistent_max >= (uint64_t{1} << 32))
+      throw kmst::Error(KMST_EINVAL, "persistent_max must be below 2^32");
     ctx->tuning = *t;
   });
 }
@@ -1164,7 +1275,8 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
   for (size_t i = 0; i < s.rounds.size(); ++i) {
     const auto& r = s.rounds[i];
     o << (i ? "," : "") << "{\"n\":" << r.n << ",\"m\":" << r.m << ",\"n_next\":" << r.n_next
-      << ",\"m_next\":" << r.m_next << "}";
+      << ",\"m_next\":" << r.m_next << ",\"persistent\":" << (r.persistent ? "true" : "false")
+      << "}";
   }
   o << "],\"filter\":[";
   for (size_t i = 0; i < s.filter.size(); ++i) {

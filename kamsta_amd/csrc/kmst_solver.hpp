@@ -20,6 +20,7 @@ struct DevBuf {
 
 struct RoundStat {
   u64 n = 0, m = 0, n_next = 0, m_next = 0;
+  bool persistent = false;  // ran inside the persistent launch
 };
 struct FilterStat {
   int depth = 0;
@@ -41,10 +42,15 @@ struct SolveStats {
 
 extern thread_local std::string g_last_error;
 
-// pinned host words: counters in [0, 64), vertex samples from HOST_SAMPLE_WORD
+// pinned host words: counters in [0, 64), then HOST_STAGE_WORDS of staging
+// (vertex samples, or the persistent rounds' control words and log)
 constexpr u32 GPAR_SAMPLES = 256;
 constexpr u32 HOST_SAMPLE_WORD = 64;
-constexpr size_t HOST_COUNTER_BYTES = (HOST_SAMPLE_WORD + GPAR_SAMPLES) * 4;
+constexpr u32 HOST_STAGE_WORDS = 512;
+static_assert(GPAR_SAMPLES <= HOST_STAGE_WORDS, "sample staging");
+static_assert(PERSIST_CTL_WORDS + 4 * PERSIST_MAX_ROUNDS <= HOST_STAGE_WORDS,
+              "persistent-round staging");
+constexpr size_t HOST_COUNTER_BYTES = (HOST_SAMPLE_WORD + HOST_STAGE_WORDS) * 4;
 
 }  // namespace kmst
 
@@ -67,10 +73,10 @@ struct kmst_context {
   kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, seg_start, seg_end,
       mst;
   // kernel scratch
-  kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback;
+  kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback, persist_log;
   kmst::SortWork sort_work;  // radix sort state over `hist`
   kmst::LookbackWork lb_work;  // compaction look-back state over `lookback`
-  kmst_tuning tuning{0, 1, kmst::SORT_WIDE_MIN};  // launch shapes of later solves
+  kmst_tuning tuning{0, 1, kmst::SORT_WIDE_MIN, kmst::PERSIST_MAX_DEFAULT};  // launch shapes of later solves
   // host-API staging
   kmst::DevBuf in_s, in_d, in_w, out_s, out_d, out_w;
   // generator scratch

@@ -128,4 +128,5 @@ def test_tuning_default_matches_python_mirror():
     t = k._Tuning()
     k.load().kmst_tuning_default(ctypes.byref(t))
     assert k.Tuning._from(t) == k.Tuning()
-    assert (t.sort_threads, t.balanced_digits, t.sort_wide_min) == (0, 1, 1 << 20)
+    assert (t.sort_threads, t.balanced_digits, t.sort_wide_min, t.persistent_max) == \
+        (0, 1, 1 << 20, 1 << 18)

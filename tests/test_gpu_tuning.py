@@ -65,7 +65,8 @@ def test_autotune_picks_a_checked_tuning(gpu, ctx, gnm):
     try:
         res = ctx.autotune(s, d, w, 2 * m, n, algorithm=k.FILTER_BORUVKA, repeats=1)
         # default + 3 other thresholds + 2 other sort widths + the other digit plan
-        assert res["candidates"] == 7
+        # + 3 other persistent-round limits
+        assert res["candidates"] == 10
         assert res["filter_threshold"] in (2, 3, 4, 6)
         assert res["tuning"].sort_threads in (0, 256, 512)
         assert res["best_ms"] > 0
