@@ -127,8 +127,11 @@ def serving(args):
     gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
 
     def run(max_batch_requests):
+        tuning = None
+        if args.persistent_max is not None:
+            tuning = dataclasses.replace(k.Tuning(), persistent_max=args.persistent_max)
         srv = k.Server(algorithm=k.FILTER_BORUVKA, filter_threshold=args.filter_threshold,
-                       max_batch_requests=max_batch_requests)
+                       max_batch_requests=max_batch_requests, tuning=tuning)
         try:
             outp = torch.empty((3, out_total), dtype=torch.int32, pin_memory=True)
         except Exception:
@@ -202,7 +205,9 @@ def serving(args):
                 + ("inputs resident in HBM (latency includes D2H of results)" if args.device_inputs
                    else f"inputs in {host_memory} host memory (latency includes H2D and D2H)"),
         "config": {"workload": "batched serving", "requests": args.requests, "rate": args.rate,
-                   "directed_records": total, "algorithm": "filter_boruvka"},
+                   "directed_records": total, "algorithm": "filter_boruvka",
+                   "persistent_max": (args.persistent_max if args.persistent_max is not None
+                                      else k.Tuning().persistent_max)},
         "batched": {kk: (round(v, 3) if isinstance(v, float) else v) for kk, v in batched.items()},
         "one_graph_per_launch": {kk: (round(v, 3) if isinstance(v, float) else v)
                                  for kk, v in alone.items()},

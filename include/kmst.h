@@ -226,6 +226,7 @@ typedef struct kmst_server_config {
   uint64_t hbm_budget_bytes;     /* 0: half of the free device memory */
   uint32_t queue_capacity;       /* queued requests before submit returns KMST_EBUSY */
   uint32_t buffer_sets;          /* batches in flight (upload / compute / delivery) */
+  kmst_tuning tuning;            /* launch shapes of the server's solves (kmst_tuning rules) */
 } kmst_server_config;
 
 typedef struct kmst_request_result {

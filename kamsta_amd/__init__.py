@@ -68,11 +68,17 @@ class _DevEdges(ctypes.Structure):
                 ("m_directed", ctypes.c_uint64), ("n", ctypes.c_uint64)]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("sort_threads", ctypes.c_uint32), ("balanced_digits", ctypes.c_uint32),
+                ("sort_wide_min", ctypes.c_uint64), ("persistent_max", ctypes.c_uint64)]
+
+
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("algorithm", ctypes.c_int32),
                 ("filter_threshold", ctypes.c_uint64), ("max_batch_records", ctypes.c_uint64),
                 ("max_batch_requests", ctypes.c_uint64), ("hbm_budget_bytes", ctypes.c_uint64),
-                ("queue_capacity", ctypes.c_uint32), ("buffer_sets", ctypes.c_uint32)]
+                ("queue_capacity", ctypes.c_uint32), ("buffer_sets", ctypes.c_uint32),
+                ("tuning", _Tuning)]
 
 
 class _RequestResult(ctypes.Structure):
@@ -81,11 +87,6 @@ class _RequestResult(ctypes.Structure):
                 ("batch_requests", ctypes.c_uint64), ("submit_ns", ctypes.c_uint64),
                 ("admit_ns", ctypes.c_uint64), ("start_ns", ctypes.c_uint64),
                 ("solved_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64)]
-
-
-class _Tuning(ctypes.Structure):
-    _fields_ = [("sort_threads", ctypes.c_uint32), ("balanced_digits", ctypes.c_uint32),
-                ("sort_wide_min", ctypes.c_uint64), ("persistent_max", ctypes.c_uint64)]
 
 
 @dataclasses.dataclass
@@ -374,11 +375,13 @@ class Server:
 
     def __init__(self, device=0, algorithm=FILTER_BORUVKA, filter_threshold=4,
                  max_batch_records=None, max_batch_requests=None, hbm_budget_bytes=0,
-                 queue_capacity=None, buffer_sets=3):
+                 queue_capacity=None, buffer_sets=3, tuning=None):
         self._lib = load()
         cfg = _ServerConfig()
         self._lib.kmst_server_config_default(ctypes.byref(cfg))
         cfg.device, cfg.algorithm, cfg.filter_threshold = device, algorithm, filter_threshold
+        if tuning is not None:
+            cfg.tuning = tuning._c()
         if max_batch_records is not None:
             cfg.max_batch_records = max_batch_records
         if max_batch_requests is not None:

@@ -486,6 +486,7 @@ void kmst_server_config_default(kmst_server_config* c) {
   c->hbm_budget_bytes = 0;
   c->queue_capacity = 1u << 20;
   c->buffer_sets = 3;
+  kmst_tuning_default(&c->tuning);
 }
 
 kmst_status kmst_server_create(const kmst_server_config* cfg, kmst_server** out) {
@@ -507,6 +508,8 @@ kmst_status kmst_server_create(const kmst_server_config* cfg, kmst_server** out)
     const u64 max_rec = s->cfg.hbm_budget_bytes / BYTES_PER_RECORD;
     s->cfg.max_batch_records = std::min<u64>(s->cfg.max_batch_records, max_rec);
     s->ctx.reset(new kmst_context(cfg->device));
+    if (kmst_context_set_tuning(s->ctx.get(), &cfg->tuning) != KMST_OK)
+      throw kmst::Error(KMST_EINVAL, std::string("server tuning: ") + kmst_last_error());
     // reserve the solver workspaces and upload buffers for full batches up
     // front (admission by footprint: a batch never grows them)
     const u64 n_res = std::max<u64>(s->cfg.max_batch_records / 4, 1024);

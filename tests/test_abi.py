@@ -130,3 +130,10 @@ def test_tuning_default_matches_python_mirror():
     assert k.Tuning._from(t) == k.Tuning()
     assert (t.sort_threads, t.balanced_digits, t.sort_wide_min, t.persistent_max) == \
         (0, 1, 1 << 20, 1 << 18)
+
+
+def test_server_config_default_carries_the_default_tuning():
+    c = k._ServerConfig()
+    k.load().kmst_server_config_default(ctypes.byref(c))
+    assert k.Tuning._from(c.tuning) == k.Tuning()
+    assert (c.algorithm, c.filter_threshold, c.buffer_sets) == (k.FILTER_BORUVKA, 4, 3)
