@@ -523,9 +523,11 @@ void Solver::persistent(RecView cur, RecView A, RecView B, u64 n, bool track_gpa
     rs.m_next = (i + 1 < r) ? log[2 * i + 2] : 0;
     rs.persistent = true;
     C.stats.rounds.push_back(rs);
-    // per round: bind (12 B per record + 12 per source), hook and jump
-    // (about 30 B per vertex), relabel (24 B read + 16 B written per record)
-    bytes += 12.0 * double(rs.m) + 40.0 * double(rs.m) + 30.0 * double(n);
+    // per round: bind (16 B per record + its source's 8-byte minimum),
+    // relabel (8 B of ids, two 4-byte label gathers, the 8-byte key; 16 B per
+    // kept record written), hook and jump (about 30 B per vertex)
+    bytes += 24.0 * double(rs.m) + 24.0 * double(rs.m) + 16.0 * double(rs.m_next) +
+             30.0 * double(n);
   }
   if (g_ktimer) g_ktimer->adjust_last(K_PERSIST, bytes);
   mst_count += added;
