@@ -265,8 +265,10 @@ void decode_device(hipStream_t st, const u8* bytes, const u64* offset, const u32
 // gpar[rep[v]] (gpar[v] without rep) takes the name of v's root every round
 // v has records.  ctl (PERSIST_CTL_WORDS, directly followed by log; both
 // zeroed by the launcher) ends with the MST edges added in ctl[4] and the
-// rounds in ctl[5]; log[2r] holds round r's records and log[2r + 1] its live
-// vertices (low half) and MST edges (high half).  One workgroup (graphs up to
+// rounds in ctl[5]; round r's PERSIST_LOG_WORDS log words hold its records,
+// its live vertices (low half) and MST edges (high half), and the wall clock
+// (s_memrealtime ticks) of workgroup 0 at the start of the round and at the end
+// of its bind, hook, jump and relabel phases.  One workgroup (graphs up to
 // PERSIST_ONE_BLOCK records and vertices) synchronises with its own barrier.
 constexpr u32 PERSIST_CTL_WORDS = 8;
 constexpr u64 PERSIST_MAX_DEFAULT = u64{1} << 18;
@@ -277,6 +279,10 @@ constexpr u64 PERSIST_MAX_DEGREE = 32;
 constexpr u64 PERSIST_SMALL = u64{1} << 14;
 constexpr u64 PERSIST_ONE_BLOCK = u64{1} << 12;
 constexpr u32 PERSIST_MAX_ROUNDS = 64;
+constexpr u32 PERSIST_LOG_WORDS = 7;
+constexpr u32 PERSIST_LOG_TOTAL = PERSIST_LOG_WORDS * PERSIST_MAX_ROUNDS;
+// ticks of the wall clock the persistent kernel stamps, per microsecond
+double persistent_ticks_per_us(int device);
 struct PersistArgs {
   const u32* s0 = nullptr;
   const u32* d0 = nullptr;

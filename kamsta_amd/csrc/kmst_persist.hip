@@ -116,8 +116,39 @@ __device__ __forceinline__ void offer_min(u64* best, u32 s, u64 k) {
   }
 }
 
+// Workgroup-aggregated reservation of output slots: every wave passes its
+// count (wave-uniform) and gets its first slot; one atomic per workgroup and
+// call (per-wave atomics on one counter serialise).  Called by all threads;
+// the two LDS sets alternate, so a wave that runs ahead into the next call
+// never overwrites values another wave has yet to read.
+struct Reserve {
+  u32 cnt[2][PW];
+  u32 base[2];
+};
+__device__ __forceinline__ u32 block_reserve(u32 wave_count, u32* counter, Reserve& s,
+                                             u32& parity) {
+  const int wid = threadIdx.x / WAVE;
+  if (lane_id() == 0) s.cnt[parity][wid] = wave_count;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 t = 0;
+    for (int w = 0; w < PW; ++w) {
+      const u32 c = s.cnt[parity][w];
+      s.cnt[parity][w] = t;
+      t += c;
+    }
+    s.base[parity] = t ? atomicAdd(counter, t) : 0u;
+  }
+  __syncthreads();
+  const u32 first = s.base[parity] + s.cnt[parity][wid];
+  parity ^= 1u;
+  return first;
+}
+
 __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
   __shared__ u32 s_live[PW], s_emit[PW];
+  __shared__ Reserve s_res;
+  u32 parity = 0;
   const int lane = lane_id(), wid = threadIdx.x / WAVE;
   const u64 gtid = u64(blockIdx.x) * PB + threadIdx.x, nthr = u64(gridDim.x) * PB;
   const u64 gwave = u64(blockIdx.x) * PW + wid, nwaves = u64(gridDim.x) * PW;
@@ -158,7 +189,12 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
   KeySrc keys = a.k0;
   u64 m = a.m0;
   while (true) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.log[2 * r] = m;
+    u64* lg = a.log + PERSIST_LOG_WORDS * r;
+    const bool stamp = blockIdx.x == 0 && threadIdx.x == 0;
+    if (stamp) {
+      lg[0] = m;
+      lg[2] = wall_clock64();
+    }
     // B: the other endpoint of every vertex's minimum
     for (u64 i0 = gtid; i0 < m; i0 += PU * nthr) {
       u32 ss[PU], dd[PU];
@@ -181,10 +217,12 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
     }
     if (!grid_sync(bar, target, a.err, r)) return;
 
-    // H: hook, MST emission
+    if (stamp) lg[3] = wall_clock64();
+    // H: hook, MST emission (workgroup-uniform loop: one slot reservation
+    // per workgroup and step)
     u32 live = 0, emit = 0;
-    for (u64 base = gwave * WAVE; base < n; base += nwaves * WAVE) {
-      const u64 v = base + lane;
+    for (u64 bb = u64(blockIdx.x) * PB; bb < n; bb += nthr) {
+      const u64 v = bb + threadIdx.x;
       bool em = false;
       u64 bv = VAL_NONE;
       if (v < n) {
@@ -204,9 +242,11 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
           ++live;
         }
       }
-      const u32 slot = wave_append(em, a.ctl + PC_MST);
+      const u64 emask = __ballot(em);
+      const u32 first =
+          block_reserve(static_cast<u32>(__popcll(emask)), a.ctl + PC_MST, s_res, parity);
       if (em) {
-        const u64 at = a.mst_base + slot;
+        const u64 at = a.mst_base + first + mbcnt(emask);
         if (at < a.mst_cap)
           a.mst[at] = bv;
         else
@@ -234,11 +274,12 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
           et += s_emit[w];
         }
         if (lt | et)
-          atomicAdd(reinterpret_cast<unsigned long long*>(a.log + 2 * r + 1),
+          atomicAdd(reinterpret_cast<unsigned long long*>(lg + 1),
                     static_cast<unsigned long long>((et << 32) | lt));
       }
     }
     if (!grid_sync(bar, target, a.err, r)) return;
+    if (stamp) lg[4] = wall_clock64();
 
     // J: pointer jumping; the minima are cleared for the next round
     for (u64 v = gtid; v < n; v += nthr) {
@@ -269,6 +310,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
     // barriers ago at the latest
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[PC_M + (r + 1) % 3] = 0;
     if (!grid_sync(bar, target, a.err, r)) return;
+    if (stamp) lg[5] = wall_clock64();
 
     // R: relabel, drop self-loops, compact (each wave's chunk of PU rows
     // keeps its order), next minima
@@ -277,7 +319,9 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
     u32* od = a.d[ob];
     u64* ov = a.v[ob];
     u32* cnt = a.ctl + PC_M + r % 3;
-    for (u64 base = gwave * PCHUNK; base < m; base += nwaves * PCHUNK) {
+    for (u64 bb = u64(blockIdx.x) * (PB * PU); bb < m; bb += nthr * PU) {
+      // workgroup-uniform steps of PW chunks; this wave takes chunk wid
+      const u64 base = bb + u64(wid) * PCHUNK;
       u32 ss[PU], dd[PU], ns[PU], nd[PU];
       u64 kk[PU], bal[PU];
 #pragma unroll
@@ -303,9 +347,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
         total += static_cast<u32>(__popcll(bal[u]));
         if (!keep) ns[u] = NONE32;
       }
-      u32 at0 = 0;
-      if (lane == 0 && total) at0 = atomicAdd(cnt, total);
-      at0 = __shfl(at0, 0, WAVE);
+      u32 at0 = block_reserve(total, cnt, s_res, parity);
 #pragma unroll
       for (int u = 0; u < PU; ++u) {
         if (kk[u] != VAL_NONE) {
@@ -325,6 +367,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
       for (int u = 0; u < PU; ++u) offer_min(a.best, ns[u], kk[u]);
     }
     if (!grid_sync(bar, target, a.err, r)) return;
+    if (stamp) lg[6] = wall_clock64();
 
     m = __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     in_s = os;
@@ -364,11 +407,22 @@ u32 persistent_blocks_max(int device) {
 
 u32 persistent_threads() { return PB; }
 
+double persistent_ticks_per_us(int device) {
+  static int cached_dev = -1;
+  static double cached = 0;
+  if (cached_dev == device) return cached;
+  int khz = 0;
+  KMST_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+  cached = khz > 0 ? khz / 1000.0 : 100.0;
+  cached_dev = device;
+  return cached;
+}
+
 void persistent_rounds(hipStream_t st, const PersistArgs& a, u32 blocks) {
   if (blocks == 0) throw Error(KMST_EDEVICE, "persistent rounds: no cooperative launch");
   if (reinterpret_cast<u64*>(a.ctl) + PERSIST_CTL_WORDS / 2 != a.log)
     throw Error(KMST_EINVAL, "persistent rounds: ctl must directly precede log");
-  fill_u64(st, reinterpret_cast<u64*>(a.ctl), PERSIST_CTL_WORDS / 2 + 2 * PERSIST_MAX_ROUNDS, 0);
+  fill_u64(st, reinterpret_cast<u64*>(a.ctl), PERSIST_CTL_WORDS / 2 + PERSIST_LOG_TOTAL, 0);
   KTScope t(st, K_PERSIST, 0.0);
   if (blocks == 1) {
     // one workgroup has no grid barrier, so it needs no co-residency

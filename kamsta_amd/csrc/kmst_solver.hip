@@ -286,7 +286,7 @@ void Solver::allocate() {
   scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 8 + 64));
   ctr = static_cast<u32*>(C.counters.get(256));
   samples = static_cast<u32*>(C.samples.get(4096 * 4));
-  persist_log = static_cast<u64*>(C.persist_log.get(PERSIST_CTL_WORDS * 4 + 2 * PERSIST_MAX_ROUNDS * 8));
+  persist_log = static_cast<u64*>(C.persist_log.get(PERSIST_CTL_WORDS * 4 + PERSIST_LOG_TOTAL * 8));
 }
 
 void apply_tuning(kmst_context& C) {
@@ -492,7 +492,7 @@ void Solver::persistent(RecView cur, RecView A, RecView B, u64 n, bool track_gpa
   persistent_rounds(st, a, blocks);
   // control words and log in one copy, into the host staging words
   KMST_HIP(hipMemcpyAsync(C.host_counters + HOST_SAMPLE_WORD, persist_log,
-                          PERSIST_CTL_WORDS * 4 + 2 * PERSIST_MAX_ROUNDS * 8,
+                          PERSIST_CTL_WORDS * 4 + PERSIST_LOG_TOTAL * 8,
                           hipMemcpyDeviceToHost, st));
   read_counters();
   const u32* pc = C.host_counters + HOST_SAMPLE_WORD;
@@ -504,24 +504,26 @@ void Solver::persistent(RecView cur, RecView A, RecView B, u64 n, bool track_gpa
     std::ostringstream w;
     w << "persistent rounds (stop bits " << pc[6] << " round " << pc[7] << ", n " << n
       << ", m " << cur.m << ", blocks " << blocks << ", records per round";
-    for (u32 i = 0; i < std::min<u32>(pc[5] ? pc[5] : PERSIST_MAX_ROUNDS, 8); ++i)
-      w << " " << plog[2 * i] << "/" << (plog[2 * i + 1] & 0xFFFFFFFFull) << "/"
-        << (plog[2 * i + 1] >> 32);
+    for (u32 i = 0; i < std::min<u32>(pc[5] ? pc[5] : PERSIST_MAX_ROUNDS, 8); ++i) {
+      const u64* lg = plog + PERSIST_LOG_WORDS * i;
+      w << " " << lg[0] << "/" << (lg[1] & 0xFFFFFFFFull) << "/" << (lg[1] >> 32);
+    }
     w << ")";
     check_errors(w.str().c_str());
   }
   const u64 added = pc[4];
   const u32 r = std::min<u32>(pc[5], PERSIST_MAX_ROUNDS);
-  u64 log[2 * PERSIST_MAX_ROUNDS];
-  std::memcpy(log, plog, sizeof(log));
+  const double tpu = persistent_ticks_per_us(C.device);
   double bytes = 0;
   for (u32 i = 0; i < r; ++i) {
+    const u64* lg = plog + PERSIST_LOG_WORDS * i;
     RoundStat rs;
-    rs.m = log[2 * i];
-    rs.n = log[2 * i + 1] & 0xFFFFFFFFull;
-    rs.n_next = rs.n - (log[2 * i + 1] >> 32);
-    rs.m_next = (i + 1 < r) ? log[2 * i + 2] : 0;
+    rs.m = lg[0];
+    rs.n = lg[1] & 0xFFFFFFFFull;
+    rs.n_next = rs.n - (lg[1] >> 32);
+    rs.m_next = (i + 1 < r) ? lg[PERSIST_LOG_WORDS] : 0;
     rs.persistent = true;
+    for (int p = 0; p < 4; ++p) rs.phase_us[p] = double(lg[3 + p] - lg[2 + p]) / tpu;
     C.stats.rounds.push_back(rs);
     // per round: bind (16 B per record + its source's 8-byte minimum),
     // relabel (8 B of ids, two 4-byte label gathers, the 8-byte key; 16 B per
@@ -1277,8 +1279,11 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
   for (size_t i = 0; i < s.rounds.size(); ++i) {
     const auto& r = s.rounds[i];
     o << (i ? "," : "") << "{\"n\":" << r.n << ",\"m\":" << r.m << ",\"n_next\":" << r.n_next
-      << ",\"m_next\":" << r.m_next << ",\"persistent\":" << (r.persistent ? "true" : "false")
-      << "}";
+      << ",\"m_next\":" << r.m_next << ",\"persistent\":" << (r.persistent ? "true" : "false");
+    if (r.persistent)
+      o << ",\"phase_us\":[" << r.phase_us[0] << "," << r.phase_us[1] << "," << r.phase_us[2]
+        << "," << r.phase_us[3] << "]";
+    o << "}";
   }
   o << "],\"filter\":[";
   for (size_t i = 0; i < s.filter.size(); ++i) {

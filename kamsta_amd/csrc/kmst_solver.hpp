@@ -21,6 +21,7 @@ struct DevBuf {
 struct RoundStat {
   u64 n = 0, m = 0, n_next = 0, m_next = 0;
   bool persistent = false;  // ran inside the persistent launch
+  double phase_us[4] = {0, 0, 0, 0};  // persistent: bind, hook, jump, relabel
 };
 struct FilterStat {
   int depth = 0;
@@ -46,9 +47,9 @@ extern thread_local std::string g_last_error;
 // (vertex samples, or the persistent rounds' control words and log)
 constexpr u32 GPAR_SAMPLES = 256;
 constexpr u32 HOST_SAMPLE_WORD = 64;
-constexpr u32 HOST_STAGE_WORDS = 512;
+constexpr u32 HOST_STAGE_WORDS = 1024;
 static_assert(GPAR_SAMPLES <= HOST_STAGE_WORDS, "sample staging");
-static_assert(PERSIST_CTL_WORDS + 4 * PERSIST_MAX_ROUNDS <= HOST_STAGE_WORDS,
+static_assert(PERSIST_CTL_WORDS + 2 * PERSIST_LOG_TOTAL <= HOST_STAGE_WORDS,
               "persistent-round staging");
 constexpr size_t HOST_COUNTER_BYTES = (HOST_SAMPLE_WORD + HOST_STAGE_WORDS) * 4;
 
