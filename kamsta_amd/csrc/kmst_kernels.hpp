@@ -281,6 +281,8 @@ constexpr u64 PERSIST_ONE_BLOCK = u64{1} << 12;
 constexpr u32 PERSIST_MAX_ROUNDS = 64;
 constexpr u32 PERSIST_LOG_WORDS = 7;
 constexpr u32 PERSIST_LOG_TOTAL = PERSIST_LOG_WORDS * PERSIST_MAX_ROUNDS;
+// grid-barrier group counters (u32 words) that directly follow log
+constexpr u32 PERSIST_GROUP_WORDS = 16 * 32;
 // ticks of the wall clock the persistent kernel stamps, per microsecond
 double persistent_ticks_per_us(int device);
 struct PersistArgs {
@@ -303,6 +305,7 @@ struct PersistArgs {
   u32* gpar = nullptr;
   u32* ctl = nullptr;
   u64* log = nullptr;
+  u32* groups = nullptr;
   u32* err = nullptr;
 };
 // Workgroups the persistent launch uses on `device` (one per CU), 0 where

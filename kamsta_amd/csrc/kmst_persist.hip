@@ -55,28 +55,39 @@ __device__ __forceinline__ void why(u32* ctl, u32 bit, u32 r) {
   atomicMax(ctl + PC_WHY_ROUND, r);
 }
 
-// All workgroups arrive, then all leave.  The barrier counter only grows, so
-// the k-th barrier waits for k * gridDim.x arrivals.  Every wave first waits
+// All workgroups arrive, then all leave.  Arrivals go to one of
+// ceil(G / BAR_GROUP) group counters (128 bytes apart) and the last arrival of
+// a group adds one to the root counter, so no address takes more than
+// BAR_GROUP arrivals (same-address atomics serialise).  Counters only grow:
+// after the k-th barrier a group counter holds k * (its workgroups) and the
+// root k * (groups).  Every wave first waits
 // until its own global stores have completed (the workgroup barrier waits for
 // LDS traffic only), then thread 0 releases the CU's writes with an
 // agent-scope fence, arrives, waits, and acquires for the whole CU before the
 // workgroup leaves.  The wait is bounded by the wall clock; on expiry, or
 // when another workgroup has already raised ERR_SPIN, the barrier fails and
 // the caller leaves the kernel, so every workgroup ends.
-__device__ __forceinline__ bool grid_sync(u32* bar, u32& target, u32* err, u32 r) {
+constexpr u32 BAR_GROUP = 16;
+constexpr u32 BAR_GROUP_STRIDE = 32;  // u32 words between group counters
+constexpr u32 BAR_GROUPS_MAX = 16;    // up to 256 workgroups
+static_assert(BAR_GROUPS_MAX * BAR_GROUP_STRIDE == PERSIST_GROUP_WORDS, "group counter area");
+__device__ __forceinline__ bool grid_sync(u32* bar, u32* groups, u32& k, u32* err, u32 r) {
   __shared__ u32 s_ok;
   if (gridDim.x == 1) {
     // one workgroup: its barrier orders its own global accesses
     __syncthreads();
     return true;
   }
-  target += gridDim.x;
+  ++k;
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are done
   __syncthreads();
   if (threadIdx.x == 0) {
     u32 ok = 1;
     __threadfence();
-    atomicAdd(bar, 1u);
+    const u32 G = gridDim.x, g = blockIdx.x / BAR_GROUP;
+    const u32 in_group = min(BAR_GROUP, G - g * BAR_GROUP);
+    if (atomicAdd(groups + g * BAR_GROUP_STRIDE, 1u) + 1 == k * in_group) atomicAdd(bar, 1u);
+    const u32 target = k * ((G + BAR_GROUP - 1) / BAR_GROUP);
     const u64 t0 = wall_clock64();
     while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
       const bool peer =
@@ -163,7 +174,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
     a.best[v] = VAL_NONE;
     a.bdst[v] = NONE32;
   }
-  if (!grid_sync(bar, target, a.err, r)) return;
+  if (!grid_sync(bar, a.groups, target, a.err, r)) return;
   // minima of the first round
   for (u64 base = gwave * PCHUNK; base < a.m0; base += nwaves * PCHUNK) {
     u32 ss[PU], dd[PU];
@@ -182,7 +193,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
 #pragma unroll
     for (int u = 0; u < PU; ++u) offer_min(a.best, ss[u], kk[u]);
   }
-  if (!grid_sync(bar, target, a.err, r)) return;
+  if (!grid_sync(bar, a.groups, target, a.err, r)) return;
 
   const u32* in_s = a.s0;
   const u32* in_d = a.d0;
@@ -215,7 +226,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
       for (int u = 0; u < PU; ++u)
         if (kk[u] != VAL_NONE && ss[u] != dd[u] && kk[u] == bb[u]) a.bdst[ss[u]] = dd[u];
     }
-    if (!grid_sync(bar, target, a.err, r)) return;
+    if (!grid_sync(bar, a.groups, target, a.err, r)) return;
 
     if (stamp) lg[3] = wall_clock64();
     // H: hook, MST emission (workgroup-uniform loop: one slot reservation
@@ -278,7 +289,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
                     static_cast<unsigned long long>((et << 32) | lt));
       }
     }
-    if (!grid_sync(bar, target, a.err, r)) return;
+    if (!grid_sync(bar, a.groups, target, a.err, r)) return;
     if (stamp) lg[4] = wall_clock64();
 
     // J: pointer jumping; the minima are cleared for the next round
@@ -309,7 +320,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
     // the count slot of the next round is free: every workgroup read it two
     // barriers ago at the latest
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[PC_M + (r + 1) % 3] = 0;
-    if (!grid_sync(bar, target, a.err, r)) return;
+    if (!grid_sync(bar, a.groups, target, a.err, r)) return;
     if (stamp) lg[5] = wall_clock64();
 
     // R: relabel, drop self-loops, compact (each wave's chunk of PU rows
@@ -366,7 +377,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
 #pragma unroll
       for (int u = 0; u < PU; ++u) offer_min(a.best, ns[u], kk[u]);
     }
-    if (!grid_sync(bar, target, a.err, r)) return;
+    if (!grid_sync(bar, a.groups, target, a.err, r)) return;
     if (stamp) lg[6] = wall_clock64();
 
     m = __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -422,7 +433,12 @@ void persistent_rounds(hipStream_t st, const PersistArgs& a, u32 blocks) {
   if (blocks == 0) throw Error(KMST_EDEVICE, "persistent rounds: no cooperative launch");
   if (reinterpret_cast<u64*>(a.ctl) + PERSIST_CTL_WORDS / 2 != a.log)
     throw Error(KMST_EINVAL, "persistent rounds: ctl must directly precede log");
-  fill_u64(st, reinterpret_cast<u64*>(a.ctl), PERSIST_CTL_WORDS / 2 + PERSIST_LOG_TOTAL, 0);
+  if (a.log + PERSIST_LOG_TOTAL != reinterpret_cast<u64*>(a.groups))
+    throw Error(KMST_EINVAL, "persistent rounds: the barrier groups must directly follow log");
+  if (blocks > BAR_GROUP * BAR_GROUPS_MAX)
+    throw Error(KMST_EINVAL, "persistent rounds: too many workgroups");
+  fill_u64(st, reinterpret_cast<u64*>(a.ctl),
+           PERSIST_CTL_WORDS / 2 + PERSIST_LOG_TOTAL + PERSIST_GROUP_WORDS / 2, 0);
   KTScope t(st, K_PERSIST, 0.0);
   if (blocks == 1) {
     // one workgroup has no grid barrier, so it needs no co-residency

@@ -286,7 +286,8 @@ void Solver::allocate() {
   scan_scratch = static_cast<u32*>(C.scan_scratch.get(scan_scratch_elems(scan_max) * 8 + 64));
   ctr = static_cast<u32*>(C.counters.get(256));
   samples = static_cast<u32*>(C.samples.get(4096 * 4));
-  persist_log = static_cast<u64*>(C.persist_log.get(PERSIST_CTL_WORDS * 4 + PERSIST_LOG_TOTAL * 8));
+  persist_log = static_cast<u64*>(
+      C.persist_log.get(PERSIST_CTL_WORDS * 4 + PERSIST_LOG_TOTAL * 8 + PERSIST_GROUP_WORDS * 4));
 }
 
 void apply_tuning(kmst_context& C) {
@@ -480,6 +481,7 @@ void Solver::persistent(RecView cur, RecView A, RecView B, u64 n, bool track_gpa
   a.gpar = track_gpar ? gpar : nullptr;
   a.ctl = reinterpret_cast<u32*>(persist_log);
   a.log = persist_log + PERSIST_CTL_WORDS / 2;
+  a.groups = reinterpret_cast<u32*>(a.log + PERSIST_LOG_TOTAL);
   a.err = ctr + C_ERR;
   // about four records or vertices per thread (one chunk per wave), at most
   // one workgroup per CU: fewer workgroups make each grid barrier cheaper,
