@@ -396,8 +396,10 @@ class Server:
 
     def submit(self, src, dst, weight, n, out=None):
         """src/dst/weight: numpy uint32 arrays (host) or torch tensors (host
-        or device); out: optional three uint32 numpy arrays of >= max(n, 1)
-        entries (e.g. views of a pinned pool) that receive the result."""
+        or device); out: optional three arrays of >= max(n, 1) 32-bit entries
+        that receive the result: numpy arrays (e.g. views of a pinned pool)
+        or torch tensors (pinned host or device; the server's kernel writes
+        device-writable outputs directly)."""
         def arr(x):
             if hasattr(x, "data_ptr"):
                 return x
@@ -413,7 +415,7 @@ class Server:
         t = ctypes.c_uint64(0)
         _check(self._lib.kmst_server_submit(
             self._h, ptr(src), ptr(dst), ptr(weight), len(src), int(n),
-            outs[0].ctypes.data, outs[1].ctypes.data, outs[2].ctypes.data, cap, ctypes.byref(t)))
+            ptr(outs[0]), ptr(outs[1]), ptr(outs[2]), cap, ctypes.byref(t)))
         self._live[t.value] = (src, dst, weight, outs)
         return t.value
 

@@ -16,8 +16,10 @@
 //              budget, uploads them (one async copy per array, pinned host
 //              memory goes at link speed) on the copy stream;
 //   compute    adds the vertex-id offsets and runs the solve;
-//   delivery   copies the batch MST back, splits it by request (subtracting
-//              offsets) and completes the tickets.
+//   delivery   splits the batch MST by request on the device (subtracting
+//              offsets), writing it straight into output arrays the device
+//              can write (device memory, pinned host memory) and copying the
+//              rest, and completes the tickets.
 // Requests that arrive while a batch runs are admitted into the next one as
 // soon as a buffer set frees up (continuous batching at batch granularity).
 #include <algorithm>
@@ -43,22 +45,55 @@ static u64 now_ns() {
                               .count());
 }
 
-// Adds each request's vertex offset to its records; request of record i is
-// the last r with r_off[r] <= i.
+// The request of record i: the last r with r_off[r] <= i.
+__device__ __forceinline__ u32 request_of(u64 i, const u64* __restrict__ r_off, u32 count) {
+  u32 lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    if (r_off[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Adds each request's vertex offset to its records.  A wave takes chunks of
+// OFF_CHUNK records (one 16-byte vector of src and of dst per lane); a chunk
+// inside one request (almost all of them) needs one lookup for the whole
+// wave, one that crosses a request boundary looks up every record.
+constexpr u32 OFF_CHUNK = WAVE * 4;
+u32 offset_grid(u64 m) {
+  const u64 per_block = u64(OFF_CHUNK) * (BLOCK / WAVE);
+  return static_cast<u32>(std::max<u64>(1, std::min<u64>((m + per_block - 1) / per_block, 8192)));
+}
 __global__ __launch_bounds__(BLOCK) void batch_offset_kernel(u32* __restrict__ src,
                                                              u32* __restrict__ dst, u64 m,
                                                              const u64* __restrict__ r_off,
                                                              const u32* __restrict__ v_off,
                                                              u32 count) {
-  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
-    u32 lo = 0, hi = count;  // find largest r with r_off[r] <= i
-    while (hi - lo > 1) {
-      const u32 mid = (lo + hi) >> 1;
-      if (r_off[mid] <= i) lo = mid; else hi = mid;
+  const u64 wave = (u64(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
+  const u64 nwaves = u64(gridDim.x) * (BLOCK / WAVE);
+  const u32 lane = lane_id();
+  for (u64 c = wave; c * OFF_CHUNK < m; c += nwaves) {
+    const u64 first = c * OFF_CHUNK;
+    const u64 last = min(first + OFF_CHUNK, m) - 1;
+    const u32 r0 = request_of(first, r_off, count), r1 = request_of(last, r_off, count);
+    const u64 i = first + u64(lane) * 4;
+    const bool aligned =
+        ((reinterpret_cast<uintptr_t>(src + i) | reinterpret_cast<uintptr_t>(dst + i)) & 15) == 0;
+    if (r0 == r1 && i + 3 <= last && aligned) {
+      const u32 o = v_off[r0];
+      uint4 s4 = *reinterpret_cast<const uint4*>(src + i);
+      uint4 d4 = *reinterpret_cast<const uint4*>(dst + i);
+      s4.x += o; s4.y += o; s4.z += o; s4.w += o;
+      d4.x += o; d4.y += o; d4.z += o; d4.w += o;
+      *reinterpret_cast<uint4*>(src + i) = s4;
+      *reinterpret_cast<uint4*>(dst + i) = d4;
+    } else {
+      for (u64 j = i; j < i + 4 && j <= last; ++j) {
+        const u32 o = v_off[r0 == r1 ? r0 : request_of(j, r_off, count)];
+        src[j] += o;
+        dst[j] += o;
+      }
     }
-    const u32 o = v_off[lo];
-    src[i] += o;
-    dst[i] += o;
   }
 }
 
@@ -82,26 +117,48 @@ __global__ __launch_bounds__(BLOCK) void batch_bounds_kernel(const u32* __restri
   bounds[r] = lo;
 }
 
-// Back to request-local ids; per-request weight sums (one block per request).
+// Back to request-local ids; per-request weight sums.  Block (r, j) of a
+// (requests x split) grid takes every split-th 256-edge step of request r and
+// adds its weight sum to totals[r] (zeroed by the caller).
+// table holds four words per request: the device-usable addresses of its
+// three output arrays (0: not device-usable) and their capacity.  A request
+// whose arrays are device-usable and large enough gets its slice written there
+// directly; the others are rewritten in place for the host to copy.
 __global__ __launch_bounds__(BLOCK) void batch_unoffset_kernel(u32* __restrict__ out_s,
                                                                u32* __restrict__ out_d,
                                                                const u32* __restrict__ out_w,
                                                                const u64* __restrict__ bounds,
                                                                const u32* __restrict__ v_off,
+                                                               const u64* __restrict__ table,
                                                                u64* __restrict__ totals) {
   __shared__ u64 lds[WAVES];
   const u32 r = blockIdx.x;
   const u64 b0 = bounds[r], b1 = bounds[r + 1];
+  const u64 step = u64(gridDim.y) * BLOCK;
   const u32 o = v_off[r];
+  const u64* t = table + 4 * u64(r);
+  u32* ds = reinterpret_cast<u32*>(t[0]);
+  u32* dd = reinterpret_cast<u32*>(t[1]);
+  u32* dw = reinterpret_cast<u32*>(t[2]);
+  const bool direct = ds && dd && dw && b1 - b0 <= t[3];
   u64 sum = 0;
-  for (u64 i = b0 + threadIdx.x; i < b1; i += BLOCK) {
-    out_s[i] -= o;
-    out_d[i] -= o;
-    sum += out_w[i];
+  for (u64 i = b0 + u64(blockIdx.y) * BLOCK + threadIdx.x; i < b1; i += step) {
+    const u32 w = out_w[i];
+    if (direct) {
+      ds[i - b0] = out_s[i] - o;
+      dd[i - b0] = out_d[i] - o;
+      dw[i - b0] = w;
+    } else {
+      out_s[i] -= o;
+      out_d[i] -= o;
+    }
+    sum += w;
   }
   u64 total;
   block_exclusive_sum(sum, total, lds);
-  if (threadIdx.x == 0) totals[r] = total;
+  if (threadIdx.x == 0 && total)
+    atomicAdd(reinterpret_cast<unsigned long long*>(totals + r),
+              static_cast<unsigned long long>(total));
 }
 
 struct Request {
@@ -110,15 +167,18 @@ struct Request {
   u64 m = 0, n = 0;
   u32 *out_src = nullptr, *out_dst = nullptr, *out_w = nullptr;
   u64 out_cap = 0;
+  bool out_direct = false;  // the output arrays are device-usable: written by the kernel
   kmst_request_result res{};
 };
 
 struct BufferSet {
-  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off, bounds;
+  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off, bounds, table;
   std::vector<u64> h_r_off;
   std::vector<u32> h_v_off;
   u64* h_bounds = nullptr;  // pinned: 2 x (requests + 1) words (bounds, totals)
   u64 h_bounds_cap = 0;
+  u64* h_table = nullptr;   // pinned: 4 words per request (output addresses, capacity)
+  u64 h_table_cap = 0;
   hipEvent_t uploaded = nullptr;
   std::vector<std::shared_ptr<Request>> reqs;
   u64 m = 0, n = 0;
@@ -163,6 +223,21 @@ constexpr u64 BYTES_PER_RECORD = 3 * 12 + 48;  // upload sets + solver workspace
 constexpr u64 BYTES_PER_VERTEX = 64;
 
 u64 footprint(u64 m, u64 n) { return m * BYTES_PER_RECORD + n * BYTES_PER_VERTEX; }
+
+// True when kernels on `device` can write p: memory of that device, or pinned
+// host memory the device maps (the same address under unified addressing).
+// Pageable host memory, managed memory and other devices' memory are not.
+bool device_writable(const void* p, int device) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory may report an error: clear it
+    return false;
+  }
+  if (a.isManaged) return false;
+  if (a.type == hipMemoryTypeDevice) return a.device == device;
+  return a.type == hipMemoryTypeHost && a.devicePointer != nullptr;
+}
 
 template <typename F>
 kmst_status guard(F&& f) {
@@ -319,8 +394,7 @@ void kmst_server::solve_alone(BufferSet& b) {
       u32* dvo = static_cast<u32*>(b.v_off.p);
       (void)hipMemcpyAsync(dro, rr, 16, hipMemcpyHostToDevice, ctx->stream);
       (void)hipMemcpyAsync(dvo, &neg, 4, hipMemcpyHostToDevice, ctx->stream);
-      batch_offset_kernel<<<static_cast<u32>(std::min<u64>((r.m + BLOCK - 1) / BLOCK, 65536)),
-                            BLOCK, 0, ctx->stream>>>(const_cast<u32*>(e.src),
+      batch_offset_kernel<<<offset_grid(r.m), BLOCK, 0, ctx->stream>>>(const_cast<u32*>(e.src),
                                                      const_cast<u32*>(e.dst), r.m, dro, dvo, 1);
       (void)hipStreamSynchronize(ctx->stream);
     }
@@ -332,9 +406,9 @@ void kmst_server::solve_alone(BufferSet& b) {
       if (cnt > r.out_cap) {
         r.res.status = KMST_EINVAL;
       } else if (cnt) {
-        (void)hipMemcpy(r.out_src, o.src, cnt * 4, hipMemcpyDeviceToHost);
-        (void)hipMemcpy(r.out_dst, o.dst, cnt * 4, hipMemcpyDeviceToHost);
-        (void)hipMemcpy(r.out_w, o.weight, cnt * 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(r.out_src, o.src, cnt * 4, hipMemcpyDefault);
+        (void)hipMemcpy(r.out_dst, o.dst, cnt * 4, hipMemcpyDefault);
+        (void)hipMemcpy(r.out_w, o.weight, cnt * 4, hipMemcpyDefault);
       }
     }
   }
@@ -361,8 +435,7 @@ void kmst_server::compute_loop() {
       c.device = cfg.device;
       const kmst_status st = guard([&] {
         if (b->reqs.size() > 1) {
-          batch_offset_kernel<<<static_cast<u32>(std::min<u64>((b->m + BLOCK - 1) / BLOCK, 65536)),
-                                BLOCK, 0, ctx->stream>>>(
+          batch_offset_kernel<<<offset_grid(b->m), BLOCK, 0, ctx->stream>>>(
               static_cast<u32*>(b->s.p), static_cast<u32*>(b->d.p), b->m,
               static_cast<const u64*>(b->r_off.p), static_cast<const u32*>(b->v_off.p),
               static_cast<u32>(b->reqs.size()));
@@ -422,10 +495,33 @@ void kmst_server::deliver_loop() {
             static_cast<const u32*>(b->out_s.p), k, static_cast<const u32*>(b->v_off.p), count,
             dbounds);
         KMST_HIP(hipGetLastError());
-        batch_unoffset_kernel<<<count, BLOCK, 0, down_stream>>>(
+        // output addresses for the requests whose arrays the kernel can write
+        if (b->h_table_cap < 4 * u64(count)) {
+          if (b->h_table) (void)hipHostFree(b->h_table);
+          b->h_table = nullptr;
+          const u64 cap = 4 * u64(count) + 4096;
+          KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_table), cap * 8,
+                                 hipHostMallocDefault));
+          b->h_table_cap = cap;
+        }
+        for (u32 i = 0; i < count; ++i) {
+          const Request& r = *b->reqs[i];
+          u64* t = b->h_table + 4 * u64(i);
+          t[0] = r.out_direct ? reinterpret_cast<u64>(r.out_src) : 0;
+          t[1] = r.out_direct ? reinterpret_cast<u64>(r.out_dst) : 0;
+          t[2] = r.out_direct ? reinterpret_cast<u64>(r.out_w) : 0;
+          t[3] = r.out_cap;
+        }
+        u64* dtable = static_cast<u64*>(b->table.get(4 * u64(count) * 8));
+        KMST_HIP(hipMemcpyAsync(dtable, b->h_table, 4 * u64(count) * 8, hipMemcpyHostToDevice,
+                                down_stream));
+        // several blocks per request, about 2048 in all
+        fill_u64(down_stream, dtotals, count, 0);
+        const u32 split = static_cast<u32>(std::min<u64>(64, std::max<u64>(1, 2048 / count)));
+        batch_unoffset_kernel<<<dim3(count, split), BLOCK, 0, down_stream>>>(
             static_cast<u32*>(b->out_s.p), static_cast<u32*>(b->out_d.p),
             static_cast<const u32*>(b->out_w.p), dbounds, static_cast<const u32*>(b->v_off.p),
-            dtotals);
+            dtable, dtotals);
         KMST_HIP(hipGetLastError());
         KMST_HIP(hipMemcpyAsync(b->h_bounds, dbounds, 2 * (count + 1) * 8, hipMemcpyDeviceToHost,
                                 down_stream));
@@ -442,7 +538,7 @@ void kmst_server::deliver_loop() {
           r.res.status = KMST_OK;
           r.res.mst_edges = cnt;
           r.res.total_weight = ht[i];
-          if (cnt == 0) continue;
+          if (cnt == 0 || r.out_direct) continue;  // direct: the kernel wrote it
           KMST_HIP(hipMemcpyAsync(r.out_src, static_cast<u32*>(b->out_s.p) + hb[i], cnt * 4,
                                   hipMemcpyDefault, down_stream));
           KMST_HIP(hipMemcpyAsync(r.out_dst, static_cast<u32*>(b->out_d.p) + hb[i], cnt * 4,
@@ -557,6 +653,9 @@ kmst_status kmst_server_submit(kmst_server* s, const uint32_t* src, const uint32
     r->out_dst = out_dst;
     r->out_w = out_weight;
     r->out_cap = out_capacity;
+    r->out_direct = device_writable(out_src, s->cfg.device) &&
+                    device_writable(out_dst, s->cfg.device) &&
+                    device_writable(out_weight, s->cfg.device);
     r->res.submit_ns = now_ns();
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -622,9 +721,10 @@ void kmst_server_destroy(kmst_server* s) {
   (void)hipSetDevice(s->cfg.device);
   for (auto& b : s->sets) {
     for (DevBuf* d : {&b->s, &b->d, &b->w, &b->out_s, &b->out_d, &b->out_w, &b->r_off, &b->v_off,
-                      &b->bounds})
+                      &b->bounds, &b->table})
       d->release();
     if (b->h_bounds) (void)hipHostFree(b->h_bounds);
+    if (b->h_table) (void)hipHostFree(b->h_table);
     if (b->uploaded) (void)hipEventDestroy(b->uploaded);
   }
   for (int q = 0; q < kmst_server::UPLOAD_STREAMS; ++q) {

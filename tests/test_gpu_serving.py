@@ -80,3 +80,39 @@ def test_admission_rejects_oversize(gpu):
     r, edges = srv.wait(srv.submit(s, d, w, 100))
     assert r["status"] == 0 and r["total_weight"] == oracle.kruskal(k.to_wedges(s, d, w))[1]
     srv.close()
+
+
+@pytest.mark.parametrize("where", ["device", "pinned", "mixed"])
+def test_direct_delivery_to_device_writable_outputs(gpu, where):
+    """Outputs in device memory or pinned host memory are written by the
+    split kernel itself; pageable ones are copied.  Every kind, alone or mixed
+    in one batch, gets the oracle's result."""
+    torch = gpu
+    reqs = requests(45, 7)
+    kinds = {"device": ["device"], "pinned": ["pinned"], "mixed": ["device", "pinned", "host"]}[where]
+    outs = []
+    for i, (s, d, w, n) in enumerate(reqs):
+        cap = max(n, 1)
+        kind = kinds[i % len(kinds)]
+        if kind == "device":
+            outs.append([torch.zeros(cap, dtype=torch.int32, device="cuda") for _ in range(3)])
+        elif kind == "pinned":
+            outs.append([torch.zeros(cap, dtype=torch.int32).pin_memory() for _ in range(3)])
+        else:
+            outs.append([np.zeros(cap, np.uint32) for _ in range(3)])
+    srv = k.Server(max_batch_requests=4096)
+    tickets = [srv.submit(s, d, w, n, out=o) for (s, d, w, n), o in zip(reqs, outs)]
+    res = [srv.wait(t) for t in tickets]
+    nb, _ = srv.counters()
+    srv.close()
+    assert nb < len(reqs), "requests were batched"
+    for (s, d, w, n), (r, e) in zip(reqs, res):
+        assert r["status"] == 0
+        got = [x.cpu().numpy().view(np.uint32) if hasattr(x, "cpu") else x for x in e]
+        if len(s):
+            exp, exp_w = oracle.kruskal(k.to_wedges(s, d, w))
+            assert r["total_weight"] == exp_w
+            assert np.array_equal(got[0], exp["src"]) and np.array_equal(got[1], exp["dst"])
+            assert np.array_equal(got[2], exp["weight"])
+        else:
+            assert r["mst_edges"] == 0
