@@ -2172,7 +2172,9 @@ void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32
   KMST_HIP(hipGetLastError());
 }
 
-// Vertices with records whose parent-array root is not `root`, with their
+// Vertices with records outside the largest component -- whose root is not
+// `root`, or, with the bitmap `inside` (each partition range's largest
+// component), whose bit is clear -- with their
 // record ranges [seg_start, seg_end) in the (src, dst)-sorted input (seg_end
 // is 0 for vertices without records), in vertex order.  Each block
 // owns OUT_VPT * BLOCK consecutive vertices and takes one slot range with one
@@ -2182,22 +2184,24 @@ constexpr int OUT_VPT = 16;
 
 __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
     const u32* __restrict__ gpar, const u32* __restrict__ seg_start,
-    const u32* __restrict__ seg_end, u64 n, u32 root, u32* __restrict__ list,
-    u32* __restrict__ seg_lo, u32* __restrict__ seg_hi, u32* counters) {
+    const u32* __restrict__ seg_end, u64 n, u32 root, const u64* __restrict__ inside,
+    u32* __restrict__ list, u32* __restrict__ seg_lo, u32* __restrict__ seg_hi, u32* counters) {
   __shared__ u32 s_scan[WAVES];
   __shared__ u32 s_base;
   __shared__ u64 s_len[WAVES];
   const u64 v0 = u64(blockIdx.x) * (BLOCK * OUT_VPT) + u64(threadIdx.x) * OUT_VPT;
-  u32 g[OUT_VPT], h[OUT_VPT];
+  u32 h[OUT_VPT];
+  bool out[OUT_VPT];
 #pragma unroll
   for (int k = 0; k < OUT_VPT; ++k) {
     const u64 v = min(v0 + k, n - 1);
-    g[k] = gpar[v];
     h[k] = seg_end[v];
+    const bool in = inside ? ((inside[v >> 6] >> (v & 63)) & 1ull) != 0 : gpar[v] == root;
+    out[k] = v0 + k < n && h[k] && !in;
   }
   u32 mine = 0;
 #pragma unroll
-  for (int k = 0; k < OUT_VPT; ++k) mine += (v0 + k < n && h[k] && g[k] != root) ? 1u : 0u;
+  for (int k = 0; k < OUT_VPT; ++k) mine += out[k] ? 1u : 0u;
   u32 total;
   const u32 off = block_exclusive_sum(mine, total, s_scan);
   if (threadIdx.x == 0) s_base = total ? atomicAdd(counters, total) : 0u;
@@ -2206,7 +2210,7 @@ __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
   u64 len = 0;
 #pragma unroll
   for (int k = 0; k < OUT_VPT; ++k) {
-    if (v0 + k < n && h[k] && g[k] != root) {
+    if (out[k]) {
       const u32 v = static_cast<u32>(v0 + k);
       const u32 lo = seg_start[v];
       list[slot] = v;
@@ -2230,29 +2234,32 @@ __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
 }
 
 void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u32* seg_end,
-                  u64 n, u32 root, u32* list, u32* seg_lo, u32* seg_hi, u32* counters) {
+                  u64 n, u32 root, const u64* inside, u32* list, u32* seg_lo, u32* seg_hi,
+                  u32* counters) {
   fill_u32(st, counters, 4, 0);
   if (n == 0) return;
-  KTScope t(st, K_OTHER, 8.0 * n);
+  KTScope t(st, K_OTHER, inside ? 4.0 * n + n / 8.0 : 8.0 * n);
   const u32 blocks = ceil_div_u32(n, BLOCK * OUT_VPT);
-  outside_list_kernel<<<blocks, BLOCK, 0, st>>>(gpar, seg_start, seg_end, n, root, list, seg_lo,
-                                                seg_hi, counters);
+  outside_list_kernel<<<blocks, BLOCK, 0, st>>>(gpar, seg_start, seg_end, n, root, inside, list,
+                                                seg_lo, seg_hi, counters);
   KMST_HIP(hipGetLastError());
 }
 
 // Each block owns BLOCK consecutive listed vertices u and walks their records
 // as one flattened range (a thread finds its vertex by binary search in LDS).
 // Records (u, d, key) with key in [kmin, kmax] and gpar[d] != gpar[u] leave as
-// (gpar[u], gpar[d], key); when d's root is `root` (d is not listed, so its
-// own range is not walked) the reverse record is emitted as well.  A counting
+// (gpar[u], gpar[d], key); when d is inside the largest component (its root
+// is `root`, or its bit in `inside` is set: d is not listed, so its own range
+// is not walked) the reverse record is emitted as well.  A counting
 // pass sizes the block's output, one global atomic places it, and an emitting
 // pass (whose reads hit the cache) writes it.  Output order is arbitrary;
 // positions at or past cap raise ERR_BOUNDS.
 __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
     const u32* __restrict__ list, const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
     u32 n_list, const u32* __restrict__ in_d, const KeySrc keys, u64 kmin, u64 kmax,
-    const u32* __restrict__ gpar, u32 root, u32* __restrict__ s_out, u32* __restrict__ d_out,
-    u64* __restrict__ v_out, u64 cap, u32* count, u32* err) {
+    const u32* __restrict__ gpar, u32 root, const u64* __restrict__ inside,
+    u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out, u64 cap,
+    u32* count, u32* err) {
   __shared__ u64 s_start[BLOCK + 1];
   __shared__ u32 s_u[BLOCK], s_gu[BLOCK], s_lo[BLOCK];
   __shared__ u64 s_scan[WAVES];
@@ -2296,7 +2303,7 @@ __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
     if (key < kmin || key > kmax) return;
     gd = gpar[d];
     keep = gd != gu;
-    both = keep && gd == root;
+    both = keep && (inside ? ((inside[d >> 6] >> (d & 63)) & 1ull) != 0 : gd == root);
   };
   u32 mine = 0;
   for (u64 t0 = 0; t0 < total; t0 += BLOCK) {
@@ -2352,15 +2359,16 @@ __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
 
 void outside_gather(hipStream_t st, const u32* list, const u32* seg_lo, const u32* seg_hi,
                     u32 n_list, u64 total, const u32* in_d, const KeySrc& keys, u64 kmin, u64 kmax,
-                    const u32* gpar, u32 root, u32* s_out, u32* d_out, u64* v_out, u64 cap,
-                    u32* count, u32* err) {
+                    const u32* gpar, u32 root, const u64* inside, u32* s_out, u32* d_out,
+                    u64* v_out, u64 cap, u32* count, u32* err) {
   fill_u32(st, count, 1, 0);
   if (n_list == 0) return;
   // reads of the ranges (dst + key) and the root gathers; writes are added by
   // the caller
   KTScope t(st, K_RELABEL_SCATTER, (keys.v ? 16.0 : 8.0) * double(total) + 12.0 * n_list);
   outside_gather_kernel<<<ceil_div_u32(n_list, BLOCK), BLOCK, 0, st>>>(
-      list, seg_lo, seg_hi, n_list, in_d, keys, kmin, kmax, gpar, root, s_out, d_out, v_out, cap,
+      list, seg_lo, seg_hi, n_list, in_d, keys, kmin, kmax, gpar, root, inside, s_out, d_out, v_out,
+      cap,
       count, err);
   KMST_HIP(hipGetLastError());
 }
@@ -2371,6 +2379,74 @@ __global__ __launch_bounds__(BLOCK) void component_bits_kernel(const u32* __rest
   const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
   const u64 b = __ballot(v < n && gpar[v] == root);
   if (lane_id() == 0 && v < n) bits[v >> 6] = b;
+}
+
+// One block per partition range r = [part[r], part[r + 1]): giant[r] = the
+// most frequent parent-array root among up to BLOCK evenly spaced vertices of
+// the range that have records (NONE32 when none has).
+__global__ __launch_bounds__(BLOCK) void part_giant_kernel(const u32* __restrict__ gpar,
+                                                           const u32* __restrict__ seg_end,
+                                                           const u32* __restrict__ part,
+                                                           u32* __restrict__ giant) {
+  __shared__ u32 s_x[BLOCK];
+  __shared__ u64 s_best[WAVES];
+  const u32 r = blockIdx.x;
+  const u32 lo = part[r], len = part[r + 1] - lo;
+  const u32 samples = min<u32>(len, BLOCK);
+  u32 x = NONE32;
+  if (threadIdx.x < samples) {
+    const u32 v = lo + static_cast<u32>((u64(threadIdx.x) * len) / samples);
+    if (seg_end[v]) x = gpar[v];
+  }
+  s_x[threadIdx.x] = x;
+  __syncthreads();
+  u32 c = 0;
+  if (x != NONE32)
+    for (u32 j = 0; j < samples; ++j) c += s_x[j] == x ? 1u : 0u;
+  u64 key = c ? (u64(c) << 32) | x : 0;  // most samples wins (any root would be correct)
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    const u64 y = __shfl_xor(key, o, WAVE);
+    key = y > key ? y : key;
+  }
+  if (lane_id() == 0) s_best[wave_id()] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 b = 0;
+    for (int w = 0; w < WAVES; ++w) b = s_best[w] > b ? s_best[w] : b;
+    giant[r] = b ? static_cast<u32>(b) : NONE32;
+  }
+}
+
+// bits[v / 64] bit v % 64 = (gpar[v] == giant[range of v]) for v < n.
+__global__ __launch_bounds__(BLOCK) void part_bits_kernel(const u32* __restrict__ gpar, u64 n,
+                                                          const u32* __restrict__ part, u32 count,
+                                                          const u32* __restrict__ giant,
+                                                          u64* __restrict__ bits) {
+  const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
+  bool in = false;
+  if (v < n && v < part[count]) {
+    u32 lo = 0, hi = count;  // last range r with part[r] <= v
+    while (hi - lo > 1) {
+      const u32 mid = (lo + hi) >> 1;
+      if (part[mid] <= v) lo = mid;
+      else hi = mid;
+    }
+    const u32 g = giant[lo];
+    in = g != NONE32 && gpar[v] == g;
+  }
+  const u64 b = __ballot(in);
+  if (lane_id() == 0 && v < n) bits[v >> 6] = b;
+}
+
+void partition_giant_bits(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n,
+                          const u32* part, u32 count, u32* giant, u64* bits) {
+  if (n == 0 || count == 0) return;
+  KTScope t(st, K_OTHER, 4.0 * n + n / 8.0 + 8.0 * BLOCK * count);
+  part_giant_kernel<<<count, BLOCK, 0, st>>>(gpar, seg_end, part, giant);
+  KMST_HIP(hipGetLastError());
+  part_bits_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(gpar, n, part, count, giant, bits);
+  KMST_HIP(hipGetLastError());
 }
 
 void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits) {

@@ -121,22 +121,30 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
 // bits[v / 64] bit v % 64 = (gpar[v] == root) for v < n; bits needs
 // ceil(n / 64) entries.
 void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits);
+// The same for a partition of the vertices into disjoint ranges [part[r],
+// part[r + 1]) (r < count) that no record joins: the bit is set when v's root
+// is the most frequent root of a sample of its range's vertices with records
+// (giant[r], count entries of scratch).
+void partition_giant_bits(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n,
+                          const u32* part, u32 count, u32* giant, u64* bits);
 
-// Heavy records by the ranges of the vertices outside one component: every
-// record whose ends lie in different components has an end outside the
-// component of `root`, and in a symmetric (src, dst)-sorted input that end's
-// range [seg_start, seg_end) holds it.  outside_list writes the vertices with
-// records whose root is not `root` (list, seg_lo, seg_hi: up to n entries
+// Heavy records by the ranges of the vertices outside the largest component:
+// that of `root`, or, with the bitmap `inside` (partition_giant_bits), the
+// largest of every partition range.  Every record whose ends lie in different
+// components has an end outside, and in a symmetric (src, dst)-sorted input
+// that end's range [seg_start, seg_end) holds it.  outside_list writes the
+// vertices with records outside (list, seg_lo, seg_hi: up to n entries
 // each); counters[0] = count, counters[2..3] = u64 total range length.
 // outside_gather then emits, for keys in [kmin, kmax], the relabelled
 // cross-component records in both directions (see the kernel), at most
 // 2 * total of them.
 void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u32* seg_end,
-                  u64 n, u32 root, u32* list, u32* seg_lo, u32* seg_hi, u32* counters);
+                  u64 n, u32 root, const u64* inside, u32* list, u32* seg_lo, u32* seg_hi,
+                  u32* counters);
 void outside_gather(hipStream_t st, const u32* list, const u32* seg_lo, const u32* seg_hi,
                     u32 n_list, u64 total, const u32* in_d, const KeySrc& keys, u64 kmin, u64 kmax,
-                    const u32* gpar, u32 root, u32* s_out, u32* d_out, u64* v_out, u64 cap,
-                    u32* count, u32* err);
+                    const u32* gpar, u32 root, const u64* inside, u32* s_out, u32* d_out,
+                    u64* v_out, u64 cap, u32* count, u32* err);
 
 // K6 stable LSD radix sort by key (s << dbits) | d over total_bits bits; with
 // dbits == 0 the key is s alone and d is carried along.  Returns true when the

@@ -58,7 +58,10 @@ __device__ __forceinline__ u32 request_of(u64 i, const u64* __restrict__ r_off, 
 // Adds each request's vertex offset to its records.  A wave takes chunks of
 // OFF_CHUNK records (one 16-byte vector of src and of dst per lane); a chunk
 // inside one request (almost all of them) needs one lookup for the whole
-// wave, one that crosses a request boundary looks up every record.
+// wave, one that crosses a request boundary looks up every record.  With err,
+// a record with an id at or past its request's vertex count (it would join
+// another request's id range) raises ERR_INVALID; v_off then has count + 1
+// entries.
 constexpr u32 OFF_CHUNK = WAVE * 4;
 u32 offset_grid(u64 m) {
   const u64 per_block = u64(OFF_CHUNK) * (BLOCK / WAVE);
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(BLOCK) void batch_offset_kernel(u32* __restrict__ s
                                                              u32* __restrict__ dst, u64 m,
                                                              const u64* __restrict__ r_off,
                                                              const u32* __restrict__ v_off,
-                                                             u32 count) {
+                                                             u32 count, u32* err) {
   const u64 wave = (u64(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
   const u64 nwaves = u64(gridDim.x) * (BLOCK / WAVE);
   const u32 lane = lane_id();
@@ -83,13 +86,21 @@ __global__ __launch_bounds__(BLOCK) void batch_offset_kernel(u32* __restrict__ s
       const u32 o = v_off[r0];
       uint4 s4 = *reinterpret_cast<const uint4*>(src + i);
       uint4 d4 = *reinterpret_cast<const uint4*>(dst + i);
+      if (err) {
+        const u32 nr = v_off[r0 + 1] - o;
+        if (max(max(max(s4.x, s4.y), max(s4.z, s4.w)), max(max(d4.x, d4.y), max(d4.z, d4.w))) >= nr)
+          atomicOr(err, ERR_INVALID);
+      }
       s4.x += o; s4.y += o; s4.z += o; s4.w += o;
       d4.x += o; d4.y += o; d4.z += o; d4.w += o;
       *reinterpret_cast<uint4*>(src + i) = s4;
       *reinterpret_cast<uint4*>(dst + i) = d4;
     } else {
       for (u64 j = i; j < i + 4 && j <= last; ++j) {
-        const u32 o = v_off[r0 == r1 ? r0 : request_of(j, r_off, count)];
+        const u32 rj = r0 == r1 ? r0 : request_of(j, r_off, count);
+        const u32 o = v_off[rj];
+        if (err && (src[j] >= v_off[rj + 1] - o || dst[j] >= v_off[rj + 1] - o))
+          atomicOr(err, ERR_INVALID);
         src[j] += o;
         dst[j] += o;
       }
@@ -172,7 +183,8 @@ struct Request {
 };
 
 struct BufferSet {
-  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off, bounds, table;
+  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off, bounds, table, flag;
+  u32* h_flag = nullptr;    // pinned: the offset pass's error word
   std::vector<u64> h_r_off;
   std::vector<u32> h_v_off;
   u64* h_bounds = nullptr;  // pinned: 2 x (requests + 1) words (bounds, totals)
@@ -394,8 +406,8 @@ void kmst_server::solve_alone(BufferSet& b) {
       u32* dvo = static_cast<u32*>(b.v_off.p);
       (void)hipMemcpyAsync(dro, rr, 16, hipMemcpyHostToDevice, ctx->stream);
       (void)hipMemcpyAsync(dvo, &neg, 4, hipMemcpyHostToDevice, ctx->stream);
-      batch_offset_kernel<<<offset_grid(r.m), BLOCK, 0, ctx->stream>>>(const_cast<u32*>(e.src),
-                                                     const_cast<u32*>(e.dst), r.m, dro, dvo, 1);
+      batch_offset_kernel<<<offset_grid(r.m), BLOCK, 0, ctx->stream>>>(
+          const_cast<u32*>(e.src), const_cast<u32*>(e.dst), r.m, dro, dvo, 1, nullptr);
       (void)hipStreamSynchronize(ctx->stream);
     }
     u64 cnt = 0, tw = 0;
@@ -435,17 +447,35 @@ void kmst_server::compute_loop() {
       c.device = cfg.device;
       const kmst_status st = guard([&] {
         if (b->reqs.size() > 1) {
+          // the offset pass also checks that no record leaves its request's id
+          // range (the partitioned solve relies on it); its error word is
+          // read back after the solve
+          u32* dflag = static_cast<u32*>(b->flag.get(4));
+          if (!b->h_flag)
+            KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_flag), 4, hipHostMallocDefault));
+          fill_u32(ctx->stream, dflag, 1, 0);
           batch_offset_kernel<<<offset_grid(b->m), BLOCK, 0, ctx->stream>>>(
               static_cast<u32*>(b->s.p), static_cast<u32*>(b->d.p), b->m,
               static_cast<const u64*>(b->r_off.p), static_cast<const u32*>(b->v_off.p),
-              static_cast<u32>(b->reqs.size()));
+              static_cast<u32>(b->reqs.size()), dflag);
           KMST_HIP(hipGetLastError());
+          KMST_HIP(hipMemcpyAsync(b->h_flag, dflag, 4, hipMemcpyDeviceToHost, ctx->stream));
         }
         kmst_dev_edges e{static_cast<u32*>(b->s.p), static_cast<u32*>(b->d.p),
                          static_cast<u32*>(b->w.p), b->m, b->n};
         kmst_dev_mst o{static_cast<u32*>(b->out_s.p), static_cast<u32*>(b->out_d.p),
                        static_cast<u32*>(b->out_w.p), std::max<u64>(b->n, 1)};
-        const kmst_status s2 = kmst_solve_device(ctx.get(), &e, &c, &o, &b->mst_n, &b->mst_w);
+        const kmst_status s2 =
+            b->reqs.size() > 1
+                ? solve_device_partitioned(ctx.get(), &e, &c, &o, &b->mst_n, &b->mst_w,
+                                           static_cast<const u32*>(b->v_off.p),
+                                           static_cast<u32>(b->reqs.size()))
+                : kmst_solve_device(ctx.get(), &e, &c, &o, &b->mst_n, &b->mst_w);
+        if (b->reqs.size() > 1) {
+          KMST_HIP(hipStreamSynchronize(ctx->stream));
+          if (*b->h_flag)
+            throw kmst::Error(KMST_EINVAL, "a request has a vertex id at or past its n");
+        }
         if (s2 != KMST_OK) throw kmst::Error(s2, kmst_last_error());
       });
       if (st != KMST_OK) {
@@ -721,10 +751,11 @@ void kmst_server_destroy(kmst_server* s) {
   (void)hipSetDevice(s->cfg.device);
   for (auto& b : s->sets) {
     for (DevBuf* d : {&b->s, &b->d, &b->w, &b->out_s, &b->out_d, &b->out_w, &b->r_off, &b->v_off,
-                      &b->bounds, &b->table})
+                      &b->bounds, &b->table, &b->flag})
       d->release();
     if (b->h_bounds) (void)hipHostFree(b->h_bounds);
     if (b->h_table) (void)hipHostFree(b->h_table);
+    if (b->h_flag) (void)hipHostFree(b->h_flag);
     if (b->uploaded) (void)hipEventDestroy(b->uploaded);
   }
   for (int q = 0; q < kmst_server::UPLOAD_STREAMS; ++q) {

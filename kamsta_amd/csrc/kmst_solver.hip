@@ -130,7 +130,7 @@ std::vector<DevBuf*> kmst_context::all_bufs() {
           &tiles,    &scan_scratch, &hist,  &counters,  &lookback, &in_s, &in_d,
           &in_w,     &out_s,      &out_d,   &out_w,     &samples, &gen_a, &gen_b,
           &gen_c,    &gen_d,      &tune,    &codec_bytes, &codec_off, &codec_first,
-          &persist_log};
+          &persist_log, &part_giant};
 }
 
 namespace kmst {
@@ -159,8 +159,12 @@ struct RecView {
 
 class Solver {
  public:
-  Solver(kmst_context& c, const u32* is, const u32* id, const u32* iw, u64 m, u64 n)
-      : C(c), st(c.stream), in_s(is), in_d(id), in_w(iw), m0(m), n0(static_cast<u32>(n)) {}
+  // part (count + 1 device words, optional): ranges [part[r], part[r + 1]) of
+  // vertex ids that no record joins
+  Solver(kmst_context& c, const u32* is, const u32* id, const u32* iw, u64 m, u64 n,
+         const u32* part_ranges = nullptr, u32 part_ranges_count = 0)
+      : C(c), st(c.stream), in_s(is), in_d(id), in_w(iw), m0(m), n0(static_cast<u32>(n)),
+        part(part_ranges), part_count(part_ranges_count) {}
 
   void run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64 out_cap, u64* out_n,
            u64* total);
@@ -176,6 +180,9 @@ class Solver {
   const u32 *in_s, *in_d, *in_w;
   u64 m0;
   u32 n0;
+  const u32* part = nullptr;
+  u32 part_count = 0;
+  u32* part_giant = nullptr;  // largest component of every range (part_count words)
   u32* ctr = nullptr;
   u64 filter_threshold = 4;
   bool packed = true;   // strict key packed as (w, min, max), else (w, uid)
@@ -288,6 +295,7 @@ void Solver::allocate() {
   samples = static_cast<u32*>(C.samples.get(4096 * 4));
   persist_log = static_cast<u64*>(
       C.persist_log.get(PERSIST_CTL_WORDS * 4 + PERSIST_LOG_TOTAL * 8 + PERSIST_GROUP_WORDS * 4));
+  part_giant = part_count ? static_cast<u32*>(C.part_giant.get(u64(part_count) * 4 + 4)) : nullptr;
 }
 
 void apply_tuning(kmst_context& C) {
@@ -712,10 +720,18 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
     return;
   }
   bool gathered = false;
-  if (from_input && giant_share >= 0.25) {
-    // few vertices outside the largest component: scan only their ranges
+  // a partitioned solve (ranges of vertex ids that no record joins, e.g. a
+  // batch of requests) marks the largest component of every range in a bitmap
+  // (best_val is free here)
+  const u64* inside = nullptr;
+  if (part_count > 1) {
+    partition_giant_bits(st, gpar, seg_end, n0, part, part_count, part_giant, best_val);
+    inside = best_val;
+  }
+  if (from_input && (inside || giant_share >= 0.25)) {
+    // few vertices outside the largest component(s): scan only their ranges
     // (the per-vertex arrays of the rounds are free here)
-    outside_list(st, gpar, seg_start, seg_end, n0, giant_root, best_dst, parent, label,
+    outside_list(st, gpar, seg_start, seg_end, n0, giant_root, inside, best_dst, parent, label,
                  ctr + C_OUTSIDE);
     read_counters();
     const u64 n_list = C.host_counters[C_OUTSIDE];
@@ -726,8 +742,8 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
     }
     if (3 * total <= m0 && 2 * total <= scratch.m) {
       outside_gather(st, best_dst, parent, label, static_cast<u32>(n_list), total, in_d,
-                     keys_of(heavy), key_min, key_max, gpar, giant_root, scratch.s, scratch.d,
-                     scratch.v, scratch.m, ctr + C_MNEXT, ctr + C_ERR);
+                     keys_of(heavy), key_min, key_max, gpar, giant_root, inside, scratch.s,
+                     scratch.d, scratch.v, scratch.m, ctr + C_MNEXT, ctr + C_ERR);
       gathered = true;
     }
   }
@@ -736,8 +752,8 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
     // (n/8 bytes, cache resident) so that records inside it are dropped
     // without the key read and the two parent-array gathers (best_val is free
     // here)
-    const u64* giant = nullptr;
-    if (giant_share >= 0.25) {
+    const u64* giant = inside;
+    if (!giant && giant_share >= 0.25) {
       component_bits(st, gpar, n0, giant_root, best_val);
       giant = best_val;
     }
@@ -869,7 +885,8 @@ void check_config(const kmst_config* cfg) {
 }
 
 void solve_device_impl(kmst_context* ctx, const kmst_dev_edges* in, const kmst_config* cfg,
-                       kmst_dev_mst* out, uint64_t* out_n, uint64_t* total_weight) {
+                       kmst_dev_mst* out, uint64_t* out_n, uint64_t* total_weight,
+                       const kmst::u32* part = nullptr, kmst::u32 part_count = 0) {
   check_config(cfg);
   if (!ctx || !in || !out || !out_n || !total_weight)
     throw kmst::Error(KMST_EINVAL, "null argument");
@@ -887,7 +904,7 @@ void solve_device_impl(kmst_context* ctx, const kmst_dev_edges* in, const kmst_c
       *out_n = 0;
       *total_weight = 0;
     } else {
-      kmst::Solver s(*ctx, in->src, in->dst, in->weight, in->m_directed, in->n);
+      kmst::Solver s(*ctx, in->src, in->dst, in->weight, in->m_directed, in->n, part, part_count);
       s.run(*cfg, out->src, out->dst, out->weight, out->capacity, out_n, total_weight);
     }
     if (ctx->timing) ctx->timer.collect();
@@ -1313,3 +1330,11 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
 }
 
 }  // extern "C"
+
+namespace kmst {
+kmst_status solve_device_partitioned(kmst_context* ctx, const kmst_dev_edges* in,
+                                     const kmst_config* cfg, kmst_dev_mst* out, uint64_t* out_n,
+                                     uint64_t* total_weight, const u32* part, u32 count) {
+  return guarded([&] { solve_device_impl(ctx, in, cfg, out, out_n, total_weight, part, count); });
+}
+}  // namespace kmst

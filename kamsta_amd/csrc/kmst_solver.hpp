@@ -74,7 +74,8 @@ struct kmst_context {
   kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, seg_start, seg_end,
       mst;
   // kernel scratch
-  kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback, persist_log;
+  kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback, persist_log,
+      part_giant;
   kmst::SortWork sort_work;  // radix sort state over `hist`
   kmst::LookbackWork lb_work;  // compaction look-back state over `lookback`
   kmst_tuning tuning{0, 1, kmst::SORT_WIDE_MIN, kmst::PERSIST_MAX_DEFAULT};  // launch shapes of later solves
@@ -89,3 +90,13 @@ struct kmst_context {
 
   std::vector<kmst::DevBuf*> all_bufs();
 };
+
+namespace kmst {
+// kmst_solve_device for a solve whose vertex ids form ranges [part[r],
+// part[r + 1]) (r < count; part is a device array of count + 1 words) that no
+// record joins, e.g. a batch of requests: the heavy weight classes then skip
+// the records inside the largest component of every range.
+kmst_status solve_device_partitioned(kmst_context* ctx, const kmst_dev_edges* in,
+                                     const kmst_config* cfg, kmst_dev_mst* out, uint64_t* out_n,
+                                     uint64_t* total_weight, const u32* part, u32 count);
+}  // namespace kmst

@@ -116,3 +116,22 @@ def test_direct_delivery_to_device_writable_outputs(gpu, where):
             assert np.array_equal(got[2], exp["weight"])
         else:
             assert r["mst_edges"] == 0
+
+
+def test_request_leaving_its_id_range_is_refused(gpu):
+    """A request with a vertex id at or past its own n could, after the batch
+    offsets, join the next request's id range while the batch stays sorted
+    and symmetric.  The offset pass catches it: that request reports
+    KMST_EINVAL (as it does alone) and the other is solved correctly."""
+    a = (np.array([0, 1, 3, 12], np.uint32), np.array([1, 0, 12, 3], np.uint32),
+         np.array([5, 5, 7, 7], np.uint32), 10)
+    b = (np.array([3, 4], np.uint32), np.array([4, 3], np.uint32), np.array([9, 9], np.uint32), 5)
+    srv = k.Server(max_batch_requests=4096)
+    ta = srv.submit(*a)
+    tb = srv.submit(*b)
+    ra, _ = srv.wait(ta)
+    rb, eb = srv.wait(tb)
+    srv.close()
+    assert ra["status"] == k.KMST_EINVAL
+    assert rb["status"] == 0 and rb["total_weight"] == 9
+    assert list(eb[0]) == [3] and list(eb[1]) == [4]
