@@ -55,55 +55,100 @@ __device__ __forceinline__ u32 request_of(u64 i, const u64* __restrict__ r_off, 
   return lo;
 }
 
-// Adds each request's vertex offset to its records.  A wave takes chunks of
-// OFF_CHUNK records (one 16-byte vector of src and of dst per lane); a chunk
-// inside one request (almost all of them) needs one lookup for the whole
-// wave, one that crosses a request boundary looks up every record.  With err,
-// a record with an id at or past its request's vertex count (it would join
-// another request's id range) raises ERR_INVALID; v_off then has count + 1
-// entries.
+// The request of x for a whole wave (x wave-uniform): the last r < count with
+// r_off[r] <= x.  Each step probes 64 evenly spaced entries of the current
+// range at once, so 4096 requests take two dependent loads, not twelve.
+__device__ __forceinline__ u32 wave_request_of(u64 x, const u64* __restrict__ r_off, u32 count) {
+  const u32 lane = lane_id();
+  u32 lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const u32 step = (hi - lo + WAVE - 1) / WAVE;
+    const u32 p = lo + lane * step;
+    const u64 ok = __ballot(p < hi && r_off[p] <= x);  // lanes 0..k, r_off[lo] <= x
+    const u32 k = 63u - static_cast<u32>(__clzll(static_cast<long long>(ok)));
+    lo += k * step;
+    hi = min(hi, lo + step);
+  }
+  return lo;
+}
+
+// Adds each request's vertex offset to its records; with gather (three
+// device addresses per request: src, dst, weight; 0 when the request's
+// records were copied into the batch arrays) it first reads the request's
+// records from its own arrays and writes all three batch arrays.  A wave
+// takes chunks of OFF_CHUNK records; a chunk inside one request (almost all
+// of them) needs one request lookup for the whole wave, one that crosses a
+// request boundary looks up every record.  With err, a record with an id at
+// or past its request's vertex count (it would join another request's id
+// range) raises ERR_INVALID; v_off then has count + 1 entries.
 constexpr u32 OFF_CHUNK = WAVE * 4;
 u32 offset_grid(u64 m) {
   const u64 per_block = u64(OFF_CHUNK) * (BLOCK / WAVE);
   return static_cast<u32>(std::max<u64>(1, std::min<u64>((m + per_block - 1) / per_block, 8192)));
 }
-__global__ __launch_bounds__(BLOCK) void batch_offset_kernel(u32* __restrict__ src,
-                                                             u32* __restrict__ dst, u64 m,
-                                                             const u64* __restrict__ r_off,
-                                                             const u32* __restrict__ v_off,
-                                                             u32 count, u32* err) {
+
+__device__ __forceinline__ void offset_one(u64 j, u32 r, u32* __restrict__ src,
+                                           u32* __restrict__ dst, u32* __restrict__ w,
+                                           const u64* __restrict__ r_off,
+                                           const u32* __restrict__ v_off,
+                                           const u64* __restrict__ gather, u32* err) {
+  const u32 o = v_off[r];
+  u32 s, d;
+  const u64* g = gather ? gather + 3 * u64(r) : nullptr;
+  if (g && g[0]) {
+    const u64 l = j - r_off[r];
+    s = reinterpret_cast<const u32*>(g[0])[l];
+    d = reinterpret_cast<const u32*>(g[1])[l];
+    w[j] = reinterpret_cast<const u32*>(g[2])[l];
+  } else {
+    s = src[j];
+    d = dst[j];
+  }
+  if (err && (s >= v_off[r + 1] - o || d >= v_off[r + 1] - o)) atomicOr(err, ERR_INVALID);
+  src[j] = s + o;
+  dst[j] = d + o;
+}
+
+__global__ __launch_bounds__(BLOCK) void batch_offset_kernel(
+    u32* __restrict__ src, u32* __restrict__ dst, u32* __restrict__ w, u64 m,
+    const u64* __restrict__ r_off, const u32* __restrict__ v_off, const u64* __restrict__ gather,
+    u32 count, u32* err) {
   const u64 wave = (u64(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
   const u64 nwaves = u64(gridDim.x) * (BLOCK / WAVE);
   const u32 lane = lane_id();
   for (u64 c = wave; c * OFF_CHUNK < m; c += nwaves) {
     const u64 first = c * OFF_CHUNK;
     const u64 last = min(first + OFF_CHUNK, m) - 1;
-    const u32 r0 = request_of(first, r_off, count), r1 = request_of(last, r_off, count);
-    const u64 i = first + u64(lane) * 4;
-    const bool aligned =
-        ((reinterpret_cast<uintptr_t>(src + i) | reinterpret_cast<uintptr_t>(dst + i)) & 15) == 0;
-    if (r0 == r1 && i + 3 <= last && aligned) {
-      const u32 o = v_off[r0];
-      uint4 s4 = *reinterpret_cast<const uint4*>(src + i);
-      uint4 d4 = *reinterpret_cast<const uint4*>(dst + i);
-      if (err) {
-        const u32 nr = v_off[r0 + 1] - o;
-        if (max(max(max(s4.x, s4.y), max(s4.z, s4.w)), max(max(d4.x, d4.y), max(d4.z, d4.w))) >= nr)
-          atomicOr(err, ERR_INVALID);
+    const u32 r0 = wave_request_of(first, r_off, count);
+    const bool one = r0 + 1 >= count || r_off[r0 + 1] > last;  // chunk inside request r0
+    const u64* g = gather ? gather + 3 * u64(r0) : nullptr;
+    if (one && !(g && g[0])) {
+      // in place: one 16-byte vector of src and of dst per lane when aligned
+      const u64 i = first + u64(lane) * 4;
+      const bool aligned =
+          ((reinterpret_cast<uintptr_t>(src + i) | reinterpret_cast<uintptr_t>(dst + i)) & 15) == 0;
+      if (i + 3 <= last && aligned) {
+        const u32 o = v_off[r0];
+        uint4 s4 = *reinterpret_cast<const uint4*>(src + i);
+        uint4 d4 = *reinterpret_cast<const uint4*>(dst + i);
+        if (err) {
+          const u32 nr = v_off[r0 + 1] - o;
+          if (max(max(max(s4.x, s4.y), max(s4.z, s4.w)), max(max(d4.x, d4.y), max(d4.z, d4.w))) >= nr)
+            atomicOr(err, ERR_INVALID);
+        }
+        s4.x += o; s4.y += o; s4.z += o; s4.w += o;
+        d4.x += o; d4.y += o; d4.z += o; d4.w += o;
+        *reinterpret_cast<uint4*>(src + i) = s4;
+        *reinterpret_cast<uint4*>(dst + i) = d4;
+      } else {
+        for (u64 j = i; j < i + 4 && j <= last; ++j)
+          offset_one(j, r0, src, dst, w, r_off, v_off, gather, err);
       }
-      s4.x += o; s4.y += o; s4.z += o; s4.w += o;
-      d4.x += o; d4.y += o; d4.z += o; d4.w += o;
-      *reinterpret_cast<uint4*>(src + i) = s4;
-      *reinterpret_cast<uint4*>(dst + i) = d4;
     } else {
-      for (u64 j = i; j < i + 4 && j <= last; ++j) {
-        const u32 rj = r0 == r1 ? r0 : request_of(j, r_off, count);
-        const u32 o = v_off[rj];
-        if (err && (src[j] >= v_off[rj + 1] - o || dst[j] >= v_off[rj + 1] - o))
-          atomicOr(err, ERR_INVALID);
-        src[j] += o;
-        dst[j] += o;
-      }
+      // gathered or crossing a request boundary: lane-interleaved records
+      for (u64 j = first + lane; j <= last; j += WAVE)
+        offset_one(j, one ? r0 : request_of(j, r_off, count), src, dst, w, r_off, v_off, gather,
+                   err);
     }
   }
 }
@@ -179,11 +224,14 @@ struct Request {
   u32 *out_src = nullptr, *out_dst = nullptr, *out_w = nullptr;
   u64 out_cap = 0;
   bool out_direct = false;  // the output arrays are device-usable: written by the kernel
+  bool in_device = false;   // the input arrays are this device's memory: read in place
   kmst_request_result res{};
 };
 
 struct BufferSet {
-  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off, bounds, table, flag;
+  DevBuf s, d, w, out_s, out_d, out_w, r_off, v_off, bounds, table, flag, gather;
+  std::vector<u64> h_gather;  // src, dst, weight addresses per request (0: copied)
+  bool any_gather = false;
   u32* h_flag = nullptr;    // pinned: the offset pass's error word
   std::vector<u64> h_r_off;
   std::vector<u32> h_v_off;
@@ -239,6 +287,17 @@ u64 footprint(u64 m, u64 n) { return m * BYTES_PER_RECORD + n * BYTES_PER_VERTEX
 // True when kernels on `device` can write p: memory of that device, or pinned
 // host memory the device maps (the same address under unified addressing).
 // Pageable host memory, managed memory and other devices' memory are not.
+// True when p is memory of `device` itself (kernels read it in place).
+bool device_resident(const void* p, int device) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return !a.isManaged && a.type == hipMemoryTypeDevice && a.device == device;
+}
+
 bool device_writable(const void* p, int device) {
   if (!p) return false;
   hipPointerAttribute_t a;
@@ -345,7 +404,9 @@ void kmst_server::admit_loop() {
         for (size_t i = 0; i < b->reqs.size(); ++i) {
           const Request& r = *b->reqs[i];
           const u32* h = a == 0 ? r.src : (a == 1 ? r.dst : r.w);
-          if (r.m == 0) continue;
+          // device-resident requests of a multi-request batch are gathered by
+          // the offset pass instead (a single request is solved in place)
+          if (r.m == 0 || (r.in_device && b->reqs.size() > 1)) continue;
           if (run_src && h == run_src + run_len && b->h_r_off[i] == run_off + run_len &&
               run_len + r.m <= (u64{1} << 24)) {
             run_len += r.m;
@@ -366,6 +427,23 @@ void kmst_server::admit_loop() {
                               copy_stream));
       KMST_HIP(hipMemcpyAsync(dvo, b->h_v_off.data(), b->h_v_off.size() * 4, hipMemcpyHostToDevice,
                               copy_stream));
+      b->any_gather = false;
+      if (b->reqs.size() > 1) {
+        b->h_gather.assign(3 * b->reqs.size(), 0);
+        for (size_t i = 0; i < b->reqs.size(); ++i) {
+          const Request& r = *b->reqs[i];
+          if (!r.in_device || r.m == 0) continue;
+          b->h_gather[3 * i] = reinterpret_cast<u64>(r.src);
+          b->h_gather[3 * i + 1] = reinterpret_cast<u64>(r.dst);
+          b->h_gather[3 * i + 2] = reinterpret_cast<u64>(r.w);
+          b->any_gather = true;
+        }
+        if (b->any_gather) {
+          u64* dg = static_cast<u64*>(b->gather.get(b->h_gather.size() * 8));
+          KMST_HIP(hipMemcpyAsync(dg, b->h_gather.data(), b->h_gather.size() * 8,
+                                  hipMemcpyHostToDevice, copy_stream));
+        }
+      }
       KMST_HIP(hipEventRecord(b->uploaded, copy_stream));
       // the host vectors above must outlive the copies
       KMST_HIP(hipEventSynchronize(b->uploaded));
@@ -407,7 +485,8 @@ void kmst_server::solve_alone(BufferSet& b) {
       (void)hipMemcpyAsync(dro, rr, 16, hipMemcpyHostToDevice, ctx->stream);
       (void)hipMemcpyAsync(dvo, &neg, 4, hipMemcpyHostToDevice, ctx->stream);
       batch_offset_kernel<<<offset_grid(r.m), BLOCK, 0, ctx->stream>>>(
-          const_cast<u32*>(e.src), const_cast<u32*>(e.dst), r.m, dro, dvo, 1, nullptr);
+          const_cast<u32*>(e.src), const_cast<u32*>(e.dst), const_cast<u32*>(e.weight), r.m, dro,
+          dvo, nullptr, 1, nullptr);
       (void)hipStreamSynchronize(ctx->stream);
     }
     u64 cnt = 0, tw = 0;
@@ -455,14 +534,20 @@ void kmst_server::compute_loop() {
             KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_flag), 4, hipHostMallocDefault));
           fill_u32(ctx->stream, dflag, 1, 0);
           batch_offset_kernel<<<offset_grid(b->m), BLOCK, 0, ctx->stream>>>(
-              static_cast<u32*>(b->s.p), static_cast<u32*>(b->d.p), b->m,
-              static_cast<const u64*>(b->r_off.p), static_cast<const u32*>(b->v_off.p),
+              static_cast<u32*>(b->s.p), static_cast<u32*>(b->d.p), static_cast<u32*>(b->w.p),
+              b->m, static_cast<const u64*>(b->r_off.p), static_cast<const u32*>(b->v_off.p),
+              b->any_gather ? static_cast<const u64*>(b->gather.p) : nullptr,
               static_cast<u32>(b->reqs.size()), dflag);
           KMST_HIP(hipGetLastError());
           KMST_HIP(hipMemcpyAsync(b->h_flag, dflag, 4, hipMemcpyDeviceToHost, ctx->stream));
         }
         kmst_dev_edges e{static_cast<u32*>(b->s.p), static_cast<u32*>(b->d.p),
                          static_cast<u32*>(b->w.p), b->m, b->n};
+        if (b->reqs.size() == 1 && b->reqs[0]->in_device) {
+          // one device-resident request: solved in place, nothing was copied
+          const Request& r = *b->reqs[0];
+          e = kmst_dev_edges{r.src, r.dst, r.w, b->m, b->n};
+        }
         kmst_dev_mst o{static_cast<u32*>(b->out_s.p), static_cast<u32*>(b->out_d.p),
                        static_cast<u32*>(b->out_w.p), std::max<u64>(b->n, 1)};
         const kmst_status s2 =
@@ -683,6 +768,8 @@ kmst_status kmst_server_submit(kmst_server* s, const uint32_t* src, const uint32
     r->out_dst = out_dst;
     r->out_w = out_weight;
     r->out_cap = out_capacity;
+    r->in_device = device_resident(src, s->cfg.device) && device_resident(dst, s->cfg.device) &&
+                   device_resident(weight, s->cfg.device);
     r->out_direct = device_writable(out_src, s->cfg.device) &&
                     device_writable(out_dst, s->cfg.device) &&
                     device_writable(out_weight, s->cfg.device);
@@ -751,7 +838,7 @@ void kmst_server_destroy(kmst_server* s) {
   (void)hipSetDevice(s->cfg.device);
   for (auto& b : s->sets) {
     for (DevBuf* d : {&b->s, &b->d, &b->w, &b->out_s, &b->out_d, &b->out_w, &b->r_off, &b->v_off,
-                      &b->bounds, &b->table, &b->flag})
+                      &b->bounds, &b->table, &b->flag, &b->gather})
       d->release();
     if (b->h_bounds) (void)hipHostFree(b->h_bounds);
     if (b->h_table) (void)hipHostFree(b->h_table);

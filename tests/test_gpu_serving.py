@@ -135,3 +135,30 @@ def test_request_leaving_its_id_range_is_refused(gpu):
     assert ra["status"] == k.KMST_EINVAL
     assert rb["status"] == 0 and rb["total_weight"] == 9
     assert list(eb[0]) == [3] and list(eb[1]) == [4]
+
+
+@pytest.mark.parametrize("max_batch", [4096, 1])
+def test_device_resident_inputs_gathered_or_in_place(gpu, max_batch):
+    """Requests whose arrays are in device memory are not copied: a batch's
+    offset pass reads them in place (mixed here with host requests, which
+    are copied), and a request alone is solved straight from its arrays."""
+    torch = gpu
+    reqs = requests(40, 11)
+    srv = k.Server(max_batch_requests=max_batch)
+    subs = []
+    for i, (s, d, w, n) in enumerate(reqs):
+        if i % 2 == 0:
+            dev = [torch.from_numpy(np.ascontiguousarray(x, np.uint32).view(np.int32)).cuda()
+                   for x in (s, d, w)]
+            subs.append(srv.submit(*dev, n))
+        else:
+            subs.append(srv.submit(s, d, w, n))
+    res = [srv.wait(t) for t in subs]
+    srv.close()
+    for (s, d, w, n), (r, e) in zip(reqs, res):
+        assert r["status"] == 0
+        if len(s):
+            exp, exp_w = oracle.kruskal(k.to_wedges(s, d, w))
+            assert r["total_weight"] == exp_w
+            assert np.array_equal(e[0], exp["src"]) and np.array_equal(e[1], exp["dst"])
+            assert np.array_equal(e[2], exp["weight"])
