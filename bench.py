@@ -59,6 +59,9 @@ def parse():
     p.add_argument("--serving", action="store_true",
                    help="batched-serving benchmark instead of the one-graph throughput run")
     p.add_argument("--requests", type=int, default=4096)
+    p.add_argument("--max-batch-records", type=int, default=None,
+                   help="serving: admission bound per batch in directed records (default: the "
+                        "server's, 2^28)")
     p.add_argument("--rate", type=float, default=20000.0, help="Poisson arrival rate, req/s")
     p.add_argument("--device-inputs", action="store_true",
                    help="serving: request graphs resident in HBM instead of pinned host memory")
@@ -131,7 +134,8 @@ def serving(args):
         if args.persistent_max is not None:
             tuning = dataclasses.replace(k.Tuning(), persistent_max=args.persistent_max)
         srv = k.Server(algorithm=k.FILTER_BORUVKA, filter_threshold=args.filter_threshold,
-                       max_batch_requests=max_batch_requests, tuning=tuning)
+                       max_batch_requests=max_batch_requests, tuning=tuning,
+                       max_batch_records=args.max_batch_records)
         try:
             outp = torch.empty((3, out_total), dtype=torch.int32, pin_memory=True)
         except Exception:
@@ -206,6 +210,7 @@ def serving(args):
                    else f"inputs in {host_memory} host memory (latency includes H2D and D2H)"),
         "config": {"workload": "batched serving", "requests": args.requests, "rate": args.rate,
                    "directed_records": total, "algorithm": "filter_boruvka",
+                   "max_batch_records": args.max_batch_records or (1 << 28),
                    "persistent_max": (args.persistent_max if args.persistent_max is not None
                                       else k.Tuning().persistent_max)},
         "batched": {kk: (round(v, 3) if isinstance(v, float) else v) for kk, v in batched.items()},

@@ -221,7 +221,7 @@ typedef struct kmst_server_config {
   int32_t device;
   int32_t algorithm;             /* kmst_algorithm used for every batch */
   uint64_t filter_threshold;
-  uint64_t max_batch_records;    /* admission bound per batch (directed records) */
+  uint64_t max_batch_records;    /* admission bound per batch (directed records; default 2^28) */
   uint64_t max_batch_requests;   /* 1 = one graph per launch */
   uint64_t hbm_budget_bytes;     /* 0: half of the free device memory */
   uint32_t queue_capacity;       /* queued requests before submit returns KMST_EBUSY */

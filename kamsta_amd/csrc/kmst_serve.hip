@@ -692,7 +692,7 @@ void kmst_server_config_default(kmst_server_config* c) {
   c->device = 0;
   c->algorithm = KMST_ALGO_FILTER_BORUVKA;
   c->filter_threshold = 4;
-  c->max_batch_records = u64{1} << 27;
+  c->max_batch_records = u64{1} << 28;
   c->max_batch_requests = 4096;
   c->hbm_budget_bytes = 0;
   c->queue_capacity = 1u << 20;
