@@ -236,15 +236,4 @@ __device__ __forceinline__ u64 load_key(const KeySrc& k, u64 i, u32 s, u32 d) {
   return k.v ? k.v[i] : pack_key(k.w[i], s, d, k.vbits);
 }
 
-// Wave-aggregated append: returns the slot for lanes with pred set.
-__device__ __forceinline__ u32 wave_append(bool pred, u32* counter) {
-  const u64 mask = __ballot(pred);
-  if (mask == 0) return 0;
-  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-  u32 base = 0;
-  if (lane_id() == leader) base = atomicAdd(counter, static_cast<u32>(__popcll(mask)));
-  base = __shfl(base, leader, WAVE);
-  return base + mbcnt(mask);
-}
-
 }  // namespace kmst
