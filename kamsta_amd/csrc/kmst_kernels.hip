@@ -972,6 +972,68 @@ void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const KeyS
 constexpr u64 ST_ROOT = 1ull;
 constexpr u64 ST_EMIT = 1ull << 32;
 
+// K1 without grouping: records in any order.  The offer pass folds the keys
+// of adjacent lanes with equal sources (order-keeping compactions keep the
+// records of one original vertex adjacent, so long runs survive the rounds)
+// and takes one 64-bit atomic minimum per run; the bind pass names the other
+// endpoint of every minimum.
+constexpr int UA_ROWS = 4;  // rows of 64 records a wave loads before it offers
+__global__ __launch_bounds__(BLOCK) void argmin_offer_kernel(const u32* __restrict__ src,
+                                                             const u32* __restrict__ dst,
+                                                             const KeySrc keys, u64 m,
+                                                             u64* __restrict__ best) {
+  constexpr u64 CH = u64(WAVE) * UA_ROWS;
+  const u64 gw = (u64(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
+  const u64 nw = u64(gridDim.x) * WAVES;
+  const int lane = lane_id();
+  for (u64 base = gw * CH; base < m; base += nw * CH) {
+    u32 ss[UA_ROWS];
+    u64 kk[UA_ROWS];
+#pragma unroll
+    for (int u = 0; u < UA_ROWS; ++u) {
+      const u64 i = base + u64(u) * WAVE + lane;
+      ss[u] = i < m ? src[i] : NONE32;
+    }
+#pragma unroll
+    for (int u = 0; u < UA_ROWS; ++u) {
+      const u64 i = base + u64(u) * WAVE + lane;
+      kk[u] = i < m ? load_key(keys, i, ss[u], keys.v ? 0u : dst[i]) : VAL_NONE;
+    }
+#pragma unroll
+    for (int u = 0; u < UA_ROWS; ++u) offer_min(best, ss[u], kk[u]);
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void argmin_bind_kernel(const u32* __restrict__ src,
+                                                            const u32* __restrict__ dst,
+                                                            const KeySrc keys, u64 m,
+                                                            const u64* __restrict__ best,
+                                                            u32* __restrict__ best_dst) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
+    const u32 s = src[i], d = dst[i];
+    const u64 k = load_key(keys, i, s, d);
+    if (k != VAL_NONE && k == best[s]) best_dst[s] = d;
+  }
+}
+
+void unordered_argmin(hipStream_t st, const u32* src, const u32* dst, const KeySrc& keys, u64 m,
+                      u64 n, u64* best_val, u32* best_dst) {
+  fill_u64(st, best_val, n, VAL_NONE);
+  if (m == 0) return;
+  {
+    KTScope t(st, K_ARGMIN, 12.0 * double(m));
+    argmin_offer_kernel<<<grid_for((m + WAVE * UA_ROWS - 1) / (WAVE * UA_ROWS) * WAVE, 16384),
+                          BLOCK, 0, st>>>(src, dst, keys, m, best_val);
+    KMST_HIP(hipGetLastError());
+  }
+  {
+    KTScope t(st, K_ARGMIN, 24.0 * double(m) + 4.0 * double(n));
+    argmin_bind_kernel<<<grid_for(m, 16384), BLOCK, 0, st>>>(src, dst, keys, m, best_val,
+                                                              best_dst);
+    KMST_HIP(hipGetLastError());
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void hook_kernel(u64 n, const u64* __restrict__ best_val,
                                                      const u32* __restrict__ best_dst,
                                                      u32* __restrict__ parent,

@@ -80,6 +80,11 @@ void segmented_argmin(hipStream_t st, const u32* src, const u32* dst, const KeyS
                       u64 m, u64 n, u64* best_val, u32* best_dst, ArgPiece* pieces,
                       u32* piece_count);
 
+// K1 over records in any order (atomic minima; see the kernels): best_val is
+// filled here, best_dst set for every vertex with records.
+void unordered_argmin(hipStream_t st, const u32* src, const u32* dst, const KeySrc& keys, u64 m,
+                      u64 n, u64* best_val, u32* best_dst);
+
 // K2 hook (state[v]: bit 0 = root with an edge, bit 32 = emits its edge);
 // K3 pointer jumping; renumbering + MST emission from the scanned state.
 void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* parent,

@@ -107,26 +107,6 @@ __device__ __forceinline__ bool grid_sync(u32* bar, u32* groups, u32& k, u32* er
   return s_ok != 0;
 }
 
-// Called by all lanes of a wave holding consecutive records: folds the keys
-// of lanes with equal sources (k == VAL_NONE: nothing to offer) and lets the
-// last lane of each run offer the minimum.  A lane only ever folds keys of
-// its own source, so the value offered is always one of that source's keys,
-// and the last lane of a run sees at least the whole run.
-__device__ __forceinline__ void offer_min(u64* best, u32 s, u64 k) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int o = 1; o < WAVE; o <<= 1) {
-    const u32 so = __shfl_up(s, o, WAVE);
-    const u64 ko = __shfl_up(k, o, WAVE);
-    if (lane >= o && so == s && ko < k) k = ko;
-  }
-  const u32 sn = __shfl_down(s, 1, WAVE);
-  if ((lane == WAVE - 1 || sn != s) && k != VAL_NONE) {
-    unsigned long long* p = reinterpret_cast<unsigned long long*>(best + s);
-    if (k < *p) atomicMin(p, static_cast<unsigned long long>(k));
-  }
-}
-
 // Workgroup-aggregated reservation of output slots: every wave passes its
 // count (wave-uniform) and gets its first slot; one atomic per workgroup and
 // call (per-wave atomics on one counter serialise).  Called by all threads;

@@ -185,6 +185,10 @@ class Solver {
   u32* part_giant = nullptr;  // largest component of every range (part_count words)
   u32* ctr = nullptr;
   u64 filter_threshold = 4;
+  // plain Boruvka: rounds with many records per vertex skip the sort and take
+  // their minima by atomics over the compaction order (long runs of one
+  // original vertex keep the atomics few)
+  bool unordered_rounds = false;
   bool packed = true;   // strict key packed as (w, min, max), else (w, uid)
   u32 vbits = 0;
   RecView X, Y;
@@ -412,6 +416,7 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
   u64 n = n_start ? n_start : n0;  // by default the first round runs in the original id space
   bool first = !rep_given;
   bool deduped = false;  // cur holds at most n^2 live records, the rest marked dead
+  bool grouped = true;   // cur is grouped by src (the input, or a sort's output)
   while (m > 0 && n > 1) {
     if (use_persistent(deduped ? std::min(m, n * n) : m, n)) {
       persistent(cur, A, B, n, track_gpar, first);
@@ -420,8 +425,11 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     RoundStat rs;
     rs.n = n;
     rs.m = m;
-    segmented_argmin(st, cur.s, cur.d, keys_of(cur), m, n, best_val, best_dst, pieces,
-                     ctr + C_PIECES);
+    if (grouped)
+      segmented_argmin(st, cur.s, cur.d, keys_of(cur), m, n, best_val, best_dst, pieces,
+                       ctr + C_PIECES);
+    else
+      unordered_argmin(st, cur.s, cur.d, keys_of(cur), m, n, best_val, best_dst);
     hook(st, n, best_val, best_dst, parent, state);
     // best_dst is free after the hook: it takes the new ids of the roots
     scan_exclusive_u64(st, state, pos, n, reinterpret_cast<u64*>(scan_scratch),
@@ -448,8 +456,19 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     if (m == 0 || n <= 1) break;
     RecView alt = (cur.s == A.s) ? B : A;
     // dense rounds (more records than n^2 / 8) carry many parallel records
-    deduped = m * 8 > n * n;
-    cur = deduped ? sort_dedup(cur, alt, bits_for(n)) : sort_by_src(cur, alt, bits_for(n));
+    // dense rounds sort by (src, dst) and keep the lightest parallel record;
+    // rounds without sorts dedup only when that removes most records (a
+    // multigraph with m near n^2 records keeps about 2/3 of them)
+    deduped = unordered_rounds ? m > 8 * n * n : m * 8 > n * n;
+    if (deduped) {
+      cur = sort_dedup(cur, alt, bits_for(n));
+      grouped = true;
+    } else if (unordered_rounds && m >= 8 * n) {
+      grouped = false;  // the relabel kept the order: no sort
+    } else {
+      cur = sort_by_src(cur, alt, bits_for(n));
+      grouped = true;
+    }
     first = false;
   }
   if (track_gpar) pointer_jump(st, n0, gpar, ctr + C_ERR);
@@ -819,6 +838,7 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
       filter_classes();
     }
   } else {
+    unordered_rounds = true;
     rounds(input, X, Y, false);
   }
   // finish: sort the MST keys by (min, max) -- the low 2 bits(n) bits of a
