@@ -185,10 +185,6 @@ class Solver {
   u32* part_giant = nullptr;  // largest component of every range (part_count words)
   u32* ctr = nullptr;
   u64 filter_threshold = 4;
-  // plain Boruvka: rounds with many records per vertex skip the sort and take
-  // their minima by atomics over the compaction order (long runs of one
-  // original vertex keep the atomics few)
-  bool unordered_rounds = false;
   bool packed = true;   // strict key packed as (w, min, max), else (w, uid)
   u32 vbits = 0;
   RecView X, Y;
@@ -455,15 +451,16 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     cur = RecView{out.s, out.d, out.v, m};
     if (m == 0 || n <= 1) break;
     RecView alt = (cur.s == A.s) ? B : A;
-    // dense rounds (more records than n^2 / 8) carry many parallel records
-    // dense rounds sort by (src, dst) and keep the lightest parallel record;
-    // rounds without sorts dedup only when that removes most records (a
+    // rounds with many records per vertex skip the sort and take their
+    // minima by atomics over the compaction order (long runs of one earlier
+    // vertex keep the atomics few); a sort by (src, dst) that keeps the
+    // lightest parallel record pays only where it removes most records (a
     // multigraph with m near n^2 records keeps about 2/3 of them)
-    deduped = unordered_rounds ? m > 8 * n * n : m * 8 > n * n;
+    deduped = m > 8 * n * n;
     if (deduped) {
       cur = sort_dedup(cur, alt, bits_for(n));
       grouped = true;
-    } else if (unordered_rounds && m >= 8 * n) {
+    } else if (m >= 8 * n) {
       grouped = false;  // the relabel kept the order: no sort
     } else {
       cur = sort_by_src(cur, alt, bits_for(n));
@@ -838,7 +835,6 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
       filter_classes();
     }
   } else {
-    unordered_rounds = true;
     rounds(input, X, Y, false);
   }
   // finish: sort the MST keys by (min, max) -- the low 2 bits(n) bits of a

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 ALL = 1 << 22   # every round of these graphs fits
 OFF = 0
-MAX_DEGREE = 32  # rounds with more records per vertex id sort (and dedup) instead
+MAX_DEGREE = 32  # rounds with more records per vertex id run as separate launches
 SMALL = 1 << 14  # ... unless they are this small
 
 
@@ -31,9 +31,9 @@ def qualifies(m, n, pm, deduped=False):
 
 def deduped_before(rounds, i):
     """Round i starts on records a dedup sort marked (the previous round
-    ended dense)."""
+    ended with more than 8 n^2 records)."""
     prev = rounds[i - 1] if i else None
-    return prev is not None and prev["m_next"] * 8 > prev["n_next"] ** 2
+    return prev is not None and prev["m_next"] > 8 * prev["n_next"] ** 2
 
 
 def round_qualifies(rounds, i, pm):
@@ -160,25 +160,31 @@ def test_switch_inside_a_solve(gpu, ctx):
 
 @pytest.mark.parametrize("algo", [k.BORUVKA, k.FILTER_BORUVKA])
 def test_tail_after_dedup_rounds(gpu, ctx, algo):
-    """Dense rounds sort by (src, dst) and mark all but the lightest of each
-    run of parallel records dead (key all ones); a persistent tail that
-    starts on such records must drop the dead ones."""
+    """Rounds with more than 8 n^2 records sort by (src, dst) and mark all
+    but the lightest of each run of parallel records dead (key all ones); a
+    persistent tail that starts on such records must drop the dead ones."""
     torch = gpu
     n, m = 1 << 10, 1 << 17
     s, d, w = k.gen_gnm_host(n, m, 254, 35)
     rec = k.to_wedges(s, d, w)
     exp, exp_w = oracle.kruskal(rec)
+    # a filter threshold above the average degree (256) sends the filter
+    # straight to its base case (rounds that also track the component of every
+    # original vertex) on the whole, dense graph; base cases of a split graph
+    # hold about threshold * n records and never reach 8 n^2
+    thr = 1 << 12
     # thresholds at the sizes of this solve's own qualifying rounds: the tail
     # starts there, after dense rounds of separate launches
-    _, _, _, st0 = solve(torch, ctx, rec, n, algo, OFF, 3)
+    _, _, _, st0 = solve(torch, ctx, rec, n, algo, OFF, thr)
     rounds = st0["rounds"]
-    assert any(r["m"] * 8 > r["n"] ** 2 for r in rounds), "no dense (dedup) round"
-    assert any(deduped_before(rounds, i) for i in range(len(rounds)))
+    if algo == k.FILTER_BORUVKA:
+        assert [f["base"] for f in st0["filter"]] == [True], st0["filter"]
+    assert any(deduped_before(rounds, i) for i in range(len(rounds))), "no dedup round"
     sizes = sorted({live_bound(rounds, i) for i in range(1, len(rounds))
                     if round_qualifies(rounds, i, ALL) and deduped_before(rounds, i)})
     assert sizes, rounds
     for pm in sizes[-3:]:
-        cnt, tw, got, st = solve(torch, ctx, rec, n, algo, pm, 3)
+        cnt, tw, got, st = solve(torch, ctx, rec, n, algo, pm, thr)
         assert tw == exp_w and cnt == len(exp), (pm, tw, exp_w)
         assert np.array_equal(got[0], exp["src"]) and np.array_equal(got[1], exp["dst"])
         assert np.array_equal(got[2], exp["weight"])
