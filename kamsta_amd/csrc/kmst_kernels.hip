@@ -1876,15 +1876,17 @@ __global__ __launch_bounds__(THREADS) void radix_onesweep_u64_kernel(
 }
 
 bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWork& work,
-                    u32* err) {
+                    u32* err, u32 lo) {
   if (m <= 1 || bits == 0) return false;
   const u32 threads = sort_threads(work, m);
   const u32 tiles = ceil_div_u32(m, u64(threads) * SORT_IPT);
   if (tiles > work.tiles_cap) throw Error(KMST_EINVAL, "sort workspace too small");
-  if (bits > 64) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
-  const DigitPlan pl = plan_digits(bits, work.balanced);
+  if (lo + bits > 64) throw Error(KMST_EINVAL, "sort key wider than 64 bits");
+  DigitPlan pl = plan_digits(bits, work.balanced);
+  for (u32 p = 0; p < pl.passes; ++p) pl.shift[p] = static_cast<u8>(pl.shift[p] + lo);
   const u32 passes = pl.passes;
-  const u64 mask = bits >= 64 ? ~0ull : ((u64{1} << bits) - 1);  // bits above are not key
+  // bits above the key are masked off, bits below it are never a digit
+  const u64 mask = lo + bits >= 64 ? ~0ull : ((u64{1} << (lo + bits)) - 1);
   u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);  // zero between sorts
   u32* base = ghist + MAX_PASSES * RADIX;
   {
@@ -1956,41 +1958,59 @@ void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m) {
 // ============================================================== K9 output
 // MST keys -> canonical (min, max, w): packed keys decode directly,
 // positional keys gather the twin record (GetMstEdge, misc.hpp:145-206).
-__global__ __launch_bounds__(BLOCK) void mst_output_kernel(
-    const u64* __restrict__ keys, u64 count, u32 vbits, bool packed,
-    const u32* __restrict__ in_src, const u32* __restrict__ in_dst, const u32* __restrict__ in_w,
-    u32* __restrict__ out_s, u32* __restrict__ out_d, u32* __restrict__ out_w, u64* total) {
-  __shared__ u64 s_sum[WAVES];
-  u64 sum = 0;
-  const u64 mask = (u64{1} << vbits) - 1;
-  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < count; i += u64(gridDim.x) * BLOCK) {
-    const u64 key = keys[i];
-    u32 a, b, x;
-    if (packed) {
-      a = static_cast<u32>((key >> vbits) & mask);
-      b = static_cast<u32>(key & mask);
-      x = static_cast<u32>(key >> (2 * vbits));
-    } else {
-      const u32 uid = static_cast<u32>(key);
-      const u32 s = in_src[uid], d = in_dst[uid];
-      a = s < d ? s : d;
-      b = s < d ? d : s;
-      x = in_w[uid];
-    }
-    out_s[i] = a;
-    out_d[i] = b;
-    out_w[i] = x;
-    sum += x;
+struct MstDecode {
+  u32 vbits;
+  bool packed;
+  const u32* in_src;
+  const u32* in_dst;
+  const u32* in_w;
+  u32* out_s;
+  u32* out_d;
+  u32* out_w;
+};
+
+// Writes key's edge at output position i; returns its weight.
+__device__ __forceinline__ u32 mst_write(const MstDecode& o, u64 key, u64 i) {
+  u32 a, b, x;
+  if (o.packed) {
+    const u64 mask = (u64{1} << o.vbits) - 1;
+    a = static_cast<u32>((key >> o.vbits) & mask);
+    b = static_cast<u32>(key & mask);
+    x = static_cast<u32>(key >> (2 * o.vbits));
+  } else {
+    const u32 uid = static_cast<u32>(key);
+    const u32 s = o.in_src[uid], d = o.in_dst[uid];
+    a = s < d ? s : d;
+    b = s < d ? d : s;
+    x = o.in_w[uid];
   }
+  o.out_s[i] = a;
+  o.out_d[i] = b;
+  o.out_w[i] = x;
+  return x;
+}
+
+// Adds the workgroup's weight sums to *total (one atomic per workgroup).
+template <int THREADS>
+__device__ __forceinline__ void add_block_sum(u64 sum, u64* s_sum, u64* total) {
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, WAVE);
   if (lane_id() == 0) s_sum[wave_id()] = sum;
   __syncthreads();
   if (threadIdx.x == 0) {
     u64 t = 0;
-    for (int w = 0; w < WAVES; ++w) t += s_sum[w];
+    for (int w = 0; w < THREADS / WAVE; ++w) t += s_sum[w];
     if (t) atomicAdd(reinterpret_cast<unsigned long long*>(total), t);
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void mst_output_kernel(const u64* __restrict__ keys,
+                                                           u64 count, MstDecode o, u64* total) {
+  __shared__ u64 s_sum[WAVES];
+  u64 sum = 0;
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < count; i += u64(gridDim.x) * BLOCK)
+    sum += mst_write(o, keys[i], i);
+  add_block_sum<BLOCK>(sum, s_sum, total);
 }
 
 void mst_output(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
@@ -1998,9 +2018,141 @@ void mst_output(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool pack
                 u32* out_w, u64* total_dev) {
   if (count == 0) return;
   KTScope t(st, K_GATHER, (packed ? 20.0 : 24.0) * count);
-  mst_output_kernel<<<grid_for(count, 1024), BLOCK, 0, st>>>(keys, count, vbits, packed, in_src,
-                                                             in_dst, in_w, out_s, out_d, out_w,
-                                                             total_dev);
+  const MstDecode o{vbits, packed, in_src, in_dst, in_w, out_s, out_d, out_w};
+  mst_output_kernel<<<grid_for(count, 1024), BLOCK, 0, st>>>(keys, count, o, total_dev);
+  KMST_HIP(hipGetLastError());
+}
+
+// Output order from keys sorted by group only.  A key's place inside its
+// group is the number of group members with a smaller order field (an MSF
+// holds each (min, max) once, so fields and places are distinct); the group's
+// first key sits at the key's sorted position minus the members before it.
+// A workgroup stages its BLOCK keys' fields with MST_GROUP_SMALL more on each
+// side in LDS, and every thread reads its neighbours there, up to
+// MST_GROUP_SMALL either way: groups hold one or two keys at GNM and RGG2D.
+// The first key of a longer group (a hub's edges) lists the group.  Positions
+// outside [0, count) hold all ones, which no group shares (fields are
+// narrower than 64 bits).
+__global__ __launch_bounds__(BLOCK) void mst_rank_kernel(const u64* __restrict__ keys, u64 count,
+                                                         u64 fmask, u32 lbits, MstDecode o,
+                                                         u32* __restrict__ list,
+                                                         u32* __restrict__ list_count,
+                                                         u64* total) {
+  constexpr u32 HALO = MST_GROUP_SMALL;
+  constexpr u32 SPAN = BLOCK + 2 * HALO;
+  __shared__ u64 f_lds[SPAN];
+  __shared__ u64 s_sum[WAVES];
+  u64 sum = 0;
+  const u32 t = threadIdx.x;
+  for (u64 base = u64(blockIdx.x) * BLOCK; base < count; base += u64(gridDim.x) * BLOCK) {
+    __syncthreads();  // the previous tile's reads are done
+    for (u32 j = t; j < SPAN; j += BLOCK) {
+      const u64 q = base + j;  // position q - HALO
+      f_lds[j] = (q >= HALO && q - HALO < count) ? (keys[q - HALO] & fmask) : ~0ull;
+    }
+    __syncthreads();
+    const u64 p = base + t;
+    if (p >= count) continue;
+    const u32 c = HALO + t;
+    const u64 f = f_lds[c], g = f >> lbits;
+    u32 before = 0, after = 0, rank = 0;
+    while (before < HALO) {
+      const u64 x = f_lds[c - 1 - before];
+      if ((x >> lbits) != g) break;
+      ++before;
+      rank += x < f ? 1u : 0u;
+    }
+    while (after < HALO) {
+      const u64 x = f_lds[c + 1 + after];
+      if ((x >> lbits) != g) break;
+      ++after;
+      rank += x < f ? 1u : 0u;
+    }
+    // a side is closed when its scan stopped at another group within HALO
+    const bool closed = before < HALO && after < HALO;
+    if (closed && before + 1 + after <= MST_GROUP_SMALL) {
+      sum += mst_write(o, keys[p], p - before + rank);
+    } else if (before == 0) {
+      list[atomicAdd(list_count, 1u)] = static_cast<u32>(p);
+    }
+  }
+  add_block_sum<BLOCK>(sum, s_sum, total);
+}
+
+// Listed groups: one workgroup per group finds the group's length (up to
+// MST_GROUP_MAX + MST_CHUNK keys), sorts the group's keys by order field with
+// a bitonic network in LDS (padded to a power of two with all-ones keys,
+// whose field exceeds every real one) and writes them out.  A longer group is
+// left to the host (flag).
+__global__ __launch_bounds__(MST_CHUNK) void mst_rank_long_kernel(
+    const u64* __restrict__ keys, u64 count, u64 fmask, u32 lbits, MstDecode o,
+    const u32* __restrict__ list, const u32* __restrict__ list_count, u32* too_long, u64* total) {
+  __shared__ u64 sk[MST_GROUP_MAX];
+  __shared__ u64 s_sum[MST_CHUNK / WAVE];
+  __shared__ u32 s_end;
+  const u32 t = threadIdx.x;
+  const u32 groups = *list_count;
+  u64 sum = 0;
+  for (u32 e = blockIdx.x; e < groups; e += gridDim.x) {
+    const u64 start = list[e];
+    const u64 g = (keys[start] & fmask) >> lbits;
+    if (t == 0) s_end = NONE32;
+    __syncthreads();
+    u32 len = NONE32;  // the group's length, the same in every thread
+    for (u32 b = 0; b < MST_GROUP_MAX + MST_CHUNK; b += MST_CHUNK) {
+      const u64 j = start + b + t;
+      if (j >= count || ((keys[j] & fmask) >> lbits) != g) atomicMin(&s_end, b + t);
+      __syncthreads();
+      len = s_end;
+      __syncthreads();
+      if (len != NONE32) break;
+    }
+    if (len > MST_GROUP_MAX) {
+      if (t == 0) atomicOr(too_long, 1u);
+      continue;
+    }
+    u32 size = MST_CHUNK;
+    while (size < len) size <<= 1;
+    for (u32 i = t; i < size; i += MST_CHUNK) sk[i] = i < len ? keys[start + i] : ~0ull;
+    __syncthreads();
+    for (u32 k = 2; k <= size; k <<= 1) {
+      for (u32 j = k >> 1; j > 0; j >>= 1) {
+        for (u32 i = t; i < size; i += MST_CHUNK) {
+          const u32 x = i ^ j;
+          if (x > i) {
+            const u64 a = sk[i], b = sk[x];
+            const bool up = (i & k) == 0;
+            if (((a & fmask) > (b & fmask)) == up) {
+              sk[i] = b;
+              sk[x] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (u32 i = t; i < len; i += MST_CHUNK) sum += mst_write(o, sk[i], start + i);
+    __syncthreads();  // sk is reloaded for the next group
+  }
+  add_block_sum<MST_CHUNK>(sum, s_sum, total);
+}
+
+void mst_order(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed, u32 lbits,
+               const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
+               u32* out_w, u64* total_dev, u32* list, u32* list_count, u32* too_long) {
+  if (count == 0) return;
+  if (count > 0xFFFFFFFFull) throw Error(KMST_EINVAL, "more than 2^32 - 1 MST edges");
+  const u32 fbits = packed ? 2 * vbits : 32;
+  const u64 fmask = fbits >= 64 ? ~0ull : ((u64{1} << fbits) - 1);
+  const MstDecode o{vbits, packed, in_src, in_dst, in_w, out_s, out_d, out_w};
+  KTScope t(st, K_GATHER, (packed ? 20.0 : 24.0) * count);
+  // few workgroups: each ends with one atomic on the single total word
+  mst_rank_kernel<<<grid_for(count, 1024), BLOCK, 0, st>>>(keys, count, fmask, lbits, o, list,
+                                                           list_count, total_dev);
+  KMST_HIP(hipGetLastError());
+  // listed groups only; without any the workgroups leave at once
+  mst_rank_long_kernel<<<1024, MST_CHUNK, 0, st>>>(keys, count, fmask, lbits, o, list,
+                                                   list_count, too_long, total_dev);
   KMST_HIP(hipGetLastError());
 }
 

@@ -203,15 +203,31 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
 // K7 keep the lightest record of each (s, d) run (others marked dead).
 void dedup_mark(hipStream_t st, const u32* s, const u32* d, u64* v, u64 m);
 
-// The same sort for one u64 array by its low `bits` bits.
+// The same sort for one u64 array by its key bits [lo, lo + bits).
 bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWork& work,
-                    u32* err);
+                    u32* err, u32 lo = 0);
 
 // K9 MST keys (sorted by their low bits: (min, max) for packed keys, the twin
 // position for positional ones) -> canonical (min, max, w) output + u64 weight.
 void mst_output(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
                 const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
                 u32* out_w, u64* total_dev);
+
+// K9 without the full sort.  The order field of an MST key is (min, max) for a
+// packed key (its low 2 vbits bits), the twin position for a positional one
+// (its low 32 bits); its group is the field without the low `lbits` bits
+// (min, or the position / 64).  Given keys sorted by group, writes every key
+// at its place in field order, decoded, and adds the weights to *total_dev.
+// Groups longer than MST_GROUP_MAX are left unwritten and set *too_long.
+// `list` holds at least count / MST_GROUP_SMALL + 1 words; *list_count and
+// *too_long start at 0.
+constexpr u32 MST_RANK_BITS = 6;  // positional keys: positions / 64 form the groups
+constexpr u32 MST_GROUP_SMALL = 64;
+constexpr u32 MST_CHUNK = 1024;
+constexpr u32 MST_GROUP_MAX = 8 * MST_CHUNK;  // one group sorted in 64 KiB of LDS
+void mst_order(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed, u32 lbits,
+               const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
+               u32* out_w, u64* total_dev, u32* list, u32* list_count, u32* too_long);
 
 // Filter-Boruvka helpers.
 void iota_u32(hipStream_t st, u32* p, u64 n);

@@ -141,6 +141,7 @@ void apply_tuning(kmst_context& C);
 // Device counter slots (u32 words; u64 values at even word offsets >= 16).
 enum Ctr : int {
   C_ERR = 0, C_FWD = 1, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6, C_MAXW = 7,
+  C_MSTLIST = 8, C_MSTLONG = 9,  // output order: listed long groups, a group too long
   C_TOTAL64 = 16, C_HSUM64 = 18, C_ROUND64 = 22,
   C_CLASS64 = 24,   // MAX_CLASSES u64 class counts (words 24..41)
   C_OUTSIDE = 60    // outside_list: count (60), u64 total range length (62..63)
@@ -837,19 +838,36 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
   } else {
     rounds(input, X, Y, false);
   }
-  // finish: sort the MST keys by (min, max) -- the low 2 bits(n) bits of a
+  // finish: the output is in (min, max) order -- the low 2 bits(n) bits of a
   // packed key, the twin position (monotone in (min, max)) of a positional
-  // one -- then decode them into the canonical output
+  // one.  The keys are sorted by group only (the min field; the position
+  // without its low MST_RANK_BITS bits, at most bits(m0) bits), and one pass
+  // orders each group while it decodes.
   const u64 cnt = mst_count;
   if (cnt > out_cap) throw Error(KMST_EINVAL, "output capacity too small");
-  const u64* keys = mst;
-  if (cnt > 1 && radix_sort_u64(st, mst, X.v, cnt, packed ? 2 * vbits : 32, C.sort_work,
-                                ctr + C_ERR))
+  const u32 fbits = packed ? 2 * vbits : bits_for(m0);
+  const u32 lbits = std::min(fbits, packed ? vbits : MST_RANK_BITS);
+  u64* keys = mst;
+  if (cnt > 1 && radix_sort_u64(st, mst, X.v, cnt, fbits - lbits, C.sort_work, ctr + C_ERR, lbits))
     keys = X.v;
   u64* total_dev = reinterpret_cast<u64*>(ctr + C_TOTAL64);
   fill_u64(st, total_dev, 1, 0);
-  mst_output(st, keys, cnt, vbits, packed, in_s, in_d, in_w, out_s, out_d, out_w, total_dev);
+  fill_u32(st, ctr + C_MSTLIST, 2, 0);
+  // parent (n0 words) is free here: it lists the long groups' first keys
+  mst_order(st, keys, cnt, vbits, packed, lbits, in_s, in_d, in_w, out_s, out_d, out_w, total_dev,
+            parent, ctr + C_MSTLIST, ctr + C_MSTLONG);
   check_errors("finish");
+  C.stats.mst_long_groups = C.host_counters[C_MSTLIST];
+  C.stats.mst_resorted = C.host_counters[C_MSTLONG] != 0;
+  if (C.host_counters[C_MSTLONG]) {
+    // a vertex is the smaller endpoint of more than MST_GROUP_MAX MST edges
+    // (a star): sort the whole order field and write everything again
+    u64* alt = keys == mst ? X.v : mst;
+    if (radix_sort_u64(st, keys, alt, cnt, fbits, C.sort_work, ctr + C_ERR)) keys = alt;
+    fill_u64(st, total_dev, 1, 0);
+    mst_output(st, keys, cnt, vbits, packed, in_s, in_d, in_w, out_s, out_d, out_w, total_dev);
+    check_errors("finish");
+  }
   *out_n = cnt;
   *total = counter64(C_TOTAL64);
   C.stats.mst_edges = cnt;
@@ -1310,7 +1328,8 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
   o << "{\"n\":" << s.n << ",\"m\":" << s.m << ",\"algorithm\":" << s.algorithm
     << ",\"packed_keys\":" << (s.packed_keys ? "true" : "false")
     << ",\"mst_edges\":" << s.mst_edges << ",\"total_weight\":" << s.total_weight
-    << ",\"rounds\":[";
+    << ",\"mst_long_groups\":" << s.mst_long_groups
+    << ",\"mst_resorted\":" << (s.mst_resorted ? "true" : "false") << ",\"rounds\":[";
   for (size_t i = 0; i < s.rounds.size(); ++i) {
     const auto& r = s.rounds[i];
     o << (i ? "," : "") << "{\"n\":" << r.n << ",\"m\":" << r.m << ",\"n_next\":" << r.n_next

@@ -37,6 +37,11 @@ struct SolveStats {
   int algorithm = 0;
   bool packed_keys = true;
   u64 max_weight = 0;
+  // output order: groups of one smaller endpoint (or position range) longer
+  // than MST_GROUP_SMALL keys, and whether one exceeded MST_GROUP_MAX (the
+  // whole order field was then sorted)
+  u64 mst_long_groups = 0;
+  bool mst_resorted = false;
   std::vector<RoundStat> rounds;
   std::vector<FilterStat> filter;
 };
