@@ -62,11 +62,14 @@ __device__ __forceinline__ u32 scan_pad(u32 i) { return i + i / SCAN_IPT; }
 // With flagged_low, element i also gets flagged_low[i] = low 32 bits of its
 // exclusive sum when bit 0 of in[i] is set, NONE32 otherwise (the new id of
 // a root in the hook state scan, one 4-byte gather later instead of two
-// 8-byte ones).
+// 8-byte ones).  With tag_only only flagged elements write, their low 32 bits
+// tagged with ROOT_TAG (into the hook's parent array: the root's new id rides
+// on the load that finds the root).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void scan_down_kernel(const T* in, T* out, u64 n,
                                                           const T* __restrict__ partial,
-                                                          T* total_dev, u32* flagged_low) {
+                                                          T* total_dev, u32* flagged_low,
+                                                          bool tag_only) {
   __shared__ T lds[WAVES];
   __shared__ T tile[SCAN_TILE + SCAN_TILE / SCAN_IPT];
   const u64 base = u64(blockIdx.x) * SCAN_TILE + threadIdx.x;
@@ -101,7 +104,10 @@ __global__ __launch_bounds__(BLOCK) void scan_down_kernel(const T* in, T* out, u
     if (i < n) {
       const T o = tile[scan_pad(k * BLOCK + threadIdx.x)];
       out[i] = o;
-      if (flagged_low) flagged_low[i] = (v[k] & 1) ? static_cast<u32>(o) : NONE32;
+      if (flagged_low) {
+        if (!tag_only) flagged_low[i] = (v[k] & 1) ? static_cast<u32>(o) : NONE32;
+        else if (v[k] & 1) flagged_low[i] = static_cast<u32>(o) | ROOT_TAG;
+      }
     }
   }
   if (total_dev && partial == nullptr && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1)
@@ -119,7 +125,7 @@ u64 scan_scratch_elems(u64 n) {
 
 template <typename T>
 void scan_exclusive(hipStream_t st, const T* in, T* out, u64 n, T* scratch, T* total_dev,
-                    u32* flagged_low = nullptr) {
+                    u32* flagged_low = nullptr, bool tag_only = false) {
   if (n == 0) {
     if (total_dev) {
       if (sizeof(T) == 8) fill_u64(st, reinterpret_cast<u64*>(total_dev), 1, 0);
@@ -130,7 +136,8 @@ void scan_exclusive(hipStream_t st, const T* in, T* out, u64 n, T* scratch, T* t
   const u32 tiles = ceil_div_u32(n, SCAN_TILE);
   if (tiles == 1) {
     KTScope t(st, K_SCAN_DOWN, 2.0 * sizeof(T) * n);
-    scan_down_kernel<T><<<1, BLOCK, 0, st>>>(in, out, n, nullptr, total_dev, flagged_low);
+    scan_down_kernel<T><<<1, BLOCK, 0, st>>>(in, out, n, nullptr, total_dev, flagged_low,
+                                             tag_only);
     KMST_HIP(hipGetLastError());
     return;
   }
@@ -142,7 +149,8 @@ void scan_exclusive(hipStream_t st, const T* in, T* out, u64 n, T* scratch, T* t
   }
   scan_exclusive<T>(st, partial, partial, tiles, scratch + tiles, total_dev);
   KTScope t(st, K_SCAN_DOWN, 2.0 * sizeof(T) * n);
-  scan_down_kernel<T><<<tiles, BLOCK, 0, st>>>(in, out, n, partial, nullptr, flagged_low);
+  scan_down_kernel<T><<<tiles, BLOCK, 0, st>>>(in, out, n, partial, nullptr, flagged_low,
+                                               tag_only);
   KMST_HIP(hipGetLastError());
 }
 
@@ -151,8 +159,9 @@ void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scr
   scan_exclusive<u32>(st, in, out, n, scratch, total_dev);
 }
 void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scratch,
-                        u64* total_dev, u32* flagged_low) {
-  scan_exclusive<u64>(st, in, out, n, scratch, total_dev, flagged_low);
+                        u64* total_dev, u32* flagged_low, bool tag_only) {
+  if (tag_only && n > ROOT_TAG) throw Error(KMST_EINVAL, "root tags need n <= 2^31");
+  scan_exclusive<u64>(st, in, out, n, scratch, total_dev, flagged_low, tag_only);
 }
 
 // ============================================================== setup
@@ -1098,9 +1107,14 @@ void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err) {
 
 // Pointer jumping with the round's bookkeeping folded in (the state scan only
 // needs the hook, so it runs first): once v has reached its root r,
-// label[v] = root_id[r] (NONE32 for finished vertices), an emitter writes its
+// label[v] = r's new id (NONE32 for finished vertices), an emitter writes its
 // edge key into the MST list at mst_base + its slot, and with gpar the
 // parent array over the original ids is updated as gpar_kernel does.
+// TAGGED: the scan left every surviving root's new id in its parent entry
+// with ROOT_TAG set, so the load that reaches the root also yields the id;
+// roots keep that entry, finished vertices point at themselves untagged.
+// Otherwise root_id[r] holds the ids.
+template <bool TAGGED>
 __global__ __launch_bounds__(BLOCK) void jump_label_kernel(
     u64 n, u32* parent, const u32* __restrict__ root_id, const u64* __restrict__ state,
     const u64* __restrict__ pos, const u64* __restrict__ best_val, u32* __restrict__ label,
@@ -1109,19 +1123,27 @@ __global__ __launch_bounds__(BLOCK) void jump_label_kernel(
   const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
   if (v >= n) return;
   u32 x = parent[v];
-  u32 y = parent[x];
-  u32 guard = 0;
-  while (x != y) {
-    parent[v] = y;
-    x = y;
-    y = parent[x];
-    if (++guard > (1u << 26)) {
-      atomicOr(err, ERR_SPIN);
-      return;
+  u32 id;
+  if (TAGGED && (x & ROOT_TAG)) {
+    id = x & ~ROOT_TAG;
+    x = static_cast<u32>(v);
+  } else {
+    u32 y = parent[x];
+    u32 guard = 0;
+    while (x != y && !(TAGGED && (y & ROOT_TAG))) {
+      parent[v] = y;
+      x = y;
+      y = parent[x];
+      if (++guard > (1u << 26)) {
+        atomicOr(err, ERR_SPIN);
+        return;
+      }
     }
+    parent[v] = x;
+    if (TAGGED) id = (y & ROOT_TAG) ? (y & ~ROOT_TAG) : NONE32;
+    else id = root_id[x];
   }
-  parent[v] = x;
-  label[v] = root_id[x];
+  label[v] = id;
   const u64 sv = state[v];
   if (sv & ST_EMIT) mst[mst_base + (pos[v] >> 32)] = best_val[v];
   if (gpar && sv != 0) {
@@ -1135,9 +1157,13 @@ void jump_and_label(hipStream_t st, u64 n, u32* parent, const u32* root_id, cons
                     const u64* pos, const u64* best_val, u32* label, u64* mst, u64 mst_base,
                     const u32* rep, u32* gpar, u32* rep_next, u32* err) {
   if (n == 0) return;
-  KTScope t(st, K_JUMP, 40.0 * n + (gpar ? 16.0 * n : 0.0));
-  jump_label_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(
-      n, parent, root_id, state, pos, best_val, label, mst, mst_base, rep, gpar, rep_next, err);
+  KTScope t(st, K_JUMP, (root_id ? 40.0 : 36.0) * n + (gpar ? 16.0 * n : 0.0));
+  if (root_id)
+    jump_label_kernel<false><<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(
+        n, parent, root_id, state, pos, best_val, label, mst, mst_base, rep, gpar, rep_next, err);
+  else
+    jump_label_kernel<true><<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(
+        n, parent, nullptr, state, pos, best_val, label, mst, mst_base, rep, gpar, rep_next, err);
   KMST_HIP(hipGetLastError());
 }
 

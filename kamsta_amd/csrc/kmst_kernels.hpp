@@ -37,9 +37,11 @@ u64 scan_scratch_elems(u64 n);
 void scan_exclusive_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* scratch,
                         u32* total_dev);
 // With flagged_low (n entries) also flagged_low[i] = (u32) out[i] where bit 0
-// of in[i] is set, NONE32 elsewhere.
+// of in[i] is set, NONE32 elsewhere; with tag_only the flagged entries get
+// (u32) out[i] | ROOT_TAG and the others are left as they are.
+constexpr u32 ROOT_TAG = 0x80000000u;
 void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scratch,
-                        u64* total_dev, u32* flagged_low = nullptr);
+                        u64* total_dev, u32* flagged_low = nullptr, bool tag_only = false);
 
 // Setup: contract checks (hsum[0..1] = forward / reverse multiset hashes),
 // CSR offsets, strict keys (packed when they fit 64 bits, positional else).
@@ -91,7 +93,9 @@ void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* 
           u64* state);
 void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err);
 // Pointer jumping, labels, MST emission and (with gpar) the parent-array
-// update of one round in one pass; the state scan must have run.
+// update of one round in one pass; the state scan must have run.  Without
+// root_id (n < 2^31) the scan has written every surviving root's new id into
+// its own parent entry, tagged with ROOT_TAG.
 void jump_and_label(hipStream_t st, u64 n, u32* parent, const u32* root_id, const u64* state,
                     const u64* pos, const u64* best_val, u32* label, u64* mst, u64 mst_base,
                     const u32* rep, u32* gpar, u32* rep_next, u32* err);

@@ -428,10 +428,14 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     else
       unordered_argmin(st, cur.s, cur.d, keys_of(cur), m, n, best_val, best_dst);
     hook(st, n, best_val, best_dst, parent, state);
-    // best_dst is free after the hook: it takes the new ids of the roots
+    // the new ids of the roots: tagged into the roots' own parent entries
+    // (below 2^31 vertices), else into best_dst, free after the hook
+    const bool tagged = n <= ROOT_TAG;
     scan_exclusive_u64(st, state, pos, n, reinterpret_cast<u64*>(scan_scratch),
-                       reinterpret_cast<u64*>(ctr + C_ROUND64), best_dst);
-    jump_and_label(st, n, parent, best_dst, state, pos, best_val, label, mst, mst_count,
+                       reinterpret_cast<u64*>(ctr + C_ROUND64), tagged ? parent : best_dst,
+                       tagged);
+    jump_and_label(st, n, parent, tagged ? nullptr : best_dst, state, pos, best_val, label, mst,
+                   mst_count,
                    (track_gpar && !first) ? rep : nullptr, track_gpar ? gpar : nullptr,
                    rep_next, ctr + C_ERR);
     if (track_gpar) std::swap(rep, rep_next);
