@@ -2416,10 +2416,11 @@ void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32
 // `root`, or, with the bitmap `inside` (each partition range's largest
 // component), whose bit is clear -- with their
 // record ranges [seg_start, seg_end) in the (src, dst)-sorted input (seg_end
-// is 0 for vertices without records), in vertex order.  Each block
-// owns OUT_VPT * BLOCK consecutive vertices and takes one slot range with one
-// global atomic.  counters[0] = list length, counters[2..3] = u64 total range
-// length.
+// is 0 for vertices without records).  Each block owns OUT_VPT * BLOCK
+// consecutive vertices, thread t those at t + k * BLOCK (coalesced loads), and
+// takes one slot range with one global atomic; the list holds a block's
+// vertices thread by thread, the blocks in the order of their atomics.
+// counters[0] = list length, counters[2..3] = u64 total range length.
 constexpr int OUT_VPT = 16;
 
 __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
@@ -2429,15 +2430,15 @@ __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
   __shared__ u32 s_scan[WAVES];
   __shared__ u32 s_base;
   __shared__ u64 s_len[WAVES];
-  const u64 v0 = u64(blockIdx.x) * (BLOCK * OUT_VPT) + u64(threadIdx.x) * OUT_VPT;
+  const u64 v0 = u64(blockIdx.x) * (BLOCK * OUT_VPT) + threadIdx.x;
   u32 h[OUT_VPT];
   bool out[OUT_VPT];
 #pragma unroll
   for (int k = 0; k < OUT_VPT; ++k) {
-    const u64 v = min(v0 + k, n - 1);
+    const u64 v = min(v0 + u64(k) * BLOCK, n - 1);
     h[k] = seg_end[v];
     const bool in = inside ? ((inside[v >> 6] >> (v & 63)) & 1ull) != 0 : gpar[v] == root;
-    out[k] = v0 + k < n && h[k] && !in;
+    out[k] = v0 + u64(k) * BLOCK < n && h[k] && !in;
   }
   u32 mine = 0;
 #pragma unroll
@@ -2451,7 +2452,7 @@ __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
 #pragma unroll
   for (int k = 0; k < OUT_VPT; ++k) {
     if (out[k]) {
-      const u32 v = static_cast<u32>(v0 + k);
+      const u32 v = static_cast<u32>(v0 + u64(k) * BLOCK);
       const u32 lo = seg_start[v];
       list[slot] = v;
       seg_lo[slot] = lo;
