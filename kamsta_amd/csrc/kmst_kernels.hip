@@ -1407,11 +1407,14 @@ __device__ __forceinline__ u32 plan_digit(u64 key, const DigitPlan& pl, u32 p) {
 // records, relabelled alike), so most LDS atomics fold away.
 constexpr int UP_RPT = 8;
 
+// With vec (s and d 16-byte aligned) a thread's UP_RPT keys come in two
+// 16-byte loads per array.
 template <bool HAS_D>
 __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const u32* __restrict__ s,
                                                               const u32* __restrict__ d, u64 m,
                                                               u32 dbits, const DigitPlan pl,
-                                                              u32* __restrict__ ghist) {
+                                                              u32* __restrict__ ghist, bool vec) {
+  static_assert(UP_RPT == 8, "two 16-byte loads per array");
   const u32 passes = pl.passes;
   __shared__ u32 h[WAVES][MAX_PASSES][RADIX];
   for (int i = threadIdx.x; i < WAVES * MAX_PASSES * RADIX; i += BLOCK) (&h[0][0][0])[i] = 0;
@@ -1421,10 +1424,25 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const u32* __restr
   for (u64 base = u64(blockIdx.x) * (BLOCK * UP_RPT) + u64(threadIdx.x) * UP_RPT; base < m;
        base += stride) {
     u64 key[UP_RPT];
+    if (vec && base + UP_RPT <= m) {
+      const uint4 s0 = *reinterpret_cast<const uint4*>(s + base);
+      const uint4 s1 = *reinterpret_cast<const uint4*>(s + base + 4);
+      const u32 sv[UP_RPT] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+      u32 dv[UP_RPT] = {};
+      if (HAS_D) {
+        const uint4 d0 = *reinterpret_cast<const uint4*>(d + base);
+        const uint4 d1 = *reinterpret_cast<const uint4*>(d + base + 4);
+        dv[0] = d0.x; dv[1] = d0.y; dv[2] = d0.z; dv[3] = d0.w;
+        dv[4] = d1.x; dv[5] = d1.y; dv[6] = d1.z; dv[7] = d1.w;
+      }
 #pragma unroll
-    for (int k = 0; k < UP_RPT; ++k) {
-      const u64 e = min(base + k, m - 1);
-      key[k] = sort_key_of<HAS_D>(s[e], HAS_D ? d[e] : 0u, dbits);
+      for (int k = 0; k < UP_RPT; ++k) key[k] = sort_key_of<HAS_D>(sv[k], dv[k], dbits);
+    } else {
+#pragma unroll
+      for (int k = 0; k < UP_RPT; ++k) {
+        const u64 e = min(base + k, m - 1);
+        key[k] = sort_key_of<HAS_D>(s[e], HAS_D ? d[e] : 0u, dbits);
+      }
     }
     const u32 valid = static_cast<u32>(min<u64>(UP_RPT, m - base));
     for (u32 p = 0; p < passes; ++p) {
@@ -1732,11 +1750,14 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
   u32* ghist = work.words + 2 * (work.tiles_cap * RADIX + 1);  // zero between sorts
   u32* base = ghist + MAX_PASSES * RADIX;
   {
-    KTScope t(st, K_SORT_HIST, (d ? 8.0 : 4.0) * m);
+    const bool use_d = d && dbits;  // d is only carried along otherwise
+    KTScope t(st, K_SORT_HIST, (use_d ? 8.0 : 4.0) * m);
     // few, fat blocks: every block adds its histograms to the same global bins
     const u32 grid = grid_for((m + UP_RPT - 1) / UP_RPT, 512);
-    if (d) radix_upsweep_kernel<true><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, pl, ghist);
-    else radix_upsweep_kernel<false><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, pl, ghist);
+    const bool vec = ((reinterpret_cast<uintptr_t>(s) |
+                       (use_d ? reinterpret_cast<uintptr_t>(d) : 0)) & 15) == 0;
+    if (use_d) radix_upsweep_kernel<true><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, pl, ghist, vec);
+    else radix_upsweep_kernel<false><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, pl, ghist, vec);
     KMST_HIP(hipGetLastError());
     radix_digit_base_kernel<<<1, BLOCK, 0, st>>>(ghist, passes, base);
     KMST_HIP(hipGetLastError());
