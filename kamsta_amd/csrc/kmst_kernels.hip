@@ -2203,6 +2203,36 @@ void mst_order(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packe
   KMST_HIP(hipGetLastError());
 }
 
+__global__ __launch_bounds__(1024) void run_starts_kernel(const u32* __restrict__ s,
+                                                          const u32* m_dev, u32* count) {
+  __shared__ u32 s_c[1024 / WAVE];
+  const u64 m = *m_dev;
+  u32 c = 0;
+  for (u32 k = threadIdx.x; k < RUN_SAMPLES; k += 1024) {
+    const u64 i = (u64(k) * m) / RUN_SAMPLES;  // < m when m > 0
+    if (m > 0 && (i == 0 || s[i] != s[i - 1])) {
+      // low half: run starts; high half: those whose source lies in another
+      // 16-entry block (another 128-byte line of the per-vertex minima)
+      c += (i == 0 || (s[i] >> 4) != (s[i - 1] >> 4)) ? 0x10001u : 1u;
+    }
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
+  if (lane_id() == 0) s_c[wave_id()] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 t = 0;
+    for (int w = 0; w < 1024 / WAVE; ++w) t += s_c[w];
+    *count = t;
+  }
+}
+
+void sample_run_starts(hipStream_t st, const u32* s, const u32* m_dev, u32* count) {
+  KTScope t(st, K_OTHER, 8.0 * RUN_SAMPLES);
+  run_starts_kernel<<<1, 1024, 0, st>>>(s, m_dev, count);
+  KMST_HIP(hipGetLastError());
+}
+
 // ============================================================== filter
 __global__ __launch_bounds__(BLOCK) void iota_kernel(u32* p, u64 n) {
   for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += u64(gridDim.x) * BLOCK)

@@ -233,6 +233,18 @@ void mst_order(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packe
                const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
                u32* out_w, u64* total_dev, u32* list, u32* list_count, u32* too_long);
 
+// How many of RUN_SAMPLES evenly spread records of s[0..*m_dev) start a run
+// of equal s (differ from the record before them; record 0 counts), in the
+// low 16 bits of *count, and how many of those start far from the record
+// before (s / 16 differs), in the high 16 bits: estimates of how many
+// atomic minima a round without a sort takes and how many of them hit a
+// line the previous run did not.
+constexpr u32 RUN_SAMPLES = 4096;
+constexpr u64 UNORDERED_DEGREE = 8;
+constexpr u64 UNORDERED_MIN_DEGREE = 2;
+constexpr u64 UNORDERED_SMALL_N = u64{1} << 19;  // 4 MiB of u64 minima
+void sample_run_starts(hipStream_t st, const u32* s, const u32* m_dev, u32* count);
+
 // Filter-Boruvka helpers.
 void iota_u32(hipStream_t st, u32* p, u64 n);
 void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* parent,

@@ -142,6 +142,7 @@ void apply_tuning(kmst_context& C);
 enum Ctr : int {
   C_ERR = 0, C_FWD = 1, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6, C_MAXW = 7,
   C_MSTLIST = 8, C_MSTLONG = 9,  // output order: listed long groups, a group too long
+  C_RUNS = 10,                    // run starts among the next round's sampled records
   C_TOTAL64 = 16, C_HSUM64 = 18, C_ROUND64 = 22,
   C_CLASS64 = 24,   // MAX_CLASSES u64 class counts (words 24..41)
   C_OUTSIDE = 60    // outside_list: count (60), u64 total range length (62..63)
@@ -233,6 +234,7 @@ class Solver {
   // true when the remaining rounds of a graph with m live records and ids
   // below n run as one persistent launch
   bool use_persistent(u64 m, u64 n) const;
+  bool skip_sort(u64 m, u64 n, u32 far_starts) const;
   // every remaining round of `cur` in the persistent kernel (same contract
   // as one pass of the rounds loop, run to the end)
   void persistent(RecView cur, RecView A, RecView B, u64 n, bool track_gpar, bool first);
@@ -442,6 +444,7 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     RecView out = (cur.s == A.s) ? B : A;
     relabel_compact(st, cur.s, cur.d, keys_of(cur), 0, m, label, out.s, out.d, out.v, C.lb_work,
                     ctr + C_MNEXT, ctr + C_ERR);
+    sample_run_starts(st, out.s, ctr + C_MNEXT, ctr + C_RUNS);
     check_errors("round");
     if (g_ktimer) g_ktimer->adjust_last(K_RELABEL_SCATTER, 16.0 * C.host_counters[C_MNEXT]);
     const u64 round_total = counter64(C_ROUND64);
@@ -450,22 +453,24 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     const u64 m_next = C.host_counters[C_MNEXT];
     rs.n_next = n_next;
     rs.m_next = m_next;
+    rs.run_starts = m_next ? (C.host_counters[C_RUNS] & 0xFFFFu) : 0;
+    rs.far_starts = m_next ? (C.host_counters[C_RUNS] >> 16) : 0;
     C.stats.rounds.push_back(rs);
     m = m_next;
     n = n_next;
     cur = RecView{out.s, out.d, out.v, m};
     if (m == 0 || n <= 1) break;
     RecView alt = (cur.s == A.s) ? B : A;
-    // rounds with many records per vertex skip the sort and take their
-    // minima by atomics over the compaction order (long runs of one earlier
-    // vertex keep the atomics few); a sort by (src, dst) that keeps the
-    // lightest parallel record pays only where it removes most records (a
-    // multigraph with m near n^2 records keeps about 2/3 of them)
+    // rounds with many records per vertex, or whose runs of equal source
+    // mostly continue near the previous run, skip the sort and take their
+    // minima by atomics over the compaction order; a sort by (src, dst) that
+    // keeps the lightest parallel record pays only where it removes most
+    // records (a multigraph with m near n^2 records keeps about 2/3 of them)
     deduped = m > 8 * n * n;
     if (deduped) {
       cur = sort_dedup(cur, alt, bits_for(n));
       grouped = true;
-    } else if (m >= 8 * n) {
+    } else if (skip_sort(m, n, rs.far_starts)) {
       grouped = false;  // the relabel kept the order: no sort
     } else {
       cur = sort_by_src(cur, alt, bits_for(n));
@@ -474,6 +479,21 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     first = false;
   }
   if (track_gpar) pointer_jump(st, n0, gpar, ctr + C_ERR);
+}
+
+// A round on records in compaction order skips the sort when it has at
+// least UNORDERED_DEGREE records per vertex (plain Boruvka at GNM: runs of
+// about 32 records of one original vertex), or at least
+// UNORDERED_MIN_DEGREE and its atomic minima mostly stay near one another:
+// at most 2/5 of the sampled records start a run in another 128-byte line of
+// the minima (geometric and hyperbolic graphs, whose neighbouring vertices
+// merge into neighbouring ids), or the minima fit in an XCD's L2.  A GNM
+// base case after its first round (more than half of the runs start far
+// away, on 4·10^6 vertices) sorts.
+bool Solver::skip_sort(u64 m, u64 n, u32 far_starts) const {
+  if (m >= UNORDERED_DEGREE * n) return true;
+  if (m < UNORDERED_MIN_DEGREE * n) return false;
+  return n <= UNORDERED_SMALL_N || u64(far_starts) * 5 <= u64(RUN_SAMPLES) * 2;
 }
 
 bool Solver::use_persistent(u64 m, u64 n) const {
@@ -1337,7 +1357,9 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
   for (size_t i = 0; i < s.rounds.size(); ++i) {
     const auto& r = s.rounds[i];
     o << (i ? "," : "") << "{\"n\":" << r.n << ",\"m\":" << r.m << ",\"n_next\":" << r.n_next
-      << ",\"m_next\":" << r.m_next << ",\"persistent\":" << (r.persistent ? "true" : "false");
+      << ",\"m_next\":" << r.m_next << ",\"run_starts\":" << r.run_starts
+      << ",\"far_starts\":" << r.far_starts
+      << ",\"persistent\":" << (r.persistent ? "true" : "false");
     if (r.persistent)
       o << ",\"phase_us\":[" << r.phase_us[0] << "," << r.phase_us[1] << "," << r.phase_us[2]
         << "," << r.phase_us[3] << "]";

@@ -20,6 +20,8 @@ struct DevBuf {
 
 struct RoundStat {
   u64 n = 0, m = 0, n_next = 0, m_next = 0;
+  u32 run_starts = 0;       // of RUN_SAMPLES sampled next-round records (0: not sampled)
+  u32 far_starts = 0;       // ... whose source is 16 or more ids from the record before
   bool persistent = false;  // ran inside the persistent launch
   double phase_us[4] = {0, 0, 0, 0};  // persistent: bind, hook, jump, relabel
 };
