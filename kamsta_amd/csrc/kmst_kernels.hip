@@ -1056,8 +1056,11 @@ __global__ __launch_bounds__(BLOCK) void hook_kernel(u64 n, const u64* __restric
     return;
   }
   const u32 u = best_dst[v];
-  const bool mutual = best_val[u] == bv;
-  if (mutual && v < u) {
+  // only the lower end of a mutual pair becomes a root: a vertex above its
+  // target hooks either way and skips the gather
+  bool root = false;
+  if (v < u) root = best_val[u] == bv;
+  if (root) {
     parent[v] = static_cast<u32>(v);
     state[v] = ST_ROOT;
   } else {

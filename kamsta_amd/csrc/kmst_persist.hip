@@ -227,7 +227,7 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
             why(a.ctl, WHY_BIND, r);
             o = static_cast<u32>(v);
           }
-          const bool root = a.best[o] == bv && v < o;
+          const bool root = v < o && a.best[o] == bv;  // no gather above the target
           a.parent[v] = root ? static_cast<u32>(v) : o;
           em = !root;
           ++live;
