@@ -212,3 +212,27 @@ def test_tuning_field_validated(gpu, ctx):
         ctx.set_tuning(dataclasses.replace(entry, persistent_max=1 << 32))
     assert err.value.status == k.KMST_EINVAL
     assert ctx.get_tuning() == entry
+
+
+def test_round_run_samples(gpu, ctx):
+    """Every round that leaves records samples 4096 of them for run starts
+    (the sort-skip rule reads them): plain Boruvka on a GNM multigraph keeps
+    each original vertex's records adjacent, so runs are long at first."""
+    torch = gpu
+    n = 1 << 12
+    s, d, w = k.gen_gnm_host(n, 32 * n, 254, 36)
+    rec = k.to_wedges(s, d, w)
+    exp, exp_w = oracle.kruskal(rec)
+    cnt, tw, _, st = solve(torch, ctx, rec, n, k.BORUVKA, OFF)
+    assert tw == exp_w and cnt == len(exp)
+    rounds = st["rounds"]
+    assert rounds and not any(r["persistent"] for r in rounds)
+    for r in rounds:
+        if r["m_next"] == 0:
+            assert r["run_starts"] == 0 and r["far_starts"] == 0
+            continue
+        assert 0 < r["run_starts"] <= 4096, r
+        assert r["far_starts"] <= r["run_starts"], r
+    # 64 records per vertex in the input: the first relabel keeps runs of
+    # about that length, so far fewer than a quarter of the samples start one
+    assert rounds[0]["run_starts"] < 1024, rounds[0]
