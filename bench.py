@@ -370,12 +370,14 @@ def main():
     avg_ms = kd["ms"] / kd["launches"]
     achieved = per_launch_bytes / (avg_ms / 1000.0) / 1e9
     # HBM bytes per launch of the same kernel from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_*.md), or null
+    # FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_*.md) on the same
+    # workload, or null
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         try:
-            t = json.load(open(tfile)).get(best, {}).get(dom)
+            tj = json.load(open(tfile))
+            t = tj.get(best, {}).get(dom) if tj.get("workload") == workload else None
             traffic = round(t["bytes_per_launch"]) if t else None
         except Exception:
             traffic = None
