@@ -465,7 +465,9 @@ __global__ __launch_bounds__(BLOCK) void weight_classes_kernel(
   const u64 units = record_units_dev(m);
   const u64 steps = (units + WC_UNITS - 1) / WC_UNITS;
   const u64 wstride = u64(gridDim.x) * WAVES;
-  u32 mw = 0, seen = 0;
+  // above[j]: records with w > q[j]; for j < 2 they follow from the class 0
+  // and class 1 tests (q0 = q[0], q1 = q[1]) every record takes anyway
+  u32 mw = 0, seen = 0, tot0 = 0, tot1 = 0;
   u32 above[MAX_PIVOTS];
 #pragma unroll
   for (int j = 0; j < MAX_PIVOTS; ++j) above[j] = 0;
@@ -499,9 +501,11 @@ __global__ __launch_bounds__(BLOCK) void weight_classes_kernel(
         c0 += w <= q0 ? 1u : 0u;
         c1 += (w > q0 && w <= q1) ? 1u : 0u;
 #pragma unroll
-        for (u32 j = 0; j < MAX_PIVOTS; ++j)
+        for (u32 j = 2; j < MAX_PIVOTS; ++j)
           if (j < P.K) above[j] += (w > P.q[j]) ? 1u : 0u;
       }
+      tot0 += c0;
+      tot1 += c1;
       // per-lane counts are 0..4: sum them bit by bit with ballots
       u32 t0 = 0, t1 = 0;
 #pragma unroll
@@ -519,6 +523,8 @@ __global__ __launch_bounds__(BLOCK) void weight_classes_kernel(
       unit_c1[u0 + lane] = mine1;
     }
   }
+  if (P.K >= 1) above[0] = seen - tot0;
+  if (P.K >= 2) above[1] = seen - tot0 - tot1;
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) {
     seen += __shfl_xor(seen, o, WAVE);
