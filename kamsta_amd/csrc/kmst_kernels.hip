@@ -2212,6 +2212,8 @@ void mst_order(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packe
   KMST_HIP(hipGetLastError());
 }
 
+// One workgroup reads RUN_SAMPLES evenly spread sources of the next round's
+// records (and the source before each); m comes from the relabel's count.
 __global__ __launch_bounds__(1024) void run_starts_kernel(const u32* __restrict__ s,
                                                           const u32* m_dev, u32* count) {
   __shared__ u32 s_c[1024 / WAVE];
