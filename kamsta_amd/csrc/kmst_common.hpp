@@ -246,8 +246,12 @@ __device__ __forceinline__ void offer_min(u64* best, u32 s, u64 k) {
 #pragma unroll
   for (int o = 1; o < WAVE; o <<= 1) {
     const u32 so = __shfl_up(s, o, WAVE);
+    const bool same = lane >= o && so == s;
+    // runs are contiguous: without a same-source lane o back, no run is
+    // longer than o, and the steps so far (1 + 2 + ... + o/2 lanes) cover it
+    if (!__any(same)) break;
     const u64 ko = __shfl_up(k, o, WAVE);
-    if (lane >= o && so == s && ko < k) k = ko;
+    if (same && ko < k) k = ko;
   }
   const u32 sn = __shfl_down(s, 1, WAVE);
   if ((lane == WAVE - 1 || sn != s) && k != VAL_NONE) {
