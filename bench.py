@@ -1,12 +1,18 @@
 #!/usr/bin/env python3
-"""MST throughput benchmark on one MI355X.
+"""MST throughput and batched serving on one MI355X.
 
-Workload: GNM n=2^24 vertices, m=2^28 undirected edges (2^29 directed
-records), uint32 weights uniform in [1, 254], seed 2, generated on the device
-by the library's deterministic GNM generator.  One step = one device-resident
-solve (input already in HBM, MST left in HBM): the window of the reference's
-`mst_complete` timer (benchmarks/mst.cpp:212-230).  value = input edges/s =
-m / T.  Prints one JSON line.
+Headline workload: GNM n=2^24 vertices, m=2^28 undirected edges (2^29
+directed records), uint32 weights uniform in [1, 254], seed 2, generated on
+the device by the library's deterministic GNM generator.  One step = one
+device-resident solve (input already in HBM, MST left in HBM): the window of
+the reference's `mst_complete` timer (benchmarks/mst.cpp:212-230).
+value = input edges/s = m / T.
+
+The same JSON line carries the second half of the metric, batched serving
+(config.serving: 4096 GNM requests with n log-uniform in [2^8, 2^18], average
+degree U[8, 32], Poisson arrivals; batched vs one graph per launch, inputs in
+HBM and in pinned host memory), the CPU baselines (cpu_baseline, with every
+run in cpu_baseline.runs) and the roofline of the dominant kernel.
 """
 import argparse
 import dataclasses
@@ -44,112 +50,101 @@ def parse():
     p.add_argument("--persistent-max", type=int, default=None,
                    help="rounds on at most this many records run in one persistent launch "
                         "(kmst_tuning.persistent_max; 0 = never; default: the library's)")
-    p.add_argument("--cpu-log-n", type=int, default=23)
-    p.add_argument("--cpu-log-m", type=int, default=27)
-    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    p.add_argument("--cpu-threads", type=int, default=None,
+                   help="threads of the CPU baseline on the full input (default: "
+                        "OMP_NUM_THREADS, else the CPUs this process may run on)")
+    p.add_argument("--cpu-c2-4threads", action="store_true",
+                   help="also time the full GNM input at 1 x 4 threads (minutes of CPU)")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (a kernel of "
                         "known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     p.add_argument("--graph", choices=["gnm", "rgg2d", "rhg"], default="gnm",
                    help="gnm: device GNM (headline); rgg2d: n=2^log_n points, degree 16, weights "
-                        "U[1,2^31-1]; rhg: n=2^log_n, degree 32, gamma 2.8 (host-generated)")
+                        "U[1,2^31-1]; rhg: n=2^log_n, degree 32, gamma 2.8 (all generated on "
+                        "the device)")
     p.add_argument("--skip-identity", action="store_true",
                    help="skip the plain-vs-filter identity solve (profiling runs: only the "
                         "timed algorithm's launches)")
     p.add_argument("--serving", action="store_true",
-                   help="batched-serving benchmark instead of the one-graph throughput run")
+                   help="only the batched-serving benchmark (its own JSON line)")
+    p.add_argument("--no-serving", action="store_true",
+                   help="leave the serving figures out of the throughput line")
     p.add_argument("--requests", type=int, default=4096)
     p.add_argument("--max-batch-records", type=int, default=None,
                    help="serving: admission bound per batch in directed records (default: the "
                         "server's, 2^28)")
     p.add_argument("--rate", type=float, default=20000.0, help="Poisson arrival rate, req/s")
     p.add_argument("--device-inputs", action="store_true",
-                   help="serving: request graphs resident in HBM instead of pinned host memory")
-    p.add_argument("--verify-full", action="store_true",
-                   help="also run the oracle's Kruskal on the full input (minutes of CPU)")
+                   help="--serving: request graphs resident in HBM instead of pinned host memory")
     return p.parse_args()
 
 
-def serving(args):
-    """Batched serving: 4096 GNM requests, n log-uniform in [2^8, 2^18],
-    average degree uniform in [8, 32], weights U[1,254], seed = base + i,
-    Poisson arrivals at a fixed rate; the batched server against the
-    one-graph-per-launch server on the same arrival trace."""
+def cpu_threads(args):
+    if args.cpu_threads:
+        return args.cpu_threads
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+# ------------------------------------------------------------------ serving
+def serving_requests(args, ctx):
+    """The request stream: specs (n, m) and every request's arrays generated
+    on the device into one pool (3 x total int32)."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(args.seed if args.seed is not None else 2)
+    specs = []
+    for _ in range(args.requests):
+        n = int(round(2 ** rng.uniform(8, 18)))
+        deg = rng.uniform(8, 32)
+        specs.append((n, int(min(n * deg / 2, n * (n - 1) // 2))))
+    offs = [0] + [int(x) for x in np.cumsum([2 * m for _, m in specs])]
+    total = offs[-1]
+    t0 = time.time()
+    pool = torch.empty((3, total), dtype=torch.int32, device="cuda")
+    base_seed = (args.seed if args.seed is not None else 2) * 1000003
+    for i, (n, m) in enumerate(specs):
+        a, b = int(offs[i]), int(offs[i + 1])
+        ctx.gen_gnm_device(n, m, 254, base_seed + i, pool[0, a:b], pool[1, a:b], pool[2, a:b])
+    torch.cuda.synchronize()
+    log(f"[serve] {len(specs)} requests, {total} directed records, generated on the device in "
+        f"{time.time() - t0:.1f}s")
+    return specs, offs, pool, rng
+
+
+def serve_stream(args, reqs, specs, gaps, max_batch_requests):
+    """Submits every request at its Poisson arrival time through a server
+    with this batch bound; returns the figures and the per-request results."""
     import threading
     import numpy as np
     import torch
     import kamsta_amd as k
 
-    rng = np.random.default_rng(args.seed)
-    dev = torch.device("cuda:0")
-    ctx = k.Context(0)
-    specs = []
-    for i in range(args.requests):
-        n = int(round(2 ** rng.uniform(8, 18)))
-        deg = rng.uniform(8, 32)
-        m = int(min(n * deg / 2, n * (n - 1) // 2))
-        specs.append((n, m))
-    total = sum(2 * m for _, m in specs)
-    log(f"[serve] {args.requests} requests, {total} directed records, generating on device")
-    host_memory = "device"
-    if args.device_inputs:
-        pool = torch.empty((3, total), dtype=torch.int32, device=dev)
-    else:
-        try:
-            pool = torch.empty((3, total), dtype=torch.int32, pin_memory=True)
-            host_memory = "pinned"
-        except Exception as e:  # pinned memory is a limited resource
-            log(f"[serve] pinned allocation of {12 * total / 1e9:.1f} GB failed ({type(e).__name__}); "
-                "using pageable host memory")
-            pool = torch.empty((3, total), dtype=torch.int32)
-            host_memory = "pageable"
-    t0 = time.time()
-    offs = []
-    o = 0
-    scratch = [torch.empty(2 * max(m for _, m in specs), dtype=torch.int32, device=dev)
-               for _ in range(3)]
-    for i, (n, m) in enumerate(specs):
-        ctx.gen_gnm_device(n, m, 254, args.seed * 1000003 + i, *scratch)
-        for j in range(3):
-            pool[j, o:o + 2 * m].copy_(scratch[j][:2 * m])
-        offs.append(o)
-        o += 2 * m
-    torch.cuda.synchronize()
-    del scratch
-    log(f"[serve] generated in {time.time() - t0:.1f}s")
-    if args.device_inputs:
-        reqs = [(pool[0, a:a + 2 * m], pool[1, a:a + 2 * m], pool[2, a:a + 2 * m], n)
-                for a, (n, m) in zip(offs, specs)]
-    else:
-        pv = pool.numpy().view(np.uint32)
-        reqs = [(pv[0, a:a + 2 * m], pv[1, a:a + 2 * m], pv[2, a:a + 2 * m], n)
-                for a, (n, m) in zip(offs, specs)]
-    # result buffers: one pinned pool per run, a slice of n entries per request
-    out_total = sum(max(n, 1) for n, _ in specs)
+    tuning = None
+    if args.persistent_max is not None:
+        tuning = dataclasses.replace(k.Tuning(), persistent_max=args.persistent_max)
+    srv = k.Server(algorithm=k.FILTER_BORUVKA, filter_threshold=args.filter_threshold,
+                   max_batch_requests=max_batch_requests, tuning=tuning,
+                   max_batch_records=args.max_batch_records)
     out_offs = np.concatenate([[0], np.cumsum([max(n, 1) for n, _ in specs])])
-    gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
-
-    def run(max_batch_requests):
-        tuning = None
-        if args.persistent_max is not None:
-            tuning = dataclasses.replace(k.Tuning(), persistent_max=args.persistent_max)
-        srv = k.Server(algorithm=k.FILTER_BORUVKA, filter_threshold=args.filter_threshold,
-                       max_batch_requests=max_batch_requests, tuning=tuning,
-                       max_batch_records=args.max_batch_records)
-        try:
-            outp = torch.empty((3, out_total), dtype=torch.int32, pin_memory=True)
-        except Exception:
-            outp = torch.zeros((3, out_total), dtype=torch.int32)
-        outp = outp.numpy().view(np.uint32)
-        outs = [tuple(outp[j, out_offs[i]:out_offs[i + 1]] for j in range(3))
-                for i in range(len(reqs))]
+    try:
+        outp = torch.empty((3, int(out_offs[-1])), dtype=torch.int32, pin_memory=True)
+    except RuntimeError:
+        outp = torch.zeros((3, int(out_offs[-1])), dtype=torch.int32)
+    outp = outp.numpy().view(np.uint32)
+    outs = [tuple(outp[j, out_offs[i]:out_offs[i + 1]] for j in range(3))
+            for i in range(len(reqs))]
+    try:
         # warm the server's buffers and kernels on a few requests
-        for t in [srv.submit(*reqs[i], out=outs[i]) for i in range(8)]:
+        for t in [srv.submit(*reqs[i], out=outs[i]) for i in range(min(8, len(reqs)))]:
             srv.wait(t)
         tickets = [None] * len(reqs)
         arrivals = np.zeros(len(reqs), dtype=np.int64)
-        base = time.monotonic_ns() + 2_000_000
-        sched = base + np.cumsum(gaps * 1e9).astype(np.int64)
+        sched = time.monotonic_ns() + 2_000_000 + np.cumsum(gaps * 1e9).astype(np.int64)
 
         def submitter():
             for i, r in enumerate(reqs):
@@ -166,76 +161,213 @@ def serving(args):
                 time.sleep(0.0002)
             results.append(srv.wait(tickets[i]))
         th.join()
-        nb, nq = srv.counters()
+        nb, _ = srv.counters()
+    finally:
         srv.close()
-        done = np.array([r[0]["done_ns"] for r in results], dtype=np.int64)
-        if max_batch_requests > 1:
-            by = {}
-            for (res, _), (_, _, _, nn), a in zip(results, reqs, arrivals):
-                key = res["start_ns"]
-                e = by.setdefault(key, [res["admit_ns"], res["start_ns"], res["done_ns"], 0, 0,
-                                        res["solved_ns"]])
-                e[3] += 1
-                e[4] += nn
-            t0b = min(v[0] for v in by.values())
-            for v in sorted(by.values()):
-                log(f"[serve]   batch: {v[3]} req, {v[4]} vertices, admit {(v[0]-t0b)/1e6:.1f} ms, "
-                    f"start +{(v[1]-v[0])/1e6:.1f} ms, solve {(v[5]-v[1])/1e6:.1f} ms, "
-                    f"deliver {(v[2]-v[5])/1e6:.1f} ms")
-        lat = (done - arrivals) / 1e6
-        span = (done.max() - arrivals.min()) / 1e9
-        bad = sum(r[0]["status"] != 0 for r in results)
-        return {"req_per_s": len(reqs) / span, "p50_ms": float(np.percentile(lat, 50)),
-                "p99_ms": float(np.percentile(lat, 99)), "batches": nb - 1, "failed": bad,
-                "span_s": span}, results
+    done = np.array([r[0]["done_ns"] for r in results], dtype=np.int64)
+    lat = (done - arrivals) / 1e6
+    span = (done.max() - arrivals.min()) / 1e9
+    fig = {"req_per_s": round(len(reqs) / span, 1),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "batches": int(nb),
+           "largest_batch": int(max(r[0]["batch_requests"] for r in results)),
+           "failed": int(sum(r[0]["status"] != 0 for r in results)),
+           "span_s": round(span, 3)}
+    return fig, results
 
-    batched, rb = run(4096)
-    log(f"[serve] batched: {batched}")
-    alone, ra = run(1)
-    log(f"[serve] one graph per launch: {alone}")
-    identical = all(x[0]["total_weight"] == y[0]["total_weight"] and
+
+def serving_mode(args, specs, offs, pool, rng, inputs):
+    """Batched and one-graph-per-launch streams on the same arrival trace,
+    with the request arrays in HBM (inputs="device") or in pinned host memory
+    (inputs="pinned"); results are compared request by request."""
+    import numpy as np
+    import torch
+
+    host_memory = "device"
+    if inputs == "device":
+        src = pool
+        reqs = [(src[0, offs[i]:offs[i + 1]], src[1, offs[i]:offs[i + 1]],
+                 src[2, offs[i]:offs[i + 1]], n) for i, (n, _) in enumerate(specs)]
+    else:
+        # pinned blocks freed earlier in the run stay cached by torch: return
+        # them before the large allocation
+        if hasattr(torch._C, "_host_emptyCache"):
+            torch._C._host_emptyCache()
+        try:
+            hp = torch.empty(pool.shape, dtype=torch.int32, pin_memory=True)
+            host_memory = "pinned"
+        except RuntimeError as e:
+            log(f"[serve] pinned allocation failed ({e}); pageable host memory")
+            hp = torch.empty(pool.shape, dtype=torch.int32)
+            host_memory = "pageable"
+        hp.copy_(pool)
+        torch.cuda.synchronize()
+        pv = hp.numpy().view(np.uint32)
+        reqs = [(pv[0, offs[i]:offs[i + 1]], pv[1, offs[i]:offs[i + 1]],
+                 pv[2, offs[i]:offs[i + 1]], n) for i, (n, _) in enumerate(specs)]
+    gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
+    batched, rb = serve_stream(args, reqs, specs, gaps, len(reqs))
+    log(f"[serve] {inputs} inputs, batched: {batched}")
+    alone, ra = serve_stream(args, reqs, specs, gaps, 1)
+    log(f"[serve] {inputs} inputs, one graph per launch: {alone}")
+    identical = all(x[0]["status"] == 0 and x[0]["total_weight"] == y[0]["total_weight"] and
+                    x[0]["mst_edges"] == y[0]["mst_edges"] and
                     all(np.array_equal(p, q) for p, q in zip(x[1], y[1]))
                     for x, y in zip(rb, ra))
+    return {"inputs": host_memory, "batched": batched, "one_graph_per_launch": alone,
+            "speedup_req_per_s": round(batched["req_per_s"] / alone["req_per_s"], 2),
+            "identical_results": bool(identical)}
+
+
+def serving_figures(args, ctx):
+    specs, offs, pool, rng = serving_requests(args, ctx)
+    modes = []
+    for inputs in ("device", "pinned"):
+        modes.append(serving_mode(args, specs, offs, pool, rng, inputs))
+    del pool
+    return {
+        "workload": (f"{args.requests} GNM requests, n log-uniform [2^8,2^18], avg degree "
+                     f"U[8,32], weights U[1,254], seed base + i, Poisson arrivals "
+                     f"{args.rate:g}/s"),
+        "directed_records": int(offs[-1]),
+        "algorithm": "filter_boruvka",
+        "max_batch_records": args.max_batch_records or (1 << 28),
+        "latency": "arrival to result on the host (includes upload from host memory when the "
+                   "inputs are there, and the download of the MST)",
+        "modes": modes,
+    }
+
+
+def serving_only(args):
+    import kamsta_amd as k
+    if args.seed is None:
+        args.seed = 2
+    ctx = k.Context(0)
+    try:
+        specs, offs, pool, rng = serving_requests(args, ctx)
+        mode = serving_mode(args, specs, offs, pool, rng,
+                            "device" if args.device_inputs else "pinned")
+        del pool
+    finally:
+        ctx.close()
     line = {
         "metric": "batched-serving req/s + p50/p99",
-        "value": round(batched["req_per_s"], 1),
+        "value": mode["batched"]["req_per_s"],
         "unit": "requests/s",
         "n_gpus": 1,
         "higher_is_better": True,
         "vs_baseline": None,
         "dtype": "u32/u64",
-        "data": f"synthetic: {args.requests} GNM requests, n log-uniform [2^8,2^18], "
-                f"avg degree U[8,32], weights U[1,254], Poisson arrivals {args.rate:g}/s, "
-                + ("inputs resident in HBM (latency includes D2H of results)" if args.device_inputs
-                   else f"inputs in {host_memory} host memory (latency includes H2D and D2H)"),
+        "data": f"synthetic: {args.requests} GNM requests, inputs in {mode['inputs']} memory",
         "config": {"workload": "batched serving", "requests": args.requests, "rate": args.rate,
-                   "directed_records": total, "algorithm": "filter_boruvka",
-                   "max_batch_records": args.max_batch_records or (1 << 28),
-                   "persistent_max": (args.persistent_max if args.persistent_max is not None
-                                      else k.Tuning().persistent_max)},
-        "batched": {kk: (round(v, 3) if isinstance(v, float) else v) for kk, v in batched.items()},
-        "one_graph_per_launch": {kk: (round(v, 3) if isinstance(v, float) else v)
-                                 for kk, v in alone.items()},
-        "speedup_req_per_s": round(batched["req_per_s"] / alone["req_per_s"], 2),
-        "identical_results": bool(identical),
+                   "directed_records": int(offs[-1]), "algorithm": "filter_boruvka"},
+        **mode,
     }
     print(json.dumps(line), flush=True)
 
 
+# ------------------------------------------------------------------ CPU baselines
+def cpu_baselines(args, ctx, full, threads):
+    """The oracle (the CPU restatement of hybridMST::boruvka) timed on the
+    host cores, one process (MPI ranks = 1) x T OpenMP threads, on
+    - GNM n=2^14, m=2^18, seed 0x4B414D5354410001, at
+      exactly 4 threads, local_preprocessing 1 (the driver default: local
+      filter-Boruvka) and 0 (boruvka_core rounds);
+    - the headline input at full size with `threads` threads (and, with
+      --cpu-c2-4threads, at 4).
+    Each GPU result on the same arrays is checked identical."""
+    import numpy as np
+    import torch
+    import kamsta_amd as k
+    import oracle
+
+    def gpu_solve(s, d, w, N, n, algorithm=k.FILTER_BORUVKA):
+        outs = [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3)]
+        cnt, tw = ctx.solve_device(s, d, w, N, n, *outs, n, algorithm=algorithm)
+        torch.cuda.synchronize()
+        return cnt, tw, [o[:cnt].cpu().numpy().view(np.uint32) for o in outs]
+
+    def same(g, cm, cw):
+        cnt, tw, (gs, gd, gw) = g
+        return bool(tw == cw and cnt == len(cm) and np.array_equal(gs, cm["src"]) and
+                    np.array_equal(gd, cm["dst"]) and np.array_equal(gw, cm["weight"]))
+
+    def timed(rec, level, thr, reps):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            cm, cw = oracle.boruvka(rec, level, thr)
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        return ts[len(ts) // 2], cm, cw
+
+    runs = []
+    # config 1
+    n1, m1 = 1 << 14, 1 << 18
+    arr = [torch.empty(2 * m1, dtype=torch.int32, device="cuda") for _ in range(3)]
+    ctx.gen_gnm_device(n1, m1, 254, 0x4B414D5354410001, *arr)
+    host = [a.cpu().numpy().view(np.uint32) for a in arr]
+    rec = oracle.wedges_from_soa(*host)
+    g1 = gpu_solve(*arr, 2 * m1, n1, k.BORUVKA)
+    for level in (1, 0):
+        oracle.boruvka(rec, level, 4)  # warm-up
+        t, cm, cw = timed(rec, level, 4, 5)
+        runs.append({"config": "GNM n=2^14 m=2^18 weights U[1,254] seed 0x4B414D5354410001",
+                     "algorithm": "boruvka, local_preprocessing %d (%s)" % (
+                         level, "local filter-Boruvka" if level else "boruvka_core rounds"),
+                     "processes": 1, "threads": 4, "seconds": round(t, 4),
+                     "value": round(m1 / t, 1), "unit": "input edges/s",
+                     "timing": "median of 5 after a warm-up",
+                     "gpu_identical": same(g1, cm, cw)})
+        log(f"[bench] cpu baseline: {runs[-1]}")
+    del arr, host, rec
+    # the headline input at full size
+    s, d, w, N, n, m = full
+    t0 = time.perf_counter()
+    host = [a.cpu().numpy().view(np.uint32) for a in (s, d, w)]
+    rec = oracle.wedges_from_soa(*host)
+    del host
+    log(f"[bench] cpu baseline: full input on the host in {time.perf_counter() - t0:.1f}s")
+    g2 = gpu_solve(s, d, w, N, n)
+    plan = [threads] + ([4] if args.cpu_c2_4threads and threads != 4 else [])
+    for thr in plan:
+        t, cm, cw = timed(rec, 1, thr, 1)
+        runs.append({"config": "GNM n=2^24 m=2^28 weights U[1,254] seed %d" % args.seed,
+                     "algorithm": "boruvka, local_preprocessing 1 (local filter-Boruvka; on one "
+                                  "process filter_boruvka computes the same)",
+                     "processes": 1, "threads": thr, "seconds": round(t, 3),
+                     "value": round(m / t, 1), "unit": "input edges/s",
+                     "timing": "one run (no warm-up: a run is tens of seconds)",
+                     "gpu_identical": same(g2, cm, cw)})
+        log(f"[bench] cpu baseline: {runs[-1]}")
+    if not args.cpu_c2_4threads:
+        runs.append({"config": "GNM n=2^24 m=2^28", "processes": 1, "threads": 4,
+                     "skipped": "about four minutes of CPU at 4 threads, beyond the bench's time "
+                                "budget; run bench.py --cpu-c2-4threads for it"})
+    del rec
+    head = runs[2] if len(runs) > 2 else runs[0]
+    return {"value": head["value"], "unit": "input edges/s", "cores": head["threads"],
+            "kind": "port",
+            "sample": (f"{head['config']}, oracle {head['algorithm']}, 1 process x "
+                       f"{head['threads']} OpenMP threads, {head['seconds']} s; GPU result "
+                       f"identical: {head['gpu_identical']}"),
+            "build": "g++ -O3 -march=x86-64-v3 -fopenmp (the reference builds -march=native; "
+                     "the oracle library is built once and runs on the host of another machine)",
+            "host_cpus": len(os.sched_getaffinity(0)),
+            "runs": runs}
+
+
+# ------------------------------------------------------------------ throughput
 def main():
     args = parse()
-    if args.serving:
-        if args.seed is None:
-            args.seed = 2
-        if args.gpus != 1:
-            print(json.dumps({"error": "one GPU only", "n_gpus": args.gpus}))
-            sys.exit(2)
-        serving(args)
-        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                     "note only (see DESIGN.md)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.serving:
+        serving_only(args)
+        return
     import numpy as np
     import torch
     import kamsta_amd as k
@@ -259,31 +391,33 @@ def main():
     if args.persistent_max is not None:
         ctx.set_tuning(dataclasses.replace(ctx.get_tuning(), persistent_max=args.persistent_max))
     t0 = time.time()
-    if args.graph == "gnm":
-        log(f"[bench] generating GNM n=2^{args.log_n} m=2^{args.log_m} on device")
-        s = torch.empty(N, dtype=torch.int32, device=dev)
-        d = torch.empty(N, dtype=torch.int32, device=dev)
-        w = torch.empty(N, dtype=torch.int32, device=dev)
-        ctx.gen_gnm_device(n, m, 254, args.seed, s, d, w)
-    else:
+
+    def generate(seed):
+        if args.graph == "gnm":
+            a = [torch.empty(N, dtype=torch.int32, device=dev) for _ in range(3)]
+            ctx.gen_gnm_device(n, m, 254, seed, *a)
+            return a
         if args.graph == "rgg2d":
-            hs, hd, hw = k.gen_rgg2d_host(n, 16.0, (1 << 31) - 1, args.seed)
-            workload = f"RGG2D n=2^{args.log_n} avg degree 16 weights U[1,2^31-1]"
-        else:
-            hs, hd, hw = k.gen_rhg_host(n, 32.0, 2.8, 254, args.seed)
-            workload = f"RHG n=2^{args.log_n} avg degree 32 gamma 2.8 weights U[1,254]"
-        N = len(hs)
-        m = N // 2
-        s, d, w = (torch.from_numpy(a.view(np.int32)).to(dev) for a in (hs, hd, hw))
-        del hs, hd, hw
+            return ctx.gen_rgg2d_device(n, 16.0, (1 << 31) - 1, seed)
+        return ctx.gen_rhg_device(n, 32.0, 2.8, 254, seed)
+
+    log(f"[bench] generating {args.graph} n=2^{args.log_n}")
+    s, d, w = generate(args.seed)
+    if args.graph == "rgg2d":
+        workload = f"RGG2D n=2^{args.log_n} avg degree 16 weights U[1,2^31-1]"
+    elif args.graph == "rhg":
+        workload = f"RHG n=2^{args.log_n} avg degree 32 gamma 2.8 weights U[1,254]"
+    N = s.numel()
+    m = N // 2
     torch.cuda.synchronize()
     log(f"[bench] {workload}: {N} records, generated in {time.time() - t0:.2f}s")
     outs = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(3)]
     outs_f = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(3)]
     cfg = k.AlgorithmConfig(filter_threshold=args.filter_threshold)
 
-    def solve(algo, o):
-        return ctx.solve_device(s, d, w, N, n, o[0], o[1], o[2], n, algorithm=algo, config=cfg)
+    def solve(algo, o, c=None):
+        return ctx.solve_device(s, d, w, N, n, o[0], o[1], o[2], n, algorithm=algo,
+                                config=c or cfg)
 
     # exactness across algorithms (size-independent: the MSF is unique)
     if args.skip_identity:
@@ -300,20 +434,43 @@ def main():
             print(json.dumps({"error": "plain and filter MSTs differ"}))
             sys.exit(1)
 
-    # autotuner: filter threshold and sort launch shapes, timed on this input
-    # (every candidate is checked to return the same MST)
+    def time_steps(algo, steps, warmup, c=None):
+        for _ in range(warmup):
+            solve(algo, outs, c)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            last = solve(algo, outs, c)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000.0 / steps, last
+
+    # the library defaults before any tuning (filter threshold 4, default
+    # launch shapes), for comparison with the tuned figure
+    default_ms = None
+    if not args.no_autotune:
+        default_ms, _ = time_steps(k.FILTER_BORUVKA, args.steps, args.warmup)
+        log(f"[bench] filter_boruvka with the library defaults: {default_ms:.3f} ms/step")
+
+    # autotuner: filter threshold and sort launch shapes, timed on another
+    # input of the same shape (seed + 1), not on the timed one; every
+    # candidate is checked to return the same MST
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        res = ctx.autotune(s, d, w, N, n, algorithm=k.FILTER_BORUVKA, config=cfg, repeats=3)
+        ts, td, tw_ = generate(args.seed + 1)
+        res = ctx.autotune(ts, td, tw_, ts.numel(), n, algorithm=k.FILTER_BORUVKA, config=cfg,
+                           repeats=3)
+        del ts, td, tw_
         cfg = k.AlgorithmConfig(filter_threshold=res["filter_threshold"])
-        tune = {"filter_threshold": res["filter_threshold"],
+        tune = {"input": f"seed {args.seed + 1} (same shape as the timed input)",
+                "filter_threshold": res["filter_threshold"],
                 "sort_threads": res["tuning"].sort_threads,
                 "balanced_digits": res["tuning"].balanced_digits,
                 "sort_wide_min": res["tuning"].sort_wide_min,
                 "persistent_max": res["tuning"].persistent_max,
                 "best_ms": round(res["best_ms"], 3), "candidates": res["candidates"],
-                "seconds": round(time.perf_counter() - t0, 2)}
+                "seconds": round(time.perf_counter() - t0, 2),
+                "default_ms_per_step": round(default_ms, 3)}
         log(f"[bench] autotune: {tune}")
 
     def timed(algo, steps, warmup):
@@ -322,14 +479,7 @@ def main():
         launch (on the library's stream) for the kernel breakdown and the
         roofline.  The events widen the gaps between kernels, not the
         kernels, so the per-kernel times hold for the first pass too."""
-        for _ in range(warmup):
-            solve(algo, outs)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            last = solve(algo, outs)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
+        ms, last = time_steps(algo, steps, warmup)
         ctx.set_timing(True)
         t2 = time.perf_counter()
         for _ in range(steps):
@@ -340,7 +490,7 @@ def main():
         ctx.set_timing(False)
         st["last_result"] = last
         st["ms_per_step_instrumented"] = (t3 - t2) * 1000.0 / steps
-        return (t1 - t0) * 1000.0 / steps, st
+        return ms, st
 
     results = {}
     algos = {"boruvka": k.BORUVKA, "filter_boruvka": k.FILTER_BORUVKA}
@@ -397,32 +547,6 @@ def main():
                            f"pass of {args.steps} steps after the timed region "
                            f"({st['ms_per_step_instrumented']:.3f} ms/step with the events)")}
 
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        cn, cm = 1 << args.cpu_log_n, 1 << args.cpu_log_m
-        hs, hd, hw = (torch.empty(2 * cm, dtype=torch.int32, device=dev) for _ in range(3))
-        ctx.gen_gnm_device(cn, cm, 254, args.seed, hs, hd, hw)
-        rec = k.to_wedges(hs.cpu().numpy().view(np.uint32), hd.cpu().numpy().view(np.uint32),
-                          hw.cpu().numpy().view(np.uint32))
-        cores = min(os.cpu_count() or 1, 16)
-        ws, wd, ww = k.gen_gnm_host(1 << 10, 1 << 13, 254, 1)
-        oracle.boruvka(k.to_wedges(ws, wd, ww), 1, cores)  # load + warm
-        t0 = time.perf_counter()
-        cmst, cw = oracle.boruvka(rec, 1, cores)
-        ct = time.perf_counter() - t0
-        # the sample's GPU result must match the oracle exactly
-        os2 = [torch.empty(cn, dtype=torch.int32, device=dev) for _ in range(3)]
-        gcnt, gw = ctx.solve_device(hs, hd, hw, 2 * cm, cn, os2[0], os2[1], os2[2], cn)
-        ok = (gw == cw and gcnt == len(cmst) and
-              np.array_equal(os2[0][:gcnt].cpu().numpy().view(np.uint32), cmst["src"]) and
-              np.array_equal(os2[1][:gcnt].cpu().numpy().view(np.uint32), cmst["dst"]))
-        cpu = {"value": round(cm / ct, 1), "unit": "input edges/s", "cores": cores, "kind": "port",
-               "sample": f"GNM n=2^{args.cpu_log_n} m=2^{args.cpu_log_m} seed {args.seed}, "
-                         f"oracle local filter-Boruvka (1 process x {cores} OpenMP threads), "
-                         f"{ct:.2f} s; GPU result on the sample identical: {ok}"}
-        log(f"[bench] cpu baseline: {cpu}")
-
     # PCIe-inclusive time of the best algorithm: input uploaded from host
     # memory (pinned when the allocation succeeds), solve, MST copied back
     pcie = None
@@ -460,16 +584,13 @@ def main():
     except RuntimeError as e:
         log(f"[bench] PCIe-inclusive run skipped: {e}")
 
-    if args.verify_full:
-        import oracle
-        rec = k.to_wedges(s.cpu().numpy().view(np.uint32), d.cpu().numpy().view(np.uint32),
-                          w.cpu().numpy().view(np.uint32))
-        t0 = time.perf_counter()
-        km, kw = oracle.kruskal(rec)
-        ok = (kw == tw_p and len(km) == cnt_p and
-              np.array_equal(outs_f[0][:cnt_p].cpu().numpy().view(np.uint32), km["src"]) and
-              np.array_equal(outs_f[1][:cnt_p].cpu().numpy().view(np.uint32), km["dst"]))
-        log(f"[bench] full Kruskal check: {ok} ({time.perf_counter() - t0:.1f}s)")
+    serving = None
+    if not args.no_serving and args.requests > 0:
+        serving = serving_figures(args, ctx)
+
+    cpu = None
+    if not args.no_cpu and args.graph == "gnm":
+        cpu = cpu_baselines(args, ctx, (s, d, w, N, n, m), cpu_threads(args))
 
     line = {
         "metric": "input edges/s at GNM 2^28 (bit-exact MST)" if args.graph == "gnm"
@@ -484,8 +605,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32/u64",
-        "data": ("synthetic (device GNM generator, seed %d)" if args.graph == "gnm"
-                 else "synthetic (host generator, seed %d)") % args.seed,
+        "data": "synthetic (device %s generator, seed %d)" % (args.graph.upper(), args.seed),
         "config": {"workload": workload,
                    "algorithm": best, "filter_threshold": cfg.filter_threshold,
                    "persistent_max": ctx.get_tuning().persistent_max,
@@ -493,7 +613,8 @@ def main():
                    "directed_records": N,
                    "mst_edges": cnt_p, "mst_weight": tw_p,
                    "ms_per_step": {a: round(results[a][0], 3) for a in results},
-                   "pcie_inclusive": pcie},
+                   "pcie_inclusive": pcie,
+                   "serving": serving},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -203,6 +203,20 @@ kmst_status kmst_gen_rgg2d_host(uint64_t n, double avg_degree, uint32_t max_weig
 kmst_status kmst_gen_rhg_host(uint64_t n, double avg_degree, double gamma, uint32_t max_weight,
                               uint64_t seed, uint32_t* src, uint32_t* dst, uint32_t* weight,
                               uint64_t capacity, uint64_t* m_directed);
+/* The same graphs generated on the device into device arrays, identical to
+ * the host versions record for record (RHG's floating point is built from
+ * IEEE add, multiply, divide and square root only, evaluated alike on both
+ * sides).  Same capacity rule: *m_directed always receives the record
+ * count, and KMST_EINVAL is returned (nothing written) when capacity is
+ * smaller.  Replaces the in-process KaGen generation of the reference driver
+ * (benchmarks/mst_benchmarks.hpp:116-199). */
+kmst_status kmst_gen_rgg2d_device(kmst_context* ctx, uint64_t n, double avg_degree,
+                                  uint32_t max_weight, uint64_t seed, uint32_t* src,
+                                  uint32_t* dst, uint32_t* weight, uint64_t capacity,
+                                  uint64_t* m_directed);
+kmst_status kmst_gen_rhg_device(kmst_context* ctx, uint64_t n, double avg_degree, double gamma,
+                                uint32_t max_weight, uint64_t seed, uint32_t* src, uint32_t* dst,
+                                uint32_t* weight, uint64_t capacity, uint64_t* m_directed);
 
 /* ---- Batched serving ----------------------------------------------------
  * A persistent request loop on one GPU: requests (host SoA edge lists with

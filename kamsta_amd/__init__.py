@@ -18,10 +18,12 @@ reference, which aborts, contract violations raise ``KmstError`` with the C
 status code.  All compute runs on the GPU through libkmst; there is no CPU
 fallback, and a missing library raises ``ImportError``.
 """
+import atexit
 import ctypes
 import dataclasses
 import json
 import os
+import weakref
 
 import numpy as np
 
@@ -38,7 +40,7 @@ BORUVKA, FILTER_BORUVKA = 0, 1
 
 # every symbol include/kmst.h declares
 EXPORTED_SYMBOLS = [
-    "kmst_gen_rgg2d_host", "kmst_gen_rhg_host",
+    "kmst_gen_rgg2d_host", "kmst_gen_rhg_host", "kmst_gen_rgg2d_device", "kmst_gen_rhg_device",
     "kmst_config_default", "kmst_context_create", "kmst_context_destroy", "kmst_context_trim",
     "kmst_context_reserve",
     "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_context_set_timing",
@@ -128,6 +130,29 @@ class AlgorithmConfig:
 
 _lib = None
 
+# Contexts and servers still open when the interpreter exits are closed by an
+# atexit hook, which runs before the HIP runtime's own exit handlers; a
+# close from __del__ during interpreter teardown could come after them.
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live():
+    for obj in sorted(list(_live), key=lambda o: o._order, reverse=True):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
+_order = [0]
+
+
+def _track(obj):
+    _order[0] += 1
+    obj._order = _order[0]
+    _live.add(obj)
+
 
 def load():
     """Load libkmst.so (built by kamsta_amd.build); raise if it is missing."""
@@ -162,6 +187,10 @@ def load():
         "kmst_gen_rgg2d_host": (i32, [u64, ctypes.c_double, u32, u64, vp, vp, vp, u64, pu64]),
         "kmst_gen_rhg_host": (i32, [u64, ctypes.c_double, ctypes.c_double, u32, u64, vp, vp, vp,
                                     u64, pu64]),
+        "kmst_gen_rgg2d_device": (i32, [vp, u64, ctypes.c_double, u32, u64, vp, vp, vp, u64,
+                                        pu64]),
+        "kmst_gen_rhg_device": (i32, [vp, u64, ctypes.c_double, ctypes.c_double, u32, u64, vp, vp,
+                                      vp, u64, pu64]),
         "kmst_k_segmented_argmin": (i32, [vp, vp, vp, vp, u64, u64, vp, vp]),
         "kmst_k_sort_pairs": (i32, [vp, vp, vp, vp, u64, u32]),
         "kmst_k_pointer_jump": (i32, [vp, vp, u64]),
@@ -289,6 +318,7 @@ class Context:
         self._h = ctypes.c_void_p()
         _check(self._lib.kmst_context_create(device, ctypes.byref(self._h)))
         self.device = device
+        _track(self)
 
     def close(self):
         if self._h:
@@ -354,6 +384,37 @@ class Context:
         _check(self._lib.kmst_gen_gnm_device(self._h, n, m, max_weight, seed, _ptr(src),
                                              _ptr(dst), _ptr(weight)))
 
+    def _gen_device_sized(self, call, guess):
+        import torch
+        cap = max(int(guess), 16)
+        while True:
+            arrs = [torch.empty(cap, dtype=torch.int32, device=f"cuda:{self.device}")
+                    for _ in range(3)]
+            m = ctypes.c_uint64(0)
+            st = call(*(_ptr(a) for a in arrs), cap, ctypes.byref(m))
+            if st == KMST_OK:
+                return [a[: m.value] for a in arrs]
+            if st == KMST_EINVAL and m.value > cap:
+                del arrs
+                cap = m.value
+                continue
+            _check(st)
+
+    def gen_rgg2d_device(self, n, avg_degree=16.0, max_weight=(1 << 31) - 1, seed=3):
+        """RGG2D generated on the device (kmst_gen_rgg2d_device): three int32
+        tensors (src, dst, weight) on this context's device, identical to
+        gen_rgg2d_host."""
+        return self._gen_device_sized(lambda s, d, w, cap, m: self._lib.kmst_gen_rgg2d_device(
+            self._h, n, float(avg_degree), max_weight, seed, s, d, w, cap, m),
+            n * avg_degree * 1.1 + 1024)
+
+    def gen_rhg_device(self, n, avg_degree=32.0, gamma=2.8, max_weight=254, seed=4):
+        """RHG generated on the device (kmst_gen_rhg_device), identical to
+        gen_rhg_host."""
+        return self._gen_device_sized(lambda s, d, w, cap, m: self._lib.kmst_gen_rhg_device(
+            self._h, n, float(avg_degree), float(gamma), max_weight, seed, s, d, w, cap, m),
+            n * avg_degree * 1.2 + 1024)
+
     def segmented_argmin(self, src, dst, val, m, n, best_val, best_dst):
         _check(self._lib.kmst_k_segmented_argmin(self._h, _ptr(src), _ptr(dst), _ptr(val), m, n,
                                                  _ptr(best_val), _ptr(best_dst)))
@@ -393,6 +454,7 @@ class Server:
         self._h = ctypes.c_void_p()
         _check(self._lib.kmst_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         self._live = {}
+        _track(self)
 
     def submit(self, src, dst, weight, n, out=None):
         """src/dst/weight: numpy uint32 arrays (host) or torch tensors (host

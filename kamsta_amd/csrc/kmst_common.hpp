@@ -110,6 +110,14 @@ struct KTScope {
   KTScope& operator=(const KTScope&) = delete;
 };
 
+// SplitMix64 finaliser: the hash behind every deterministic generator.
+__host__ __device__ __forceinline__ u64 mix64(u64 x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
 // ---------------------------------------------------------------- device side
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 __device__ __forceinline__ int wave_id() { return threadIdx.x / WAVE; }

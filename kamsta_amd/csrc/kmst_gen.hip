@@ -18,13 +18,6 @@
 
 namespace kmst {
 
-__host__ __device__ __forceinline__ u64 mix64(u64 x) {  // SplitMix64 finaliser
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-
 constexpr u64 WEIGHT_SALT = 0x5745494748543A31ull;
 
 __host__ __device__ __forceinline__ void gnm_candidate(u64 seed_mix, u64 n, u64 c, u32& a, u32& b) {

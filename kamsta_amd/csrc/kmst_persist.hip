@@ -221,15 +221,18 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
         if (bv == VAL_NONE) {
           a.parent[v] = static_cast<u32>(v);
         } else {
-          u32 o = a.bdst[v];
-          if (o >= n) {  // no record bound the minimum: never so for a consistent state
+          const u32 o = a.bdst[v];
+          if (o >= n) {
+            // no record bound the minimum: never so for a consistent state;
+            // the vertex stays a root and emits nothing, the host reports it
             atomicOr(a.err, ERR_BOUNDS);
             why(a.ctl, WHY_BIND, r);
-            o = static_cast<u32>(v);
+            a.parent[v] = static_cast<u32>(v);
+          } else {
+            const bool root = v < o && a.best[o] == bv;  // no gather above the target
+            a.parent[v] = root ? static_cast<u32>(v) : o;
+            em = !root;
           }
-          const bool root = v < o && a.best[o] == bv;  // no gather above the target
-          a.parent[v] = root ? static_cast<u32>(v) : o;
-          em = !root;
           ++live;
         }
       }
@@ -366,6 +369,11 @@ __global__ __launch_bounds__(PB) void persistent_rounds_kernel(PersistArgs a) {
     keys = KeySrc{ov, nullptr, 0};
     ++r;
     if (m == 0) break;
+    // more kept records than the output holds (never so for a consistent
+    // state: a round keeps at most the records it read): ERR_BOUNDS is
+    // raised above, and every workgroup reads the same count after the
+    // barrier, so all of them leave here before anything reads past cap
+    if (m > a.cap) break;
     if (r >= PERSIST_MAX_ROUNDS) {
       // Boruvka at least halves the vertices with records every round
       if (blockIdx.x == 0 && threadIdx.x == 0) {

@@ -135,9 +135,6 @@ std::vector<DevBuf*> kmst_context::all_bufs() {
 
 namespace kmst {
 
-// The context's tuning onto its sort state (before every solve).
-void apply_tuning(kmst_context& C);
-
 // Device counter slots (u32 words; u64 values at even word offsets >= 16).
 enum Ctr : int {
   C_ERR = 0, C_FWD = 1, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6, C_MAXW = 7,
@@ -466,7 +463,8 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     // minima by atomics over the compaction order; a sort by (src, dst) that
     // keeps the lightest parallel record pays only where it removes most
     // records (a multigraph with m near n^2 records keeps about 2/3 of them)
-    deduped = m > 8 * n * n;
+    // (from 2^28 vertices on, 8 n^2 exceeds every 32-bit record count)
+    deduped = n < (u64{1} << 28) && m > 8 * n * n;
     if (deduped) {
       cur = sort_dedup(cur, alt, bits_for(n));
       grouped = true;
@@ -903,35 +901,25 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
 // =================================================================== C ABI
 namespace {
 
+// The default contexts of the host API (one per device) live until the
+// process ends and are never destroyed: a static destructor would make HIP
+// calls (stream synchronise, frees) after the runtime may already have shut
+// down during exit.  The driver reclaims their memory with the process.
 std::mutex g_default_mu;
-std::map<int, std::unique_ptr<kmst_context>> g_default;
-
-kmst_status fail(const kmst::Error& e) {
-  kmst::g_last_error = e.what();
-  return e.status;
+std::map<int, kmst_context*>& default_contexts() {
+  static auto* m = new std::map<int, kmst_context*>();
+  return *m;
 }
 
 template <typename F>
 kmst_status guarded(F&& f) {
-  try {
-    f();
-    kmst::g_last_error.clear();
-    return KMST_OK;
-  } catch (const kmst::Error& e) {
-    return fail(e);
-  } catch (const std::bad_alloc&) {
-    kmst::g_last_error = "host allocation failed";
-    return KMST_ENOMEM;
-  } catch (const std::exception& e) {
-    kmst::g_last_error = e.what();
-    return KMST_EDEVICE;
-  }
+  return kmst::guarded_call(std::forward<F>(f));
 }
 
 kmst_context* default_context(int device) {
-  auto& slot = g_default[device];
-  if (!slot) slot.reset(new kmst_context(device));
-  return slot.get();
+  auto& slot = default_contexts()[device];
+  if (!slot) slot = new kmst_context(device);
+  return slot;
 }
 
 void check_config(const kmst_config* cfg) {

@@ -2,6 +2,8 @@
 // device workspaces, per-solve statistics and the kernel timer.
 #pragma once
 
+#include <new>
+#include <string>
 #include <vector>
 
 #include "kmst_common.hpp"
@@ -49,6 +51,26 @@ struct SolveStats {
 };
 
 extern thread_local std::string g_last_error;
+
+// Runs f, mapping kmst::Error and host allocation failures to a status and
+// the thread's last-error message (the error convention of every entry point).
+template <typename F>
+kmst_status guarded_call(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return KMST_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return KMST_ENOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return KMST_EDEVICE;
+  }
+}
 
 // pinned host words: counters in [0, 64), then HOST_STAGE_WORDS of staging
 // (vertex samples, or the persistent rounds' control words and log)
@@ -99,6 +121,9 @@ struct kmst_context {
 };
 
 namespace kmst {
+// The context's tuning onto its sort state (before every solve and sort).
+void apply_tuning(kmst_context& C);
+
 // kmst_solve_device for a solve whose vertex ids form ranges [part[r],
 // part[r + 1]) (r < count; part is a device array of count + 1 words) that no
 // record joins, e.g. a batch of requests: the heavy weight classes then skip

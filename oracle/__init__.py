@@ -48,6 +48,8 @@ def lib():
         L.orc_kruskal.argtypes = [p, u64, p, pu64, pu64]
         L.orc_boruvka.argtypes = [p, u64, i32, i32, p, pu64, pu64]
         L.orc_filter_boruvka.argtypes = [p, u64, i32, u64, i32, p, pu64, pu64]
+        L.orc_wedges_from_soa.argtypes = [p, p, p, u64, p]
+        L.orc_wedges_from_soa.restype = None
         for f in (L.orc_pair_records, L.orc_kruskal, L.orc_boruvka, L.orc_filter_boruvka):
             f.restype = ctypes.c_int
         _lib = L
@@ -79,6 +81,17 @@ def pair_records(edges):
     if st != 0:
         raise OracleError(st, "pair_records")
     return uid, n.value
+
+
+def wedges_from_soa(src, dst, weight):
+    """WEDGE_DTYPE records from uint32 (src, dst, weight) arrays (parallel copy)."""
+    s, d, w = (np.ascontiguousarray(a, dtype=np.uint32) for a in (src, dst, weight))
+    assert len(s) == len(d) == len(w)
+    out = np.empty(len(s), dtype=WEDGE_DTYPE)
+    if len(s):
+        lib().orc_wedges_from_soa(s.ctypes.data, d.ctypes.data, w.ctypes.data, len(s),
+                                  out.ctypes.data)
+    return out
 
 
 def kruskal(edges):

@@ -734,10 +734,28 @@ int orc_boruvka(const WEdge* in, u64 N, int local_preprocessing, int threads,
   if (st != OK) return st;
   std::vector<u32> mst;
   if (local_preprocessing != 0) {
-    std::vector<LEdge> local;
-    local.reserve(N / 2);
-    for (u64 i = 0; i < N; ++i)
-      if (in[i].src < in[i].dst) local.push_back(LEdge{u32(in[i].src), u32(in[i].dst), in[i].weight, u32(i)});
+    // the u<v records in input order (blocked count, scan, fill)
+    const int nt = omp_get_max_threads();
+    std::vector<u64> cnt(nt + 1, 0);
+#pragma omp parallel num_threads(nt)
+    {
+      const int t = omp_get_thread_num();
+      const u64 b = N * t / nt, e = N * (t + 1) / nt;
+      u64 c = 0;
+      for (u64 i = b; i < e; ++i) c += in[i].src < in[i].dst;
+      cnt[t + 1] = c;
+    }
+    for (int t = 0; t < nt; ++t) cnt[t + 1] += cnt[t];
+    std::vector<LEdge> local(cnt[nt]);
+#pragma omp parallel num_threads(nt)
+    {
+      const int t = omp_get_thread_num();
+      const u64 b = N * t / nt, e = N * (t + 1) / nt;
+      u64 o = cnt[t];
+      for (u64 i = b; i < e; ++i)
+        if (in[i].src < in[i].dst)
+          local[o++] = LEdge{u32(in[i].src), u32(in[i].dst), in[i].weight, u32(i)};
+    }
     LocalState s;
     s.n_initial = u32(n);
     s.parents.resize(n);
@@ -780,6 +798,13 @@ int orc_filter_boruvka(const WEdge* in, u64 N, int local_preprocessing,
   filter_recursive(c, edges, n, edges.size(), 0, false);
   emit(in, c.mst, out, out_n, total);
   return OK;
+}
+
+// WEdge records from (src, dst, weight) arrays, in parallel (the checks of
+// full-size inputs start from device arrays).
+void orc_wedges_from_soa(const u32* src, const u32* dst, const u32* weight, u64 N, WEdge* out) {
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < i64(N); ++i) out[i] = WEdge{src[i], dst[i], weight[i], 0};
 }
 
 }  // extern "C"
