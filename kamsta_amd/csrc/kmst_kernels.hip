@@ -1245,19 +1245,21 @@ __device__ __forceinline__ void load_records(const u32* __restrict__ s_in,
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void relabel_compact_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 key_min,
     u64 key_max, u64 m, const u32* __restrict__ label, const u64* __restrict__ giant,
     u32* __restrict__ s_out,
     u32* __restrict__ d_out, u64* __restrict__ v_out, u64* lb_status, u32 lb_epoch,
     unsigned long long* lb_counter, unsigned long long lb_base, u32 tiles, u32* count_dev,
     u32* err) {
-  __shared__ u32 s_wave_cnt[WAVES];
+  constexpr int NW = THREADS / WAVE;
+  __shared__ u32 s_wave_cnt[NW];
   __shared__ u64 s_excl;
   __shared__ u32 s_ok;
   const u32 tile = lb_tile_id(lb_counter, lb_base);
   const int lane = lane_id(), w = wave_id();
-  const u64 wbase = u64(tile) * CMP_TILE + u64(w) * (WAVE * CMP_IPT);
+  const u64 wbase = u64(tile) * (THREADS * CMP_IPT) + u64(w) * (WAVE * CMP_IPT);
   u32 ls[CMP_IPT], ld[CMP_IPT];
   u64 vv[CMP_IPT];
   u64 bal[CMP_IPT];
@@ -1297,7 +1299,7 @@ __global__ __launch_bounds__(BLOCK) void relabel_compact_kernel(
   if (w == 0) {
     u64 agg = 0;
 #pragma unroll
-    for (int ww = 0; ww < WAVES; ++ww) agg += s_wave_cnt[ww];
+    for (int ww = 0; ww < NW; ++ww) agg += s_wave_cnt[ww];
     bool ok;
     const u64 excl = lookback_exclusive(lb_status, lb_epoch, tile, agg, err, ok);
     if (lane == 0) {
@@ -1352,7 +1354,11 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
     fill_u32(st, count_dev, 1, 0);
     return;
   }
-  const u32 tiles = ceil_div_u32(m, CMP_TILE);
+  // the look-back passes a tile's offset on in a chain whose cost per tile
+  // bounds large compactions: from RELABEL_WIDE_MIN records on a tile has 512
+  // threads (8192 records), half as many tiles
+  const bool wide = m >= RELABEL_WIDE_MIN;
+  const u32 tiles = ceil_div_u32(m, u64(wide ? 512 : BLOCK) * CMP_IPT);
   const u32 epoch = lb.next_call(st, tiles);
   // reads: record (12 B with computed keys, 16 B stored) + two 4-byte label
   // gathers; with the component bitmap only (src, dst) and the bitmap are
@@ -1360,10 +1366,14 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
   // the caller.
   KTScope t(st, K_RELABEL_SCATTER,
             giant ? 8.0 * m + double(n) / 8.0 : (keys.v ? 16.0 : 12.0) * m + 8.0 * m);
-  relabel_compact_kernel<<<tiles, BLOCK, 0, st>>>(
-      s_in, d_in, keys, key_min, key_max, m, label, giant, s_out, d_out, v_out, lb.words, epoch,
-      reinterpret_cast<unsigned long long*>(lb.words + lb.tiles_cap), lb.tiles_issued, tiles,
-      count_dev, err);
+#define KMST_RELABEL(T)                                                                        \
+  relabel_compact_kernel<T><<<tiles, T, 0, st>>>(                                              \
+      s_in, d_in, keys, key_min, key_max, m, label, giant, s_out, d_out, v_out, lb.words, epoch, \
+      reinterpret_cast<unsigned long long*>(lb.words + lb.tiles_cap), lb.tiles_issued, tiles,  \
+      count_dev, err)
+  if (wide) KMST_RELABEL(512);
+  else KMST_RELABEL(BLOCK);
+#undef KMST_RELABEL
   KMST_HIP(hipGetLastError());
   lb.tiles_issued += tiles;
 }

@@ -118,7 +118,9 @@ struct LookbackWork {
 };
 
 // K4+K5 relabel, drop self-loops and dead records, order-preserving compaction
-// in one pass; *count_dev gets the kept count.
+// in one pass; *count_dev gets the kept count.  Tiles of 512 threads from
+// RELABEL_WIDE_MIN records on, 256 below.
+constexpr u64 RELABEL_WIDE_MIN = u64{1} << 20;
 // Records whose key is below key_min are dropped too (0 keeps all).  With
 // giant (a bitmap over the n vertex ids of one component of label, see
 // component_bits) records with both ends inside it are dropped without reading
