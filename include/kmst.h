@@ -115,6 +115,20 @@ kmst_status kmst_solve_device(kmst_context* ctx, const kmst_dev_edges* in,
                               const kmst_config* cfg, kmst_dev_mst* out,
                               uint64_t* out_n, uint64_t* total_weight);
 
+/* kmst_solve_device without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (serving loops and
+ * harnesses that forbid hipMalloc).  The workspaces come from a fixed device
+ * array inside the library, one per GPU, held until the process ends
+ * (80 MiB); solves of more than 2^20 records or 2^19 vertices are refused
+ * with KMST_EINVAL.  `stream` is the hipStream_t to run on (NULL: the default
+ * stream); cfg->device selects the GPU.  The call makes no hipMalloc,
+ * hipFree, pinned-memory or stream allocation.  Every call on a GPU shares
+ * its array, which keeps the last call's contents; calls are blocking and
+ * serialised per GPU (overlapping calls on one GPU would race on the array). */
+kmst_status kmst_solve_device_noalloc(const kmst_dev_edges* in, const kmst_config* cfg,
+                                      kmst_dev_mst* out, uint64_t* out_n,
+                                      uint64_t* total_weight, void* stream);
+
 /* Per-kernel HIP-event timing of later solves on this context (measurement
  * hook; off by default).  Totals appear in kmst_context_stats_json. */
 void kmst_context_set_timing(kmst_context* ctx, int on);

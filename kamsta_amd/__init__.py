@@ -43,7 +43,8 @@ EXPORTED_SYMBOLS = [
     "kmst_gen_rgg2d_host", "kmst_gen_rhg_host", "kmst_gen_rgg2d_device", "kmst_gen_rhg_device",
     "kmst_config_default", "kmst_context_create", "kmst_context_destroy", "kmst_context_trim",
     "kmst_context_reserve",
-    "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_context_set_timing",
+    "kmst_boruvka", "kmst_filter_boruvka", "kmst_solve_device", "kmst_solve_device_noalloc",
+    "kmst_context_set_timing",
     "kmst_tuning_default", "kmst_context_set_tuning", "kmst_context_get_tuning", "kmst_autotune",
     "kmst_encode_host",
     "kmst_last_error", "kmst_context_stats_json", "kmst_gen_gnm_device", "kmst_gen_gnm_host",
@@ -173,6 +174,7 @@ def load():
         "kmst_boruvka": (i32, [vp, u64, vp, vp, pu64, pu64]),
         "kmst_filter_boruvka": (i32, [vp, u64, vp, vp, pu64, pu64]),
         "kmst_solve_device": (i32, [vp, vp, vp, vp, pu64, pu64]),
+        "kmst_solve_device_noalloc": (i32, [vp, vp, vp, pu64, pu64, vp]),
         "kmst_context_set_timing": (None, [vp, i32]),
         "kmst_tuning_default": (None, [vp]),
         "kmst_encode_host": (i32, [vp, vp, vp, u64, u32, vp, u64, vp, vp, pu64]),
@@ -235,6 +237,24 @@ def boruvka(edges, config=None):
 def filter_boruvka(edges, config=None):
     """hybridMST::filter_boruvka: returns (canonical MST records, u64 total weight)."""
     return _host_solve(load().kmst_filter_boruvka, edges, config or AlgorithmConfig())
+
+
+def solve_device_noalloc(src, dst, weight, m, n, out_src, out_dst, out_weight, capacity,
+                         algorithm=BORUVKA, config=None, stream=None):
+    """kmst_solve_device_noalloc: a device-resident solve that allocates no
+    device memory (the library's fixed per-GPU scratch; at most 2^20 records
+    and 2^19 vertices).  Device arrays are torch tensors or raw addresses;
+    stream is a raw hipStream_t (None: the default stream).  Returns (number
+    of MST edges, u64 weight)."""
+    L = load()
+    cfg = (config or AlgorithmConfig())._c(algorithm)
+    e = _DevEdges(_ptr(src).value, _ptr(dst).value, _ptr(weight).value, m, n)
+    o = _DevMst(_ptr(out_src).value, _ptr(out_dst).value, _ptr(out_weight).value, capacity)
+    n_out, total = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    _check(L.kmst_solve_device_noalloc(ctypes.byref(e), ctypes.byref(cfg), ctypes.byref(o),
+                                       ctypes.byref(n_out), ctypes.byref(total),
+                                       ctypes.c_void_p(stream or 0)))
+    return n_out.value, total.value
 
 
 def gen_gnm_host(n, m, max_weight=254, seed=1):

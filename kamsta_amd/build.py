@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkmst.so")
 SOURCES = ["kmst_kernels.hip", "kmst_solver.hip", "kmst_gen.hip", "kmst_serve.hip",
-           "kmst_codec.hip", "kmst_persist.hip", "kmst_gen_geo.hip"]
+           "kmst_codec.hip", "kmst_persist.hip", "kmst_gen_geo.hip",
+           "kmst_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -83,6 +83,16 @@ const char* kernel_kind_name(int k) {
 // ------------------------------------------------------------ buffers
 void* DevBuf::get(size_t need) {
   if (need == 0) need = 1;
+  if (arena && need > bytes) {
+    const size_t at = (arena->used + 255) & ~size_t{255};
+    if (arena->frozen || at > arena->size || need > arena->size - at)
+      throw Error(KMST_EINVAL, "a workspace of " + std::to_string(need) +
+                                   " bytes exceeds the fixed scratch");
+    p = arena->base + at;
+    bytes = need;
+    arena->used = at + need;
+    return p;
+  }
   if (need > bytes) {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -100,6 +110,7 @@ void* DevBuf::get(size_t need) {
   return p;
 }
 void DevBuf::release() {
+  if (arena) return;  // carved storage is never freed
   if (p) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
@@ -116,7 +127,12 @@ kmst_context::kmst_context(int dev) : device(dev) {
   KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_counters), HOST_COUNTER_BYTES,
                          hipHostMallocDefault));
 }
+kmst_context::kmst_context(int dev, kmst::u32* host_words, kmst::Arena* arena)
+    : device(dev), owns_resources(false), host_counters(host_words) {
+  for (DevBuf* b : all_bufs()) b->arena = arena;
+}
 kmst_context::~kmst_context() {
+  if (!owns_resources) return;
   (void)hipSetDevice(device);
   (void)hipStreamSynchronize(stream);
   for (DevBuf* b : all_bufs()) b->release();

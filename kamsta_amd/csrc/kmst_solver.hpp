@@ -11,11 +11,22 @@
 
 namespace kmst {
 
+// A fixed region of device memory that buffers carve their storage from
+// instead of allocating (the context of the no-allocation entry point).
+// While open, a buffer that must grow takes the next 256-byte aligned bytes;
+// once frozen, a request beyond a buffer's carved size is an error.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  bool frozen = false;
+};
+
 // A device allocation that only grows; reused across solves so steady-state
-// calls do not allocate.
+// calls do not allocate.  With an arena it never allocates (see Arena).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  Arena* arena = nullptr;
   void* get(size_t need);
   void release();
 };
@@ -86,11 +97,16 @@ constexpr size_t HOST_COUNTER_BYTES = (HOST_SAMPLE_WORD + HOST_STAGE_WORDS) * 4;
 
 struct kmst_context {
   explicit kmst_context(int device);
+  // A context that allocates nothing: no stream (the caller sets `stream`),
+  // counters in the caller's host words (HOST_COUNTER_BYTES), workspaces
+  // carved from `arena`.
+  kmst_context(int device, kmst::u32* host_words, kmst::Arena* arena);
   ~kmst_context();
   kmst_context(const kmst_context&) = delete;
   kmst_context& operator=(const kmst_context&) = delete;
 
   int device = 0;
+  bool owns_resources = true;  // created the stream and the pinned counters
   hipStream_t stream = nullptr;
   kmst::u32* host_counters = nullptr;  // pinned, HOST_COUNTER_BYTES
   bool timing = false;
