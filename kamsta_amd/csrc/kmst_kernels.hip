@@ -1245,7 +1245,12 @@ __device__ __forceinline__ void load_records(const u32* __restrict__ s_in,
   }
 }
 
-template <int THREADS>
+// With TWINS the records are symmetric (every (a, b, key) has its (b, a, key)
+// twin, or a lighter parallel record that makes both redundant): only the
+// forward records (src < dst) are read past their ids and gathered, and each
+// kept one is written in both directions, the wave's forward copies first,
+// then its reverse copies in the same order.  Half the random label gathers.
+template <int THREADS, bool TWINS>
 __global__ __launch_bounds__(THREADS) void relabel_compact_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 key_min,
     u64 key_max, u64 m, const u32* __restrict__ label, const u64* __restrict__ giant,
@@ -1272,11 +1277,12 @@ __global__ __launch_bounds__(THREADS) void relabel_compact_kernel(
 #pragma unroll
   for (int k = 0; k < CMP_IPT; ++k) {
     const u64 e = wbase + u64(k) * WAVE + lane;
+    const bool fwd = !TWINS || se[k] < de[k];
     // both ends in the marked component: a self-loop, decided without the
     // key or the label gathers
-    const bool inside = giant && ((giant[se[k] >> 6] >> (se[k] & 63)) &
-                                  (giant[de[k] >> 6] >> (de[k] & 63)) & 1ull);
-    live[k] = e < m && !inside;
+    const bool inside = fwd && giant && ((giant[se[k] >> 6] >> (se[k] & 63)) &
+                                         (giant[de[k] >> 6] >> (de[k] & 63)) & 1ull);
+    live[k] = e < m && fwd && !inside;
   }
 #pragma unroll
   for (int k = 0; k < CMP_IPT; ++k) {
@@ -1294,7 +1300,7 @@ __global__ __launch_bounds__(THREADS) void relabel_compact_kernel(
     bal[k] = __ballot(live[k] && ls[k] != ld[k]);
     cnt += static_cast<u32>(__popcll(bal[k]));
   }
-  if (lane == 0) s_wave_cnt[w] = cnt;
+  if (lane == 0) s_wave_cnt[w] = TWINS ? 2 * cnt : cnt;
   __syncthreads();
   if (w == 0) {
     u64 agg = 0;
@@ -1319,6 +1325,11 @@ __global__ __launch_bounds__(THREADS) void relabel_compact_kernel(
       s_out[p] = ls[k];
       d_out[p] = ld[k];
       v_out[p] = vv[k];
+      if (TWINS) {
+        s_out[p + cnt] = ld[k];
+        d_out[p + cnt] = ls[k];
+        v_out[p + cnt] = vv[k];
+      }
     }
     pos += static_cast<u32>(__popcll(bal[k]));
   }
@@ -1349,7 +1360,7 @@ u32 LookbackWork::next_call(hipStream_t st, u32 tiles) {
 void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                      u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
                      LookbackWork& lb, u32* count_dev, u32* err, const u64* giant, u64 n,
-                     u64 key_max) {
+                     u64 key_max, bool twins) {
   if (m == 0) {
     fill_u32(st, count_dev, 1, 0);
     return;
@@ -1360,19 +1371,23 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
   const bool wide = m >= RELABEL_WIDE_MIN;
   const u32 tiles = ceil_div_u32(m, u64(wide ? 512 : BLOCK) * CMP_IPT);
   const u32 epoch = lb.next_call(st, tiles);
-  // reads: record (12 B with computed keys, 16 B stored) + two 4-byte label
-  // gathers; with the component bitmap only (src, dst) and the bitmap are
+  // reads: (src, dst) 8 B, the key (4 B weight or 8 B stored) and two 4-byte
+  // label gathers -- with twins the key and the gathers of the forward half
+  // only; with the component bitmap only (src, dst) and the bitmap are
   // counted (a lower bound).  The 16-byte writes of kept records are added by
   // the caller.
+  const double per_read = (keys.v ? 8.0 : 4.0) + 8.0;
   KTScope t(st, K_RELABEL_SCATTER,
-            giant ? 8.0 * m + double(n) / 8.0 : (keys.v ? 16.0 : 12.0) * m + 8.0 * m);
-#define KMST_RELABEL(T)                                                                        \
-  relabel_compact_kernel<T><<<tiles, T, 0, st>>>(                                              \
+            giant ? 8.0 * m + double(n) / 8.0 : 8.0 * m + per_read * (twins ? 0.5 : 1.0) * m);
+#define KMST_RELABEL(T, TW)                                                                    \
+  relabel_compact_kernel<T, TW><<<tiles, T, 0, st>>>(                                          \
       s_in, d_in, keys, key_min, key_max, m, label, giant, s_out, d_out, v_out, lb.words, epoch, \
       reinterpret_cast<unsigned long long*>(lb.words + lb.tiles_cap), lb.tiles_issued, tiles,  \
       count_dev, err)
-  if (wide) KMST_RELABEL(512);
-  else KMST_RELABEL(BLOCK);
+  if (wide && twins) KMST_RELABEL(512, true);
+  else if (wide) KMST_RELABEL(512, false);
+  else if (twins) KMST_RELABEL(BLOCK, true);
+  else KMST_RELABEL(BLOCK, false);
 #undef KMST_RELABEL
   KMST_HIP(hipGetLastError());
   lb.tiles_issued += tiles;
@@ -2319,6 +2334,37 @@ void sample_weights(hipStream_t st, const u64* v, const u32* wt, u64 m, u32 k, u
   if (k == 0 || m == 0) return;
   KTScope t(st, K_SAMPLE, 12.0 * k);
   sample_kernel<<<ceil_div_u32(k, BLOCK), BLOCK, 0, st>>>(v, wt, m, k, wshift, out);
+  KMST_HIP(hipGetLastError());
+}
+
+// *out = how many of k evenly spread records (the positions of
+// sample_weights) join ids at least `span` apart.  One workgroup.
+__global__ __launch_bounds__(BLOCK) void sample_far_kernel(const u32* __restrict__ s,
+                                                           const u32* __restrict__ d, u64 m,
+                                                           u32 k, u32 span, u32* __restrict__ out) {
+  __shared__ u32 s_sum[WAVES];
+  u32 far = 0;
+  for (u32 j = threadIdx.x; j < k; j += BLOCK) {
+    const u64 i = (u64(j) * m) / k + m / (2ull * k);
+    const u64 ii = i < m ? i : m - 1;
+    const u32 a = s[ii], b = d[ii];
+    far += (a > b ? a - b : b - a) >= span ? 1u : 0u;
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) far += __shfl_xor(far, o, WAVE);
+  if (lane_id() == 0) s_sum[wave_id()] = far;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 t = 0;
+    for (int w = 0; w < WAVES; ++w) t += s_sum[w];
+    *out = t;
+  }
+}
+
+void sample_far(hipStream_t st, const u32* s, const u32* d, u64 m, u32 k, u32 span, u32* out) {
+  if (k == 0 || m == 0) return;
+  KTScope t(st, K_SAMPLE, 8.0 * k);
+  sample_far_kernel<<<1, BLOCK, 0, st>>>(s, d, m, k, span, out);
   KMST_HIP(hipGetLastError());
 }
 

@@ -124,11 +124,13 @@ constexpr u64 RELABEL_WIDE_MIN = u64{1} << 20;
 // Records whose key is below key_min are dropped too (0 keeps all).  With
 // giant (a bitmap over the n vertex ids of one component of label, see
 // component_bits) records with both ends inside it are dropped without reading
-// their key or labels.
+// their key or labels.  With twins (symmetric records) only the src < dst half
+// is relabelled and each kept record leaves in both directions: half the
+// label gathers, and the output is no longer grouped by src.
 void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                      u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
                      LookbackWork& lb, u32* count_dev, u32* err, const u64* giant = nullptr,
-                     u64 n = 0, u64 key_max = ~0ull);
+                     u64 n = 0, u64 key_max = ~0ull, bool twins = false);
 // bits[v / 64] bit v % 64 = (gpar[v] == root) for v < n; bits needs
 // ceil(n / 64) entries.
 void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits);
@@ -245,6 +247,11 @@ constexpr u32 RUN_SAMPLES = 4096;
 constexpr u64 UNORDERED_DEGREE = 8;
 constexpr u64 UNORDERED_MIN_DEGREE = 2;
 constexpr u64 UNORDERED_SMALL_N = u64{1} << 19;  // 4 MiB of u64 minima
+// Relabels on more vertex ids (a u32 label table past one XCD's 4 MiB L2) of
+// inputs whose sampled records mostly join ids FAR_SPAN or more apart gather
+// the src < dst half only (relabel_compact's twins).
+constexpr u64 TWINS_MIN_N = u64{1} << 20;
+constexpr u32 FAR_SPAN = 1u << 16;
 void sample_run_starts(hipStream_t st, const u32* s, const u32* m_dev, u32* count);
 
 // Filter-Boruvka helpers.
@@ -254,6 +261,8 @@ void update_parent_array(hipStream_t st, u64 n, const u32* rep, const u32* paren
 // weight of a key = key >> wshift (32 for positional keys, 2 bits(n) for packed)
 void sample_weights(hipStream_t st, const u64* v, const u32* wt, u64 m, u32 k, u32 wshift,
                     u32* out);
+// *out = how many of the same k sampled records join ids at least span apart
+void sample_far(hipStream_t st, const u32* s, const u32* d, u64 m, u32 k, u32 span, u32* out);
 void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys,
                        u64 m, u32 pivot, u32 wshift, bool write_heavy, u32* s_out, u32* d_out,
                        u64* v_out, LookbackWork& lb, u32* light_count_dev, u32* err);

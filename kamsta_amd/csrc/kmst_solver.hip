@@ -230,6 +230,9 @@ class Solver {
   void check_input();
   // set by setup when the class scatter validates the input
   bool check_in_scatter = false;
+  // most sampled input records join ids FAR_SPAN or more apart (random
+  // graphs): their relabels gather all over the label table
+  bool far_input = false;
   void plan_pivots();
   void filter_classes();
   void check_errors(const char* where);
@@ -455,8 +458,13 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
                    rep_next, ctr + C_ERR);
     if (track_gpar) std::swap(rep, rep_next);
     RecView out = (cur.s == A.s) ? B : A;
+    // a first round of far-apart ids on a label table past an L2 relabels
+    // only the src < dst half and writes both directions (half the random
+    // gathers); its output is no longer grouped, which costs nothing when the
+    // next round sorts -- few records per vertex, where it does
+    const bool twins = first && grouped && far_input && n > TWINS_MIN_N && m < UNORDERED_DEGREE * n;
     relabel_compact(st, cur.s, cur.d, keys_of(cur), 0, m, label, out.s, out.d, out.v, C.lb_work,
-                    ctr + C_MNEXT, ctr + C_ERR);
+                    ctr + C_MNEXT, ctr + C_ERR, nullptr, 0, ~0ull, twins);
     sample_run_starts(st, out.s, ctr + C_MNEXT, ctr + C_RUNS);
     check_errors("round");
     if (g_ktimer) g_ktimer->adjust_last(K_RELABEL_SCATTER, 16.0 * C.host_counters[C_MNEXT]);
@@ -466,6 +474,7 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     const u64 m_next = C.host_counters[C_MNEXT];
     rs.n_next = n_next;
     rs.m_next = m_next;
+    rs.twins = twins;
     rs.run_starts = m_next ? (C.host_counters[C_RUNS] & 0xFFFFu) : 0;
     rs.far_starts = m_next ? (C.host_counters[C_RUNS] >> 16) : 0;
     C.stats.rounds.push_back(rs);
@@ -643,9 +652,13 @@ u32 Solver::pick_pivot(RecView r) {
 void Solver::plan_pivots() {
   const u32 k = static_cast<u32>(std::min<u64>(4095, m0));
   sample_weights(st, nullptr, in_w, m0, k, 32, samples);
-  std::vector<u32> h(k);
-  KMST_HIP(hipMemcpyAsync(h.data(), samples, k * 4, hipMemcpyDeviceToHost, st));
+  // the same records' id distances decide where relabels gather far apart
+  sample_far(st, in_s, in_d, m0, k, FAR_SPAN, samples + k);
+  std::vector<u32> h(k + 1);
+  KMST_HIP(hipMemcpyAsync(h.data(), samples, (k + 1) * 4, hipMemcpyDeviceToHost, st));
   KMST_HIP(hipStreamSynchronize(st));
+  far_input = 2 * u64(h[k]) > k;
+  h.pop_back();
   std::sort(h.begin(), h.end());
   std::vector<u32> chain;  // descending
   size_t cnt = h.size();
@@ -812,8 +825,11 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
       component_bits(st, gpar, n0, giant_root, best_val);
       giant = best_val;
     }
+    // the heavy records are symmetric and sorted right after: relabel the
+    // src < dst half and write both directions (half the gathers)
     relabel_compact(st, heavy.s, heavy.d, keys_of(heavy), key_min, heavy.m, gpar, scratch.s,
-                    scratch.d, scratch.v, C.lb_work, ctr + C_MNEXT, ctr + C_ERR, giant, n0, key_max);
+                    scratch.d, scratch.v, C.lb_work, ctr + C_MNEXT, ctr + C_ERR, giant, n0, key_max,
+                    /*twins=*/true);
   }
   check_errors("filter");
   const u64 Hk = C.host_counters[C_MNEXT];
@@ -1363,7 +1379,8 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
     o << (i ? "," : "") << "{\"n\":" << r.n << ",\"m\":" << r.m << ",\"n_next\":" << r.n_next
       << ",\"m_next\":" << r.m_next << ",\"run_starts\":" << r.run_starts
       << ",\"far_starts\":" << r.far_starts
-      << ",\"persistent\":" << (r.persistent ? "true" : "false");
+      << ",\"persistent\":" << (r.persistent ? "true" : "false")
+      << ",\"twins\":" << (r.twins ? "true" : "false");
     if (r.persistent)
       o << ",\"phase_us\":[" << r.phase_us[0] << "," << r.phase_us[1] << "," << r.phase_us[2]
         << "," << r.phase_us[3] << "]";

@@ -36,6 +36,7 @@ struct RoundStat {
   u32 run_starts = 0;       // of RUN_SAMPLES sampled next-round records (0: not sampled)
   u32 far_starts = 0;       // ... whose source is 16 or more ids from the record before
   bool persistent = false;  // ran inside the persistent launch
+  bool twins = false;       // relabelled the src < dst half, both directions written
   double phase_us[4] = {0, 0, 0, 0};  // persistent: bind, hook, jump, relabel
 };
 struct FilterStat {

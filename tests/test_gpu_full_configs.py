@@ -113,6 +113,9 @@ def test_gnm_full(gpu, ctx, gnm_full, algorithm):
                    and not r["persistent"]]
         assert sampled, rounds[:4]
         assert any(r["run_starts"] > 0 for r in rounds)
+        # far-apart ids on a large label table: the first round relabels the
+        # src < dst half and writes both directions
+        assert rounds[0]["twins"] and not any(r["twins"] for r in rounds[1:])
 
 
 # ------------------------------------------------------------------ RGG2D 2^24
@@ -145,6 +148,7 @@ def test_rgg2d_full(gpu, ctx, rgg_full, algorithm):
     got = solve(gpu, ctx, arrs, n, algorithm)
     say(f"[full] RGG2D algorithm {algorithm}: {got[0]} edges, weight {got[1]}")
     assert not got[3]["packed_keys"]  # 31-bit weights: positional keys
+    assert not any(r["twins"] for r in got[3]["rounds"])  # neighbours have nearby ids
     assert_same(got, exp, f"RGG2D algorithm {algorithm}")
 
 

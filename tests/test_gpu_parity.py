@@ -314,3 +314,27 @@ def test_compression_level_1(gpu, n, m, mw, seed):
     for fn in (k.boruvka, k.filter_boruvka):
         got, got_w = fn(rec, k.AlgorithmConfig(compression=1))
         assert got_w == exp_w and np.array_equal(strip(got), strip(exp)), fn.__name__
+
+
+@pytest.mark.parametrize("threshold", [2, 4])
+def test_twin_relabels_on_a_random_graph(gpu, ctx, threshold):
+    """A random graph past 2^20 vertices: the first round of the lightest
+    class and every heavy class relabel only the src < dst half of their
+    (symmetric) records and write both directions; the MSF equals the
+    oracle's."""
+    torch = gpu
+    n, m = 1 << 21, 3 << 21
+    arrs = [torch.empty(2 * m, dtype=torch.int32, device="cuda") for _ in range(3)]
+    ctx.gen_gnm_device(n, m, 254, 91, *arrs)
+    outs = [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3)]
+    cnt, tw = ctx.solve_device(*arrs, 2 * m, n, *outs, n, algorithm=k.FILTER_BORUVKA,
+                               config=k.AlgorithmConfig(filter_threshold=threshold))
+    torch.cuda.synchronize()
+    st = ctx.stats()
+    assert st["rounds"][0]["twins"]
+    rec = oracle.wedges_from_soa(*(host(a, np.uint32) for a in arrs))
+    exp, exp_w = oracle.boruvka(rec, 1, 8)
+    assert tw == exp_w and cnt == len(exp)
+    assert np.array_equal(host(outs[0][:cnt], np.uint32), exp["src"].astype(np.uint32))
+    assert np.array_equal(host(outs[1][:cnt], np.uint32), exp["dst"].astype(np.uint32))
+    assert np.array_equal(host(outs[2][:cnt], np.uint32), exp["weight"])
