@@ -191,21 +191,31 @@ def serving_mode(args, specs, offs, pool, rng, inputs):
                  src[2, offs[i]:offs[i + 1]], n) for i, (n, _) in enumerate(specs)]
     else:
         # pinned blocks freed earlier in the run stay cached by torch: return
-        # them before the large allocation
+        # them first.  torch rounds a pinned allocation up to a power of two,
+        # so the requests go into pinned chunks of exactly 2 GiB
         if hasattr(torch._C, "_host_emptyCache"):
             torch._C._host_emptyCache()
-        try:
-            hp = torch.empty(pool.shape, dtype=torch.int32, pin_memory=True)
-            host_memory = "pinned"
-        except RuntimeError as e:
-            log(f"[serve] pinned allocation failed ({e}); pageable host memory")
-            hp = torch.empty(pool.shape, dtype=torch.int32)
-            host_memory = "pageable"
-        hp.copy_(pool)
+        chunk = 1 << 29  # int32 entries
+        chunks, reqs, used = [], [], chunk
+        host_memory = "pinned"
+        for i, (n, m) in enumerate(specs):
+            need = 6 * m
+            if used + need > chunk:
+                try:
+                    chunks.append(torch.empty(max(chunk, need), dtype=torch.int32,
+                                              pin_memory=host_memory == "pinned"))
+                except RuntimeError as e:
+                    log(f"[serve] pinned allocation failed ({e}); pageable host memory")
+                    host_memory = "pageable"
+                    chunks.append(torch.empty(max(chunk, need), dtype=torch.int32))
+                used = 0
+            seg = chunks[-1][used:used + need].view(3, 2 * m)
+            seg.copy_(pool[:, offs[i]:offs[i + 1]])
+            used += need
+            sv = seg.numpy().view(np.uint32)
+            reqs.append((sv[0], sv[1], sv[2], n))
         torch.cuda.synchronize()
-        pv = hp.numpy().view(np.uint32)
-        reqs = [(pv[0, offs[i]:offs[i + 1]], pv[1, offs[i]:offs[i + 1]],
-                 pv[2, offs[i]:offs[i + 1]], n) for i, (n, _) in enumerate(specs)]
+        log(f"[serve] request arrays in {len(chunks)} {host_memory} chunks of 2 GiB")
     gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
     batched, rb = serve_stream(args, reqs, specs, gaps, len(reqs))
     log(f"[serve] {inputs} inputs, batched: {batched}")
