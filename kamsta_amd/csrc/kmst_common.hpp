@@ -228,11 +228,17 @@ __device__ __forceinline__ u64 lookback_exclusive(u64* status, u32 epoch, u32 ti
 
 // Where a record's strict key comes from: stored keys, or (packed-key mode on
 // the read-only input) computed from the weight array as
-// (w << 2b) | (min(s, d) << b) | max(s, d) with b = vbits.
+// (w << 2b) | (min(s, d) << b) | max(s, d) with b = vbits.  With twin_lo set
+// (positional mode on the read-only input, taken only by the kernels that
+// read the input directly: the class scatter and the outside gather) the key
+// is (w << 32) | the position of the (min, max) twin, found for a record
+// (s, d) with s > d by a search in d's record range [twin_lo[d], twin_hi[d]).
 struct KeySrc {
   const u64* v = nullptr;
   const u32* w = nullptr;
   u32 vbits = 0;
+  const u32* twin_lo = nullptr;
+  const u32* twin_hi = nullptr;
 };
 
 __device__ __forceinline__ u64 pack_key(u32 w, u32 s, u32 d, u32 vbits) {

@@ -588,16 +588,34 @@ void build_keys_packed(hipStream_t st, const u32* src, const u32* dst, const u32
   KMST_HIP(hipGetLastError());
 }
 
-// off[v] = first record of vertex v, off[n] = m (input already validated).
+// off[v] = first record of vertex v, off[n] = m, for a validated input (sorted
+// by src, so the writes total n + 1).  A thread takes 4 consecutive records
+// per step (one 16-byte load when aligned) and opens the vertices from the
+// previous record's source + 1 to its own.
 __global__ __launch_bounds__(BLOCK) void offsets_kernel(const u32* __restrict__ src, u64 m, u64 n,
-                                                        u32* __restrict__ off) {
-  const u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x;
-  if (i >= m) return;
-  const u64 s = src[i];
-  const u64 prev = (i == 0) ? 0 : u64(src[i - 1]) + 1;  // first vertex this record may open
-  for (u64 v = prev; v <= s && v <= n; ++v) off[v] = static_cast<u32>(i);
-  if (i == m - 1)
-    for (u64 v = s + 1; v <= n; ++v) off[v] = static_cast<u32>(m);
+                                                        u32* __restrict__ off, bool vec) {
+  for (u64 q = u64(blockIdx.x) * BLOCK + threadIdx.x; 4 * q < m; q += u64(gridDim.x) * BLOCK) {
+    const u64 i0 = 4 * q;
+    u32 r[4];
+    if (vec && i0 + 3 < m) {
+      const uint4 a = *reinterpret_cast<const uint4*>(src + i0);
+      r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) r[e] = src[min(i0 + e, m - 1)];
+    }
+    u64 first = i0 == 0 ? 0 : u64(src[i0 - 1]) + 1;  // first vertex record i0 may open
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const u64 i = i0 + e;
+      if (i >= m) break;
+      const u64 s = r[e];
+      for (u64 v = first; v <= s && v <= n; ++v) off[v] = static_cast<u32>(i);
+      if (i == m - 1)
+        for (u64 v = s + 1; v <= n; ++v) off[v] = static_cast<u32>(m);
+      first = s + 1;
+    }
+  }
 }
 
 void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off) {
@@ -606,124 +624,118 @@ void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off) {
     return;
   }
   KTScope t(st, K_OFFSETS, 4.0 * m + 4.0 * n);
-  offsets_kernel<<<ceil_div_u32(m, BLOCK), BLOCK, 0, st>>>(src, m, n, off);
+  const bool vec = (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+  offsets_kernel<<<grid_for((m + 3) / 4, 8192), BLOCK, 0, st>>>(src, m, n, off, vec);
   KMST_HIP(hipGetLastError());
 }
 
-// Positional strict key, used when the packed key does not fit 64 bits:
-// val = (w << 32) | uid, uid = position of the (min, max) twin.  The reverse
-// record (u, v) with u > v finds its twin by binary search in v's segment;
+// Every vertex's record range [seg_start[v], seg_end[v]) of records sorted by
+// src: the first record of a run of equal src writes its start and the end of
+// the run before it.  One write per run boundary, so an unchecked input costs
+// no more (ids >= n are skipped; on unsorted input the ranges are garbage,
+// which the readers clamp).
+__global__ __launch_bounds__(BLOCK) void record_ranges_kernel(const u32* __restrict__ src, u64 m,
+                                                              u64 n, u32* __restrict__ seg_start,
+                                                              u32* __restrict__ seg_end, bool vec) {
+  for (u64 q = u64(blockIdx.x) * BLOCK + threadIdx.x; 4 * q < m; q += u64(gridDim.x) * BLOCK) {
+    const u64 i0 = 4 * q;
+    u32 r[4];
+    if (vec && i0 + 3 < m) {
+      const uint4 a = *reinterpret_cast<const uint4*>(src + i0);
+      r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) r[e] = src[min(i0 + e, m - 1)];
+    }
+    u32 ps = i0 == 0 ? 0u : src[i0 - 1];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const u64 i = i0 + e;
+      if (i >= m) break;
+      const u32 s = r[e];
+      if (i == 0 || ps != s) {
+        if (s < n) seg_start[s] = static_cast<u32>(i);
+        if (i > 0 && ps < n) seg_end[ps] = static_cast<u32>(i);
+      }
+      if (i + 1 == m && s < n) seg_end[s] = static_cast<u32>(m);
+      ps = s;
+    }
+  }
+}
+
+void record_ranges(hipStream_t st, const u32* src, u64 m, u64 n, u32* seg_start, u32* seg_end) {
+  if (m == 0) return;
+  KTScope t(st, K_OFFSETS, 4.0 * m + 8.0 * n);
+  const bool vec = (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+  record_ranges_kernel<<<grid_for((m + 3) / 4, 8192), BLOCK, 0, st>>>(src, m, n, seg_start,
+                                                                       seg_end, vec);
+  KMST_HIP(hipGetLastError());
+}
+
+// Input position of the twin (d, s, w) of record i = (s, d, w), found by
+// binary search in d's record range [lo, hi) of an input sorted by (src, dst);
 // within runs of equal (src, dst) (multigraph inputs) the k-th record of
-// weight w pairs with the k-th twin of weight w.
-// With CHECK the same pass validates the input as validate_kernel does (the
-// twin search then runs on unchecked records, so its row bounds are clamped
-// to the arrays and records with ids out of range are skipped).
-template <bool CHECK>
+// weight w pairs with the k-th twin of weight w.  NO_TWIN when there is none.
+constexpr u64 NO_TWIN = ~u64{0};
+
+__device__ __forceinline__ u64 find_twin(const u32* __restrict__ src, const u32* __restrict__ dst,
+                                         const u32* __restrict__ w, u64 i, u32 s, u32 d, u32 wi,
+                                         u64 lo, u64 hi) {
+  u32 before = 0;
+  for (u64 j = i; j > 0; --j) {
+    if (src[j - 1] != s || dst[j - 1] != d) break;
+    before += (w[j - 1] == wi) ? 1u : 0u;
+  }
+  const u64 end = hi;
+  while (lo < hi) {
+    const u64 mid = (lo + hi) >> 1;
+    if (dst[mid] < s) lo = mid + 1;
+    else hi = mid;
+  }
+  for (u64 j = lo; j < end && dst[j] == s; ++j) {
+    if (w[j] == wi) {
+      if (before == 0) return j;
+      --before;
+    }
+  }
+  return NO_TWIN;
+}
+
+// Positional strict key, used when the packed key does not fit 64 bits:
+// val = (w << 32) | uid, uid = position of the (min, max) twin: the record's
+// own for s < d, found by find_twin in d's range [lo[d], hi[d]) for s > d.
+// The input is validated.  (Listing the s > d records of a wave first and
+// searching them one per lane measured slower here, 4.2 against 3.2 ms at
+// RGG2D 2^24: this pass has no stage to hide the list behind.)
 __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src,
                                                      const u32* __restrict__ dst,
                                                      const u32* __restrict__ w, u64 m,
-                                                     const u32* __restrict__ off,
-                                                     u64* __restrict__ val, u32* err,
-                                                     const InputCheck chk) {
-  __shared__ u64 s_red[WAVES][3];
-  u32 bad = 0, fwd = 0;
-  u64 hf = 0, hr = 0;
+                                                     const u32* __restrict__ row_lo,
+                                                     const u32* __restrict__ row_hi,
+                                                     u64* __restrict__ val, u32* err) {
+  u32 bad = 0;
   for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
     const u32 s = src[i], d = dst[i], wi = w[i];
-    if (CHECK) {
-      const bool has_prev = i > 0;
-      const u32 ps = has_prev ? src[i - 1] : 0u, pd = has_prev ? dst[i - 1] : 0u;
-      if (s >= chk.n || d >= chk.n || s == d) bad |= ERR_INVALID;
-      if (has_prev && (ps > s || (ps == s && pd > d))) bad |= ERR_UNSORTED;
-      if (chk.seg_end && (!has_prev || ps != s)) {
-        // record i opens s's range and closes the previous vertex's
-        if (s < chk.n) chk.seg_start[s] = static_cast<u32>(i);
-        if (has_prev && ps < chk.n) chk.seg_end[ps] = static_cast<u32>(i);
-      }
-      if (chk.seg_end && i + 1 == m && s < chk.n) chk.seg_end[s] = static_cast<u32>(m);
-      const bool f = s < d;
-      const u64 h = sym_hash(f ? s : d, f ? d : s, wi);
-      fwd += f ? 1u : 0u;
-      hf += f ? h : 0ull;
-      hr += f ? 0ull : h;
-      if (s >= chk.n || d >= chk.n) {
-        val[i] = VAL_NONE;
-        continue;
-      }
-    }
     if (s < d) {
       val[i] = (u64(wi) << 32) | i;
       continue;
     }
-    // within runs of equal (src, dst) (multigraph inputs) the k-th record of
-    // weight w pairs with the k-th twin of weight w
-    u32 before = 0;
-    for (u64 j = i; j > 0; --j) {
-      if (src[j - 1] != s || dst[j - 1] != d) break;
-      before += (w[j - 1] == wi) ? 1u : 0u;
-    }
-    u64 lo = off[d], hi = off[d + 1];
-    if (CHECK) {  // unchecked offsets: keep the row inside the arrays
-      lo = min(lo, m);
-      hi = min(max(hi, lo), m);
-    }
-    const u64 end = hi;
-    while (lo < hi) {
-      const u64 mid = (lo + hi) >> 1;
-      if (dst[mid] < s) lo = mid + 1;
-      else hi = mid;
-    }
-    bool found = false;
-    for (u64 j = lo; j < end && dst[j] == s; ++j) {
-      if (w[j] == wi) {
-        if (before == 0) {
-          val[i] = (u64(wi) << 32) | j;
-          found = true;
-          break;
-        }
-        --before;
-      }
-    }
-    if (!found) {
+    const u64 j = find_twin(src, dst, w, i, s, d, wi, row_lo[d], row_hi[d]);
+    if (j == NO_TWIN) {
       val[i] = VAL_NONE;
       bad |= ERR_ASYMMETRIC;
+    } else {
+      val[i] = (u64(wi) << 32) | j;
     }
   }
   if (bad) atomicOr(err, bad);
-  if (!CHECK) return;
-#pragma unroll
-  for (int o = WAVE / 2; o > 0; o >>= 1) {
-    fwd += __shfl_xor(fwd, o, WAVE);
-    hf += __shfl_xor(hf, o, WAVE);
-    hr += __shfl_xor(hr, o, WAVE);
-  }
-  if (lane_id() == 0) {
-    s_red[wave_id()][0] = fwd;
-    s_red[wave_id()][1] = hf;
-    s_red[wave_id()][2] = hr;
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    u64 t = 0;
-    for (int ww = 0; ww < WAVES; ++ww) t += s_red[ww][threadIdx.x];
-    if (threadIdx.x == 0) {
-      if (t) atomicAdd(chk.fwd_count, static_cast<u32>(t));
-    } else if (t) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(chk.hsum + threadIdx.x - 1),
-                static_cast<unsigned long long>(t));
-    }
-  }
 }
 
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
-                const u32* off, u64* val, u32* err, const InputCheck* check) {
+                const u32* row_lo, const u32* row_hi, u64* val, u32* err) {
   if (m == 0) return;
   KTScope t(st, K_KEYS, 20.0 * m);
-  const u32 grid = grid_for(m, 4096);
-  if (check)
-    keys_kernel<true><<<grid, BLOCK, 0, st>>>(src, dst, w, m, off, val, err, *check);
-  else
-    keys_kernel<false><<<grid, BLOCK, 0, st>>>(src, dst, w, m, off, val, err, InputCheck{});
+  keys_kernel<<<grid_for(m, 4096), BLOCK, 0, st>>>(src, dst, w, m, row_lo, row_hi, val, err);
   KMST_HIP(hipGetLastError());
 }
 
@@ -1361,6 +1373,7 @@ void relabel_compact(hipStream_t st, const u32* s_in, const u32* d_in, const Key
                      u64 key_min, u64 m, const u32* label, u32* s_out, u32* d_out, u64* v_out,
                      LookbackWork& lb, u32* count_dev, u32* err, const u64* giant, u64 n,
                      u64 key_max, bool twins) {
+  if (keys.twin_lo) throw Error(KMST_EINVAL, "relabel needs stored or packed keys");
   if (m == 0) {
     fill_u32(st, count_dev, 1, 0);
     return;
@@ -2612,10 +2625,13 @@ void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u
 // is not walked) the reverse record is emitted as well.  A counting
 // pass sizes the block's output, one global atomic places it, and an emitting
 // pass (whose reads hit the cache) writes it.  Output order is arbitrary;
-// positions at or past cap raise ERR_BOUNDS.
+// positions at or past cap raise ERR_BOUNDS.  With lazy positional keys
+// (keys.twin_lo) [kmin, kmax] are weight bounds shifted by 32, and only the
+// emitted records' keys are made (their twin search in the emitting pass).
 __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
     const u32* __restrict__ list, const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
-    u32 n_list, const u32* __restrict__ in_d, const KeySrc keys, u64 kmin, u64 kmax,
+    u32 n_list, const u32* __restrict__ in_s, const u32* __restrict__ in_d, const KeySrc keys,
+    u64 kmin, u64 kmax,
     const u32* __restrict__ gpar, u32 root, const u64* __restrict__ inside,
     u32* __restrict__ s_out, u32* __restrict__ d_out, u64* __restrict__ v_out, u64 cap,
     u32* count, u32* err) {
@@ -2645,7 +2661,7 @@ __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
   __syncthreads();
   const int lane = lane_id();
   // record t of the block's range: vertex j = last with s_start[j] <= t
-  auto visit = [&](u64 t, bool& keep, bool& both, u32& gu, u32& gd, u64& key) {
+  auto visit = [&](u64 t, bool need_key, bool& keep, bool& both, u32& gu, u32& gd, u64& key) {
     keep = both = false;
     if (t >= total) return;
     u32 lo = 0, hi = nv;  // s_start[lo] <= t < s_start[hi]
@@ -2658,18 +2674,33 @@ __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
     const u32 u = s_u[lo];
     gu = s_gu[lo];
     const u32 d = in_d[e];
-    key = load_key(keys, e, u, d);
-    if (key < kmin || key > kmax) return;
+    u32 w = 0;
+    if (keys.twin_lo) {
+      w = keys.w[e];
+      if (w < (kmin >> 32) || w > (kmax >> 32)) return;
+    } else {
+      key = load_key(keys, e, u, d);
+      if (key < kmin || key > kmax) return;
+    }
     gd = gpar[d];
     keep = gd != gu;
     both = keep && (inside ? ((inside[d >> 6] >> (d & 63)) & 1ull) != 0 : gd == root);
+    if (keys.twin_lo && keep && need_key) {
+      u64 j = e;
+      if (u > d) j = find_twin(in_s, in_d, keys.w, e, u, d, w, keys.twin_lo[d], keys.twin_hi[d]);
+      if (j == NO_TWIN) {  // the input check saw every twin: not reached
+        atomicOr(err, ERR_ASYMMETRIC);
+        keep = both = false;
+      }
+      key = (u64(w) << 32) | j;
+    }
   };
   u32 mine = 0;
   for (u64 t0 = 0; t0 < total; t0 += BLOCK) {
     bool keep, both;
     u32 gu, gd;
     u64 key;
-    visit(t0 + threadIdx.x, keep, both, gu, gd, key);
+    visit(t0 + threadIdx.x, false, keep, both, gu, gd, key);
     mine += (keep ? 1u : 0u) + (both ? 1u : 0u);
   }
 #pragma unroll
@@ -2686,7 +2717,7 @@ __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
     bool keep, both;
     u32 gu = 0, gd = 0;
     u64 key = 0;
-    visit(t0 + threadIdx.x, keep, both, gu, gd, key);
+    visit(t0 + threadIdx.x, true, keep, both, gu, gd, key);
     const u64 bk = __ballot(keep), bb = __ballot(both);
     const u32 nw = static_cast<u32>(__popcll(bk) + __popcll(bb));
     if (nw == 0) continue;  // uniform per wave
@@ -2717,7 +2748,8 @@ __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
 }
 
 void outside_gather(hipStream_t st, const u32* list, const u32* seg_lo, const u32* seg_hi,
-                    u32 n_list, u64 total, const u32* in_d, const KeySrc& keys, u64 kmin, u64 kmax,
+                    u32 n_list, u64 total, const u32* in_s, const u32* in_d, const KeySrc& keys,
+                    u64 kmin, u64 kmax,
                     const u32* gpar, u32 root, const u64* inside, u32* s_out, u32* d_out,
                     u64* v_out, u64 cap, u32* count, u32* err) {
   fill_u32(st, count, 1, 0);
@@ -2726,9 +2758,8 @@ void outside_gather(hipStream_t st, const u32* list, const u32* seg_lo, const u3
   // the caller
   KTScope t(st, K_RELABEL_SCATTER, (keys.v ? 16.0 : 8.0) * double(total) + 12.0 * n_list);
   outside_gather_kernel<<<ceil_div_u32(n_list, BLOCK), BLOCK, 0, st>>>(
-      list, seg_lo, seg_hi, n_list, in_d, keys, kmin, kmax, gpar, root, inside, s_out, d_out, v_out,
-      cap,
-      count, err);
+      list, seg_lo, seg_hi, n_list, in_s, in_d, keys, kmin, kmax, gpar, root, inside, s_out, d_out,
+      v_out, cap, count, err);
   KMST_HIP(hipGetLastError());
 }
 
@@ -2821,7 +2852,10 @@ void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits)
 // weights pass).  One wave per 256-record unit, no waits between units.  With
 // CHECK the same pass validates the input as validate_kernel does (ids, order,
 // symmetry hashes, forward count, every vertex's record range), so the input
-// is read once for both.
+// is read once for both.  With LAZY (positional keys) the keys of the written
+// records are made here: (w << 32) | the record's own position for s < d, the
+// twin's (find_twin in the ranges keys.twin_lo / twin_hi) for s > d, so that
+// no key array of the whole input is built.
 // A block takes a chunk of consecutive units per step (two per wave, one
 // with the input check, which needs the registers).  The chunk's class 0 and
 // class 1 records are contiguous in the output, so they are staged in LDS and
@@ -2833,7 +2867,7 @@ struct ScatterChunk {
   static constexpr int RECORDS = UNITS * int(UNIT_RECORDS);
 };
 
-template <bool VEC, bool CHECK>
+template <bool VEC, bool CHECK, bool LAZY>
 __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
     const u32* __restrict__ s_in, const u32* __restrict__ d_in, const KeySrc keys, u64 m,
     u32 wshift, u32 q0, u32 q1, const u32* __restrict__ off0, const u32* __restrict__ off1,
@@ -2846,6 +2880,11 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
   __shared__ u32 st_d[CS_CHUNK_RECORDS];
   __shared__ u64 st_v[CS_CHUNK_RECORDS];
   __shared__ u64 s_red[WAVES][3];
+  // lazy keys: every wave's twin searches of the chunk, (stage position << 16)
+  // | record index in the chunk
+  constexpr int TASKS = LAZY ? CS_WAVE_UNITS * int(UNIT_RECORDS) : 1;
+  __shared__ u32 s_task[WAVES][TASKS];
+  static_assert(!LAZY || CS_CHUNK_RECORDS <= (1 << 16), "task encoding");
   const int lane = lane_id(), wv = wave_id();
   const u64 units = record_units_dev(m);
   const u64 chunks = (units + CS_CHUNK - 1) / CS_CHUNK;
@@ -2910,12 +2949,13 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
     else if (chunk + gridDim.x < chunks) load_chunk(chunk + gridDim.x, ns, nd, nw, nv);
     // the last record of the wave's first unit (lane 63's), for the check
     const u32 last_s = __shfl(ss[0][3], WAVE - 1, WAVE), last_d = __shfl(dd[0][3], WAVE - 1, WAVE);
+    u32 ntask = 0;  // wave-uniform
 #pragma unroll
     for (int u = 0; u < CS_WAVE_UNITS; ++u) {
       const u64 unit = first + wv * CS_WAVE_UNITS + u;  // wave-uniform
       if (unit >= units) break;
       const u64 r0 = unit * UNIT_RECORDS + 4 * u64(lane);
-      if (!keys.v) {
+      if (!keys.v && !LAZY) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) vv[u][e] = pack_key(ww[u][e], ss[u][e], dd[u][e], keys.vbits);
       }
@@ -2943,14 +2983,14 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
           }
           if (s >= chk.n || d >= chk.n || s == d) bad |= ERR_INVALID;
           if (has_prev && (ps > s || (ps == s && pd > d))) bad |= ERR_UNSORTED;
-          if (!has_prev || ps != s) {
+          if (chk.seg_end && (!has_prev || ps != s)) {
             // record i opens s's range and closes the previous vertex's
             if (s < chk.n) chk.seg_start[s] = static_cast<u32>(i);
             if (has_prev && ps < chk.n) chk.seg_end[ps] = static_cast<u32>(i);
           }
-          if (i + 1 == m && s < chk.n) chk.seg_end[s] = static_cast<u32>(m);
+          if (chk.seg_end && i + 1 == m && s < chk.n) chk.seg_end[s] = static_cast<u32>(m);
           const bool f = s < d;
-          // CHECK runs with packed keys only: ww holds the weights
+          // CHECK runs with packed or lazy keys only: ww holds the weights
           const u64 h = sym_hash(f ? s : d, f ? d : s, ww[u][e]);
           fwd += f ? 1u : 0u;
           hf += f ? h : 0ull;
@@ -2966,6 +3006,20 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
         b0[e] = __ballot(cls[e] == 0);
         b1[e] = __ballot(cls[e] == 1);
       }
+      // lazy keys: s < d records key their own position; s > d records get a
+      // twin search task (ids checked), run after the chunk is staged
+      bool need[4] = {false, false, false, false};
+      u64 bt[4] = {0, 0, 0, 0};
+      if (LAZY) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const u32 s = ss[u][e], d = dd[u][e];
+          need[e] = cls[e] < 2 && s > d && s < chk.n && d < chk.n;
+          vv[u][e] = (u64(ww[u][e]) << 32) | (s < d ? r0 + e : u64{0xFFFFFFFFu});
+          bt[e] = __ballot(need[e]);
+        }
+      }
+      u32 tslot = ntask;
       // stage positions: lane l's records follow those of lanes < l
       u64 p0 = off0[unit] - a0 + mbcnt(b0[0]) + mbcnt(b0[1]) + mbcnt(b0[2]) + mbcnt(b0[3]);
       u64 p1 = n0 + off1[unit] - a1 + mbcnt(b1[0]) + mbcnt(b1[1]) + mbcnt(b1[2]) + mbcnt(b1[3]);
@@ -2977,13 +3031,41 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
             st_s[p] = ss[u][e];
             st_d[p] = dd[u][e];
             st_v[p] = vv[u][e];
+            if (LAZY && need[e])
+              s_task[wv][tslot + mbcnt(bt[e])] =
+                  (u32(p) << 16) | static_cast<u32>(r0 + e - first * UNIT_RECORDS);
           } else {
             atomicOr(err, ERR_BOUNDS);
+            if (LAZY && need[e]) s_task[wv][tslot + mbcnt(bt[e])] = NONE32;  // no search
           }
         }
+        if (LAZY) tslot += static_cast<u32>(__popcll(bt[e]));
       }
+      ntask = tslot;
     }
     __syncthreads();
+    if (LAZY) {
+      // the wave's twin searches, one per lane
+      for (u32 t = lane; t < ntask; t += WAVE) {
+        const u32 task = s_task[wv][t];
+        if (task == NONE32) continue;
+        const u32 p = task >> 16;
+        const u64 i = first * UNIT_RECORDS + (task & 0xFFFFu);
+        const u32 s = st_s[p], d = st_d[p];
+        const u32 wi = static_cast<u32>(st_v[p] >> 32);
+        // unchecked ranges: keep the search inside the arrays
+        const u64 lo = min<u64>(keys.twin_lo[d], m);
+        const u64 hi = min<u64>(max<u64>(keys.twin_hi[d], lo), m);
+        const u64 j = find_twin(s_in, d_in, keys.w, i, s, d, wi, lo, hi);
+        if (j == NO_TWIN) {
+          bad |= ERR_ASYMMETRIC;
+          st_v[p] = VAL_NONE;
+        } else {
+          st_v[p] = (u64(wi) << 32) | j;
+        }
+      }
+      __syncthreads();
+    }
     // coalesced runs out of the stage
     const u64 nst = min<u64>(n0 + n1, CS_CHUNK_RECORDS);
     for (u32 j = threadIdx.x; j < nst; j += BLOCK) {
@@ -3039,7 +3121,9 @@ void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySr
                    u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err,
                    const InputCheck* check) {
   if (m == 0) return;
-  if (check && keys.v) throw Error(KMST_EINVAL, "input check needs packed keys");
+  if (check && keys.v) throw Error(KMST_EINVAL, "input check needs packed or lazy keys");
+  const bool lazy = keys.twin_lo != nullptr;
+  if (lazy && !check) throw Error(KMST_EINVAL, "lazy positional keys come with the input check");
   KTScope t(st, K_SPLIT_SCATTER, (keys.v ? 16.0 : 12.0) * m + 16.0 * double(written));
   const u64 chunk = check ? ScatterChunk<true>::UNITS : ScatterChunk<false>::UNITS;
   const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + chunk - 1) / chunk, 8192));
@@ -3048,14 +3132,16 @@ void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySr
                                                         : static_cast<const void*>(keys.w))) &
                     15) == 0;
   const InputCheck chk = check ? *check : InputCheck{};
-#define KMST_CLASS_SCATTER(V, C)                                                                \
-  class_scatter_kernel<V, C><<<grid, BLOCK, 0, st>>>(s_in, d_in, keys, m, wshift, q0, q1, off0, \
-                                                     off1, base1, written, s_out, d_out, v_out, \
-                                                     err, chk)
-  if (vec && check) KMST_CLASS_SCATTER(true, true);
-  else if (vec) KMST_CLASS_SCATTER(true, false);
-  else if (check) KMST_CLASS_SCATTER(false, true);
-  else KMST_CLASS_SCATTER(false, false);
+#define KMST_CLASS_SCATTER(V, C, L)                                                              \
+  class_scatter_kernel<V, C, L><<<grid, BLOCK, 0, st>>>(s_in, d_in, keys, m, wshift, q0, q1,     \
+                                                        off0, off1, base1, written, s_out, d_out, \
+                                                        v_out, err, chk)
+  if (vec && lazy) KMST_CLASS_SCATTER(true, true, true);
+  else if (lazy) KMST_CLASS_SCATTER(false, true, true);
+  else if (vec && check) KMST_CLASS_SCATTER(true, true, false);
+  else if (vec) KMST_CLASS_SCATTER(true, false, false);
+  else if (check) KMST_CLASS_SCATTER(false, true, false);
+  else KMST_CLASS_SCATTER(false, false, false);
 #undef KMST_CLASS_SCATTER
   KMST_HIP(hipGetLastError());
 }

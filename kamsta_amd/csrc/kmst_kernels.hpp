@@ -69,11 +69,13 @@ bool validate_records(hipStream_t st, const u32* src, const u32* dst, const u32*
                       u32* seg_end = nullptr, u32 q0 = 0, u32 q1 = 0, u32* unit_c0 = nullptr,
                       u32* unit_c1 = nullptr);
 void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off);
-struct InputCheck;
-// Positional keys (twin search in the CSR rows given by off); with check the
-// same pass also validates the (then unchecked) input.
+// Record ranges [seg_start[v], seg_end[v]) of records sorted by src, written
+// for the vertices with records only; bounded work on an unchecked input.
+void record_ranges(hipStream_t st, const u32* src, u64 m, u64 n, u32* seg_start, u32* seg_end);
+// Positional keys of a validated input (twin search in the record ranges
+// [row_lo[v], row_hi[v]): the offsets off / off + 1, or seg_start / seg_end).
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
-                const u32* off, u64* val, u32* err, const InputCheck* check = nullptr);
+                const u32* row_lo, const u32* row_hi, u64* val, u32* err);
 void build_keys_packed(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                        u32 vbits, u64* val);
 
@@ -150,14 +152,16 @@ void partition_giant_bits(hipStream_t st, const u32* gpar, const u32* seg_end, u
 // each); counters[0] = count, counters[2..3] = u64 total range length.
 // outside_gather then emits, for keys in [kmin, kmax], the relabelled
 // cross-component records in both directions (see the kernel), at most
-// 2 * total of them.
+// 2 * total of them.  With lazy positional keys (keys.twin_lo / twin_hi the
+// validated record ranges) [kmin, kmax] bound the weight << 32 and the keys
+// of the emitted records are made by their twin search.
 void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u32* seg_end,
                   u64 n, u32 root, const u64* inside, u32* list, u32* seg_lo, u32* seg_hi,
                   u32* counters);
 void outside_gather(hipStream_t st, const u32* list, const u32* seg_lo, const u32* seg_hi,
-                    u32 n_list, u64 total, const u32* in_d, const KeySrc& keys, u64 kmin, u64 kmax,
-                    const u32* gpar, u32 root, const u64* inside, u32* s_out, u32* d_out,
-                    u64* v_out, u64 cap, u32* count, u32* err);
+                    u32 n_list, u64 total, const u32* in_s, const u32* in_d, const KeySrc& keys,
+                    u64 kmin, u64 kmax, const u32* gpar, u32 root, const u64* inside, u32* s_out,
+                    u32* d_out, u64* v_out, u64 cap, u32* count, u32* err);
 
 // K6 stable LSD radix sort by key (s << dbits) | d over total_bits bits; with
 // dbits == 0 the key is s alone and d is carried along.  Returns true when the
@@ -293,8 +297,11 @@ struct InputCheck {
 // Weight classes 0 (w <= q0) and 1 (q0 < w <= q1) of the input, in input
 // order, to out[0, c0) and out[base1, base1 + c1); off0 / off1 are the
 // exclusive scans of the unit counts.  written = c0 + c1; positions at or past
-// it raise ERR_BOUNDS and are not written.  With check (packed keys only) the
-// pass also validates the input.
+// it raise ERR_BOUNDS and are not written.  With check (packed keys, or lazy
+// positional keys: keys.twin_lo / twin_hi the record ranges of the unchecked
+// input, from record_ranges) the pass also validates the input, and records
+// the ranges unless check->seg_end is null.  Lazy keys are made for the
+// written records only and need the check.
 void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySrc& keys, u64 m,
                    u32 wshift, u32 q0, u32 q1, const u32* off0, const u32* off1, u64 base1,
                    u64 written, u32* s_out, u32* d_out, u64* v_out, u32* err,

@@ -273,7 +273,23 @@ class Solver {
   double giant_share = 0.0;
   u32 pick_pivot(RecView r);
   u64 partition(RecView in, RecView out, u32 pivot, bool write_heavy = true);
-  KeySrc keys_of(const RecView& r) const { return KeySrc{r.v, r.w, vbits}; }
+  // keys of r: stored, packed from the weights, or (positional mode, r the
+  // input without stored keys) made by the kernels that read the input, with
+  // the twin search in the validated record ranges
+  KeySrc keys_of(const RecView& r) const {
+    KeySrc k{r.v, r.w, vbits};
+    if (!r.v && !packed) {
+      k.twin_lo = seg_start;
+      k.twin_hi = seg_end;
+    }
+    return k;
+  }
+  // positional keys of the whole input (val0), built once when a relabel
+  // reads the input
+  u64* input_keys();
+  // positional keys not stored for the input: the class scatter and the
+  // outside gather make them for the records they write
+  bool lazy_keys = false;
   // records grouped by src are all the argmin needs; where parallel records
   // abound, sort by (src, dst) and keep only the lightest of each pair
   RecView sort_dedup(RecView cur, RecView alt, u32 bits);
@@ -364,18 +380,13 @@ void Solver::setup(bool filter, bool plan_classes) {
       check_in_scatter = true;
       return;
     }
-    // positional keys: the key pass validates the input too (its twin search
-    // is bounded for unchecked records)
-    InputCheck chk;
-    chk.n = n0;
-    chk.fwd_count = ctr + C_FWD;
-    chk.hsum = hsum;
-    chk.seg_start = seg_start;
-    chk.seg_end = seg_end;
-    val0 = static_cast<u64*>(C.val0.get(m0 * 8));
-    build_offsets(st, in_s, m0, n0, label /* reused as offsets, n0+1 */);
-    build_keys(st, in_s, in_d, in_w, m0, label, val0, ctr + C_ERR, &chk);
-    check_input();
+    // positional keys: the class scatter validates the input and makes the
+    // keys of the records it writes, by a twin search in the record ranges
+    // (clamped to the arrays while the input is unchecked), recorded first
+    record_ranges(st, in_s, m0, n0, seg_start, seg_end);
+    check_in_scatter = true;
+    lazy_keys = true;
+    C.stats.lazy_keys = true;
     return;
   } else {
     validate_records(st, in_s, in_d, in_w, m0, n0, ctr + C_ERR, ctr + C_FWD, ctr + C_MAXW, hsum,
@@ -386,7 +397,7 @@ void Solver::setup(bool filter, bool plan_classes) {
   if (!packed) {
     val0 = static_cast<u64*>(C.val0.get(m0 * 8));
     build_offsets(st, in_s, m0, n0, label /* reused as offsets, n0+1 */);
-    build_keys(st, in_s, in_d, in_w, m0, label, val0, ctr + C_ERR);
+    build_keys(st, in_s, in_d, in_w, m0, label, label + 1, val0, ctr + C_ERR);
     check_errors("input");
   }
 }
@@ -694,7 +705,7 @@ void Solver::filter_classes() {
   const u64 c1 = J >= 2 ? cnt[1] : 0, written = cnt[0] + c1;
   const u32 wshift = packed ? 2 * vbits : 32;
   const RecView input{const_cast<u32*>(in_s), const_cast<u32*>(in_d), packed ? nullptr : val0, m0,
-                      packed ? in_w : nullptr};
+                      packed || lazy_keys ? in_w : nullptr};
   // validation left the per-unit class counts in X.s / X.d: their exclusive
   // scans place every unit's records
   const u64 units = record_units(m0);
@@ -704,8 +715,8 @@ void Solver::filter_classes() {
   chk.n = n0;
   chk.fwd_count = ctr + C_FWD;
   chk.hsum = reinterpret_cast<u64*>(ctr + C_HSUM64);
-  chk.seg_start = seg_start;
-  chk.seg_end = seg_end;
+  chk.seg_start = lazy_keys ? nullptr : seg_start;  // lazy keys: recorded already
+  chk.seg_end = lazy_keys ? nullptr : seg_end;
   class_scatter(st, in_s, in_d, keys_of(input), m0, wshift, q0, q1, X.s, X.d, cnt[0], written,
                 Y.s, Y.d, Y.v, ctr + C_ERR, check_in_scatter ? &chk : nullptr);
   if (check_in_scatter)
@@ -740,6 +751,14 @@ void Solver::filter_classes() {
     const u64 kmax = c < K ? key_upper(pivots.q[c]) : ~0ull;
     filter_heavy(input, X, Y, static_cast<int>(K - c), kmin, kmax, /*from_input=*/true);
   }
+}
+
+u64* Solver::input_keys() {
+  if (!val0) {
+    val0 = static_cast<u64*>(C.val0.get(m0 * 8));
+    build_keys(st, in_s, in_d, in_w, m0, seg_start, seg_end, val0, ctr + C_ERR);
+  }
+  return val0;
 }
 
 u64 Solver::partition(RecView in, RecView out, u32 pivot, bool write_heavy) {
@@ -809,7 +828,7 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
       return;
     }
     if (3 * total <= m0 && 2 * total <= scratch.m) {
-      outside_gather(st, best_dst, parent, label, static_cast<u32>(n_list), total, in_d,
+      outside_gather(st, best_dst, parent, label, static_cast<u32>(n_list), total, in_s, in_d,
                      keys_of(heavy), key_min, key_max, gpar, giant_root, inside, scratch.s,
                      scratch.d, scratch.v, scratch.m, ctr + C_MNEXT, ctr + C_ERR);
       gathered = true;
@@ -825,6 +844,7 @@ void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth
       component_bits(st, gpar, n0, giant_root, best_val);
       giant = best_val;
     }
+    if (from_input && !heavy.v && !packed) heavy.v = input_keys();
     // the heavy records are symmetric and sorted right after: relabel the
     // src < dst half and write both directions (half the gathers)
     relabel_compact(st, heavy.s, heavy.d, keys_of(heavy), key_min, heavy.m, gpar, scratch.s,
@@ -1371,6 +1391,7 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
   const kmst::SolveStats& s = ctx->stats;
   o << "{\"n\":" << s.n << ",\"m\":" << s.m << ",\"algorithm\":" << s.algorithm
     << ",\"packed_keys\":" << (s.packed_keys ? "true" : "false")
+    << ",\"lazy_keys\":" << (s.lazy_keys ? "true" : "false")
     << ",\"mst_edges\":" << s.mst_edges << ",\"total_weight\":" << s.total_weight
     << ",\"mst_long_groups\":" << s.mst_long_groups
     << ",\"mst_resorted\":" << (s.mst_resorted ? "true" : "false") << ",\"rounds\":[";

@@ -52,6 +52,7 @@ struct SolveStats {
   u64 n = 0, m = 0, mst_edges = 0, total_weight = 0;
   int algorithm = 0;
   bool packed_keys = true;
+  bool lazy_keys = false;  // positional keys made by the class scatter, not stored
   u64 max_weight = 0;
   // output order: groups of one smaller endpoint (or position range) longer
   // than MST_GROUP_SMALL keys, and whether one exceeded MST_GROUP_MAX (the

@@ -144,6 +144,38 @@ def test_multigraph(gpu):
         check_equal(graphs.records(und), threshold=1)
 
 
+def test_positional_multigraph(gpu):
+    """Parallel copies under positional keys (31-bit weights, n > 2^16), many
+    of equal weight: the k-th copy of weight w pairs with the k-th reverse copy
+    of weight w in the key pass (plain Boruvka) and in the keys that
+    filter-Boruvka's class scatter and outside gather make for the records
+    they write."""
+    rng = np.random.default_rng(41)
+    n, m = 70000, 300000
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    keep = u != v
+    u, v = u[keep], v[keep]
+    w = (1 << 30) + rng.integers(0, 1 << 12, len(u))
+    dup = rng.random(len(u)) < 0.3
+    copies = rng.integers(1, 4, int(dup.sum()))
+    du = np.repeat(u[dup], copies)
+    dv = np.repeat(v[dup], copies)
+    same = rng.random(len(du)) < 0.5
+    dw = np.where(same, np.repeat(w[dup], copies), (1 << 30) + rng.integers(0, 1 << 12, len(du)))
+    uu, vv, ww = np.concatenate([u, du]), np.concatenate([v, dv]), np.concatenate([w, dw])
+    src = np.concatenate([uu, vv])
+    dst = np.concatenate([vv, uu])
+    wt = np.concatenate([ww, ww])
+    order = np.lexsort((wt, dst, src))
+    rec = np.zeros(len(src), dtype=graphs.WEDGE_DTYPE)
+    rec["src"], rec["dst"], rec["weight"] = src[order], dst[order], wt[order]
+    runs = (rec["src"][1:] == rec["src"][:-1]) & (rec["dst"][1:] == rec["dst"][:-1])
+    assert runs.sum() > 50000  # many parallel records
+    for thr in (2, 4):
+        check_equal(rec, threshold=thr)
+
+
 @pytest.mark.parametrize("n,m,mw,seed", [(1 << 14, 1 << 18, 254, 0x4B414D5354410001),
                                          (1 << 16, 1 << 20, 254, 5), (100000, 150000, 254, 6),
                                          (1 << 12, 1 << 20, 254, 7), (70000, 900000, (1 << 31) - 1, 8)])
@@ -276,8 +308,9 @@ def test_rhg_hubs(gpu, n):
                                     (70000, 400000, (1 << 31) - 1)])   # positional keys
 def test_contract_errors_with_weight_classes(gpu, n, m, mw):
     """Inputs large enough for filter-Boruvka's weight classes: the input
-    checks then run inside the class scatter (packed keys) or the key pass
-    (positional keys) instead of validate_kernel."""
+    checks then run inside the class scatter (packed keys and positional keys,
+    whose twin search there runs on unchecked records) instead of
+    validate_kernel."""
     s, d, w = k.gen_gnm_host(n, m, mw, 13)
     good = k.to_wedges(s, d, w)
     exp, exp_w = oracle.kruskal(good)
