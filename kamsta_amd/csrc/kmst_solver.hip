@@ -642,11 +642,13 @@ void Solver::base_case(RecView P, RecView Q) {
 }
 
 u32 Solver::pick_pivot(RecView r) {
-  const u32 k = static_cast<u32>(std::min<u64>(1023, r.m));
+  const u32 k = static_cast<u32>(std::min<u64>(LEVEL_SAMPLES, r.m));
   sample_weights(st, r.v, r.w, r.m, k, packed ? 2 * vbits : 32, samples);
-  std::vector<u32> h(k);
-  KMST_HIP(hipMemcpyAsync(h.data(), samples, k * 4, hipMemcpyDeviceToHost, st));
+  // through the pinned staging words
+  KMST_HIP(hipMemcpyAsync(C.host_counters + HOST_SAMPLE_WORD, samples, k * 4,
+                          hipMemcpyDeviceToHost, st));
   KMST_HIP(hipStreamSynchronize(st));
+  std::vector<u32> h(C.host_counters + HOST_SAMPLE_WORD, C.host_counters + HOST_SAMPLE_WORD + k);
   h.erase(std::remove(h.begin(), h.end(), NONE32), h.end());
   if (h.empty()) return NONE32;
   std::sort(h.begin(), h.end());
@@ -661,13 +663,15 @@ u32 Solver::pick_pivot(RecView r) {
 // the filter threshold, it has at most 1000 records, or its share of the part
 // exceeds 0.85 (hybrid_boruvka_edgefilter.hpp:233-240, filter_helpers.hpp:130-132).
 void Solver::plan_pivots() {
-  const u32 k = static_cast<u32>(std::min<u64>(4095, m0));
+  const u32 k = static_cast<u32>(std::min<u64>(PIVOT_SAMPLES, m0));
   sample_weights(st, nullptr, in_w, m0, k, 32, samples);
   // the same records' id distances decide where relabels gather far apart
   sample_far(st, in_s, in_d, m0, k, FAR_SPAN, samples + k);
-  std::vector<u32> h(k + 1);
-  KMST_HIP(hipMemcpyAsync(h.data(), samples, (k + 1) * 4, hipMemcpyDeviceToHost, st));
+  KMST_HIP(hipMemcpyAsync(C.host_counters + HOST_SAMPLE_WORD, samples, (k + 1) * 4,
+                          hipMemcpyDeviceToHost, st));
   KMST_HIP(hipStreamSynchronize(st));
+  std::vector<u32> h(C.host_counters + HOST_SAMPLE_WORD,
+                     C.host_counters + HOST_SAMPLE_WORD + k + 1);
   far_input = 2 * u64(h[k]) > k;
   h.pop_back();
   std::sort(h.begin(), h.end());

@@ -88,9 +88,13 @@ kmst_status guarded_call(F&& f) {
 // pinned host words: counters in [0, 64), then HOST_STAGE_WORDS of staging
 // (vertex samples, or the persistent rounds' control words and log)
 constexpr u32 GPAR_SAMPLES = 256;
+constexpr u32 PIVOT_SAMPLES = 4095;  // input weights sampled for the top-level pivots
+constexpr u32 LEVEL_SAMPLES = 1023;  // weights sampled for a deeper level's pivot
 constexpr u32 HOST_SAMPLE_WORD = 64;
-constexpr u32 HOST_STAGE_WORDS = 1024;
-static_assert(GPAR_SAMPLES <= HOST_STAGE_WORDS, "sample staging");
+constexpr u32 HOST_STAGE_WORDS = 4096;
+static_assert(GPAR_SAMPLES <= HOST_STAGE_WORDS && LEVEL_SAMPLES <= HOST_STAGE_WORDS &&
+                  PIVOT_SAMPLES + 1 <= HOST_STAGE_WORDS,
+              "sample staging");
 static_assert(PERSIST_CTL_WORDS + 2 * PERSIST_LOG_TOTAL <= HOST_STAGE_WORDS,
               "persistent-round staging");
 constexpr size_t HOST_COUNTER_BYTES = (HOST_SAMPLE_WORD + HOST_STAGE_WORDS) * 4;
