@@ -557,7 +557,10 @@ void weight_classes(hipStream_t st, const u32* w, u64 m, const WeightPivots& P, 
   if (m == 0) return;
   KTScope t(st, K_VALIDATE, 4.0 * m + 8.0 * record_units(m));
   const u64 waves = (record_units(m) + WC_UNITS - 1) / WC_UNITS;
-  const u32 grid = static_cast<u32>(std::min<u64>((waves + WAVES - 1) / WAVES, 8192));
+  // 2048 workgroups (8 per CU) fill the chip; more only add block-final
+  // reductions and atomics on the few class counters (at RGG2D 2^24:
+  // 0.47 ms with 8192, 0.28 with 2048, 0.31 with 1024)
+  const u32 grid = static_cast<u32>(std::min<u64>((waves + WAVES - 1) / WAVES, 2048));
   if ((reinterpret_cast<uintptr_t>(w) & 15) == 0)
     weight_classes_kernel<true><<<grid, BLOCK, 0, st>>>(w, m, P, q0, q1, class_counts, max_w,
                                                         unit_c0, unit_c1);
@@ -3126,7 +3129,10 @@ void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySr
   if (lazy && !check) throw Error(KMST_EINVAL, "lazy positional keys come with the input check");
   KTScope t(st, K_SPLIT_SCATTER, (keys.v ? 16.0 : 12.0) * m + 16.0 * double(written));
   const u64 chunk = check ? ScatterChunk<true>::UNITS : ScatterChunk<false>::UNITS;
-  const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + chunk - 1) / chunk, 8192));
+  // more workgroups than fit at once: the dispatcher then balances the
+  // chunks, whose time varies (GNM 2^24 / 2^28: 2.54 ms with 2048, 2.20 with
+  // 4096, 2.13 with 8192, 2.08 with 16384)
+  const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + chunk - 1) / chunk, 16384));
   const bool vec = ((reinterpret_cast<uintptr_t>(s_in) | reinterpret_cast<uintptr_t>(d_in) |
                      reinterpret_cast<uintptr_t>(keys.v ? static_cast<const void*>(keys.v)
                                                         : static_cast<const void*>(keys.w))) &

@@ -52,6 +52,35 @@ def test_batched_equals_alone_and_oracle(gpu, algo):
         assert resb["done_ns"] >= resb["start_ns"] >= resb["admit_ns"] >= resb["submit_ns"]
 
 
+def test_batched_wide_weights(gpu):
+    """31-bit weights on a batch of more than 2^17 vertex ids: positional
+    keys, made by the class scatter and the outside gather of a partitioned
+    solve."""
+    rng = np.random.default_rng(17)
+    reqs = []
+    for i in range(40):
+        n = int(2 ** rng.uniform(11, 13.5))
+        m = int(n * rng.uniform(4, 16) / 2)
+        s, d, w = k.gen_gnm_host(n, m, (1 << 31) - 1, 7000 + i)
+        reqs.append((s, d, w, n))
+    assert sum(r[3] for r in reqs) > (1 << 17)
+    batched = k.Server(algorithm=k.FILTER_BORUVKA, max_batch_requests=4096)
+    alone = k.Server(algorithm=k.FILTER_BORUVKA, max_batch_requests=1)
+    rb = solve_all(batched, reqs)
+    ra = solve_all(alone, reqs)
+    batched.close()
+    alone.close()
+    assert max(r[0]["batch_requests"] for r in rb) > 1
+    for (s, d, w, n), (resb, eb), (resa, ea) in zip(reqs, rb, ra):
+        assert resb["status"] == 0 and resa["status"] == 0
+        exp, exp_w = oracle.kruskal(k.to_wedges(s, d, w))
+        assert resb["total_weight"] == exp_w == resa["total_weight"]
+        assert np.array_equal(eb[0], exp["src"]) and np.array_equal(eb[1], exp["dst"])
+        assert np.array_equal(eb[2], exp["weight"])
+        for x, y in zip(eb, ea):
+            assert np.array_equal(x, y)
+
+
 def test_bad_request_does_not_poison_batch(gpu):
     reqs = requests(12, 5)
     s, d, w, n = reqs[4]
