@@ -675,39 +675,29 @@ void record_ranges(hipStream_t st, const u32* src, u64 m, u64 n, u32* seg_start,
   KMST_HIP(hipGetLastError());
 }
 
-// Input position of the twin (d, s, w) of record i = (s, d, w), found by
-// binary search in d's record range [lo, hi) of an input sorted by (src, dst);
-// within runs of equal (src, dst) (multigraph inputs) the k-th record of
-// weight w pairs with the k-th twin of weight w.  NO_TWIN when there is none.
+// Position of the first record of the pair (x, y), x < y, in x's record
+// range [lo, hi) of an input sorted by (src, dst) (binary search), NO_TWIN
+// when the pair has no record there.  It is the positional key's uid: every
+// parallel copy of an edge, in both directions, gets the same position, so
+// copies of equal weight tie (they are the same (w, min, max) triple, as
+// with packed keys) and copies of different weights differ in the weight
+// field; the search is O(log degree) however long a run of copies is.
 constexpr u64 NO_TWIN = ~u64{0};
 
-__device__ __forceinline__ u64 find_twin(const u32* __restrict__ src, const u32* __restrict__ dst,
-                                         const u32* __restrict__ w, u64 i, u32 s, u32 d, u32 wi,
-                                         u64 lo, u64 hi) {
-  u32 before = 0;
-  for (u64 j = i; j > 0; --j) {
-    if (src[j - 1] != s || dst[j - 1] != d) break;
-    before += (w[j - 1] == wi) ? 1u : 0u;
-  }
+__device__ __forceinline__ u64 pair_position(const u32* __restrict__ dst, u32 y, u64 lo, u64 hi) {
   const u64 end = hi;
   while (lo < hi) {
     const u64 mid = (lo + hi) >> 1;
-    if (dst[mid] < s) lo = mid + 1;
+    if (dst[mid] < y) lo = mid + 1;
     else hi = mid;
   }
-  for (u64 j = lo; j < end && dst[j] == s; ++j) {
-    if (w[j] == wi) {
-      if (before == 0) return j;
-      --before;
-    }
-  }
-  return NO_TWIN;
+  return (lo < end && dst[lo] == y) ? lo : NO_TWIN;
 }
 
 // Positional strict key, used when the packed key does not fit 64 bits:
-// val = (w << 32) | uid, uid = position of the (min, max) twin: the record's
-// own for s < d, found by find_twin in d's range [lo[d], hi[d]) for s > d.
-// The input is validated.  (Listing the s > d records of a wave first and
+// val = (w << 32) | uid, uid = pair_position of (min, max) in min's range
+// [lo[min], hi[min]): the record's own position for an s < d record that no
+// copy of itself precedes, a binary search otherwise.  The input is validated.  (Listing the s > d records of a wave first and
 // searching them one per lane measured slower here, 4.2 against 3.2 ms at
 // RGG2D 2^24: this pass has no stage to hide the list behind.)
 __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src,
@@ -719,11 +709,12 @@ __global__ __launch_bounds__(BLOCK) void keys_kernel(const u32* __restrict__ src
   u32 bad = 0;
   for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
     const u32 s = src[i], d = dst[i], wi = w[i];
-    if (s < d) {
+    if (s < d && (i == 0 || src[i - 1] != s || dst[i - 1] != d)) {
       val[i] = (u64(wi) << 32) | i;
       continue;
     }
-    const u64 j = find_twin(src, dst, w, i, s, d, wi, row_lo[d], row_hi[d]);
+    const u32 x = min(s, d), y = max(s, d);
+    const u64 j = pair_position(dst, y, row_lo[x], row_hi[x]);
     if (j == NO_TWIN) {
       val[i] = VAL_NONE;
       bad |= ERR_ASYMMETRIC;
@@ -2689,8 +2680,8 @@ __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
     keep = gd != gu;
     both = keep && (inside ? ((inside[d >> 6] >> (d & 63)) & 1ull) != 0 : gd == root);
     if (keys.twin_lo && keep && need_key) {
-      u64 j = e;
-      if (u > d) j = find_twin(in_s, in_d, keys.w, e, u, d, w, keys.twin_lo[d], keys.twin_hi[d]);
+      const u32 x = min(u, d), y = max(u, d);
+      const u64 j = pair_position(in_d, y, keys.twin_lo[x], keys.twin_hi[x]);
       if (j == NO_TWIN) {  // the input check saw every twin: not reached
         atomicOr(err, ERR_ASYMMETRIC);
         keep = both = false;
@@ -2856,9 +2847,10 @@ void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits)
 // CHECK the same pass validates the input as validate_kernel does (ids, order,
 // symmetry hashes, forward count, every vertex's record range), so the input
 // is read once for both.  With LAZY (positional keys) the keys of the written
-// records are made here: (w << 32) | the record's own position for s < d, the
-// twin's (find_twin in the ranges keys.twin_lo / twin_hi) for s > d, so that
-// no key array of the whole input is built.
+// records are made here: (w << 32) | pair_position of (min, max) in the
+// ranges keys.twin_lo / twin_hi (the record's own position for an s < d
+// record that no copy of itself precedes), so that no key array of the whole
+// input is built.
 // A block takes a chunk of consecutive units per step (two per wave, one
 // with the input check, which needs the registers).  The chunk's class 0 and
 // class 1 records are contiguous in the output, so they are staged in LDS and
@@ -2883,11 +2875,9 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
   __shared__ u32 st_d[CS_CHUNK_RECORDS];
   __shared__ u64 st_v[CS_CHUNK_RECORDS];
   __shared__ u64 s_red[WAVES][3];
-  // lazy keys: every wave's twin searches of the chunk, (stage position << 16)
-  // | record index in the chunk
+  // lazy keys: every wave's searches of the chunk, by stage position
   constexpr int TASKS = LAZY ? CS_WAVE_UNITS * int(UNIT_RECORDS) : 1;
   __shared__ u32 s_task[WAVES][TASKS];
-  static_assert(!LAZY || CS_CHUNK_RECORDS <= (1 << 16), "task encoding");
   const int lane = lane_id(), wv = wave_id();
   const u64 units = record_units_dev(m);
   const u64 chunks = (units + CS_CHUNK - 1) / CS_CHUNK;
@@ -2962,6 +2952,7 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
 #pragma unroll
         for (int e = 0; e < 4; ++e) vv[u][e] = pack_key(ww[u][e], ss[u][e], dd[u][e], keys.vbits);
       }
+      bool dup[4] = {false, false, false, false};  // a copy of the record precedes it
       if (CHECK) {
         // the record before lane l's first one is lane l - 1's last
         u32 ps = __shfl_up(ss[u][3], 1, WAVE), pd = __shfl_up(dd[u][3], 1, WAVE);
@@ -2986,6 +2977,7 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
           }
           if (s >= chk.n || d >= chk.n || s == d) bad |= ERR_INVALID;
           if (has_prev && (ps > s || (ps == s && pd > d))) bad |= ERR_UNSORTED;
+          dup[e] = has_prev && ps == s && pd == d;
           if (chk.seg_end && (!has_prev || ps != s)) {
             // record i opens s's range and closes the previous vertex's
             if (s < chk.n) chk.seg_start[s] = static_cast<u32>(i);
@@ -3009,16 +3001,18 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
         b0[e] = __ballot(cls[e] == 0);
         b1[e] = __ballot(cls[e] == 1);
       }
-      // lazy keys: s < d records key their own position; s > d records get a
-      // twin search task (ids checked), run after the chunk is staged
+      // lazy keys: an s < d record that no copy of itself precedes keys its
+      // own position; the others get a pair_position task (ids checked), run
+      // after the chunk is staged
       bool need[4] = {false, false, false, false};
       u64 bt[4] = {0, 0, 0, 0};
       if (LAZY) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const u32 s = ss[u][e], d = dd[u][e];
-          need[e] = cls[e] < 2 && s > d && s < chk.n && d < chk.n;
-          vv[u][e] = (u64(ww[u][e]) << 32) | (s < d ? r0 + e : u64{0xFFFFFFFFu});
+          const bool own = s < d && !dup[e];
+          need[e] = cls[e] < 2 && !own && s != d && s < chk.n && d < chk.n;
+          vv[u][e] = (u64(ww[u][e]) << 32) | (own ? r0 + e : u64{0xFFFFFFFFu});
           bt[e] = __ballot(need[e]);
         }
       }
@@ -3034,9 +3028,7 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
             st_s[p] = ss[u][e];
             st_d[p] = dd[u][e];
             st_v[p] = vv[u][e];
-            if (LAZY && need[e])
-              s_task[wv][tslot + mbcnt(bt[e])] =
-                  (u32(p) << 16) | static_cast<u32>(r0 + e - first * UNIT_RECORDS);
+            if (LAZY && need[e]) s_task[wv][tslot + mbcnt(bt[e])] = static_cast<u32>(p);
           } else {
             atomicOr(err, ERR_BOUNDS);
             if (LAZY && need[e]) s_task[wv][tslot + mbcnt(bt[e])] = NONE32;  // no search
@@ -3050,16 +3042,15 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
     if (LAZY) {
       // the wave's twin searches, one per lane
       for (u32 t = lane; t < ntask; t += WAVE) {
-        const u32 task = s_task[wv][t];
-        if (task == NONE32) continue;
-        const u32 p = task >> 16;
-        const u64 i = first * UNIT_RECORDS + (task & 0xFFFFu);
+        const u32 p = s_task[wv][t];
+        if (p == NONE32) continue;
         const u32 s = st_s[p], d = st_d[p];
         const u32 wi = static_cast<u32>(st_v[p] >> 32);
+        const u32 x = min(s, d), y = max(s, d);
         // unchecked ranges: keep the search inside the arrays
-        const u64 lo = min<u64>(keys.twin_lo[d], m);
-        const u64 hi = min<u64>(max<u64>(keys.twin_hi[d], lo), m);
-        const u64 j = find_twin(s_in, d_in, keys.w, i, s, d, wi, lo, hi);
+        const u64 lo = min<u64>(keys.twin_lo[x], m);
+        const u64 hi = min<u64>(max<u64>(keys.twin_hi[x], lo), m);
+        const u64 j = pair_position(d_in, y, lo, hi);
         if (j == NO_TWIN) {
           bad |= ERR_ASYMMETRIC;
           st_v[p] = VAL_NONE;

@@ -146,8 +146,9 @@ def test_multigraph(gpu):
 
 def test_positional_multigraph(gpu):
     """Parallel copies under positional keys (31-bit weights, n > 2^16), many
-    of equal weight: the k-th copy of weight w pairs with the k-th reverse copy
-    of weight w in the key pass (plain Boruvka) and in the keys that
+    of equal weight: every copy of a pair, in both directions, keys the
+    position of the pair's first record (equal-weight copies tie, as with
+    packed keys), in the key pass (plain Boruvka) and in the keys that
     filter-Boruvka's class scatter and outside gather make for the records
     they write."""
     rng = np.random.default_rng(41)
