@@ -1101,6 +1101,8 @@ __global__ __launch_bounds__(BLOCK) void jump_kernel(u64 n, u32* parent, u32* er
   u32 x = parent[v];
   u32 y = parent[x];
   u32 guard = 0;
+  // every step stores its progress, which shortens the walks of vertices
+  // that pass through v; a vertex already pointing at its root stores nothing
   while (x != y) {
     parent[v] = y;
     x = y;
@@ -1110,13 +1112,57 @@ __global__ __launch_bounds__(BLOCK) void jump_kernel(u64 n, u32* parent, u32* er
       return;
     }
   }
-  parent[v] = x;
 }
 
-void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err) {
+// jump_kernel that also counts the roots with records (parent[v] == v and
+// seg_end[v] != 0) into *roots: grid-strided over at most 2048 workgroups,
+// one atomic each (the block-final atomics of a one-shot grid of n / 256
+// workgroups on one counter would serialise).
+__global__ __launch_bounds__(BLOCK) void jump_count_kernel(u64 n, u32* parent,
+                                                           const u32* __restrict__ seg_end,
+                                                           u32* roots, u32* err) {
+  __shared__ u32 s_c[WAVES];
+  u32 c = 0;
+  bool spun = false;
+  for (u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x; v < n && !spun;
+       v += u64(gridDim.x) * BLOCK) {
+    u32 x = parent[v];
+    if (x == v) {  // a root stays one: only non-roots store
+      c += seg_end[v] ? 1u : 0u;
+      continue;
+    }
+    u32 y = parent[x];
+    u32 guard = 0;
+    while (x != y) {
+      parent[v] = y;
+      x = y;
+      y = parent[x];
+      if (++guard > (1u << 26)) {
+        atomicOr(err, ERR_SPIN);
+        spun = true;
+        break;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
+  if (lane_id() == 0) s_c[wave_id()] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 t = 0;
+    for (int w = 0; w < WAVES; ++w) t += s_c[w];
+    if (t) atomicAdd(roots, t);
+  }
+}
+
+void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err, const u32* seg_end, u32* roots) {
+  if (roots) fill_u32(st, roots, 1, 0);
   if (n == 0) return;
-  KTScope t(st, K_JUMP, 12.0 * n);
-  jump_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, parent, err);
+  KTScope t(st, K_JUMP, 12.0 * n + (roots ? 4.0 * n : 0.0));
+  if (roots)
+    jump_count_kernel<<<grid_for(n, 2048), BLOCK, 0, st>>>(n, parent, seg_end, roots, err);
+  else
+    jump_kernel<<<ceil_div_u32(n, BLOCK), BLOCK, 0, st>>>(n, parent, err);
   KMST_HIP(hipGetLastError());
 }
 

@@ -93,7 +93,9 @@ void unordered_argmin(hipStream_t st, const u32* src, const u32* dst, const KeyS
 // K3 pointer jumping; renumbering + MST emission from the scanned state.
 void hook(hipStream_t st, u64 n, const u64* best_val, const u32* best_dst, u32* parent,
           u64* state);
-void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err);
+// With roots, also counts the roots v with seg_end[v] != 0 into *roots.
+void pointer_jump(hipStream_t st, u64 n, u32* parent, u32* err, const u32* seg_end = nullptr,
+                  u32* roots = nullptr);
 // Pointer jumping, labels, MST emission and (with gpar) the parent-array
 // update of one round in one pass; the state scan must have run.  Without
 // root_id (n < 2^31) the scan has written every surviving root's new id into

@@ -156,6 +156,7 @@ enum Ctr : int {
   C_ERR = 0, C_FWD = 1, C_PIECES = 3, C_NNEXT = 4, C_MNEXT = 5, C_LIGHT = 6, C_MAXW = 7,
   C_MSTLIST = 8, C_MSTLONG = 9,  // output order: listed long groups, a group too long
   C_RUNS = 10,                    // run starts among the next round's sampled records
+  C_ROOTS = 11,                   // parent-array roots with records (after a flatten)
   C_TOTAL64 = 16, C_HSUM64 = 18, C_ROUND64 = 22,
   C_CLASS64 = 24,   // MAX_CLASSES u64 class counts (words 24..41)
   C_OUTSIDE = 60    // outside_list: count (60), u64 total range length (62..63)
@@ -269,6 +270,9 @@ class Solver {
   // the most frequent root of a sample of vertices (giant_root) and its
   // sample share
   bool one_component();
+  // ctr + C_ROOTS holds the parent array's roots with records: the last
+  // rounds flattened it and nothing changed it since
+  bool roots_counted = false;
   u32 giant_root = 0;
   double giant_share = 0.0;
   u32 pick_pivot(RecView r);
@@ -512,7 +516,12 @@ void Solver::rounds(RecView recs, RecView A, RecView B, bool track_gpar, u64 n_s
     }
     first = false;
   }
-  if (track_gpar) pointer_jump(st, n0, gpar, ctr + C_ERR);
+  if (track_gpar) {
+    // flattened, the parent array's roots with records are counted in the
+    // same pass (the one-component test before a heavy class reads them)
+    pointer_jump(st, n0, gpar, ctr + C_ERR, seg_end, ctr + C_ROOTS);
+    roots_counted = true;
+  }
 }
 
 // A round on records in compaction order skips the sort when it has at
@@ -778,7 +787,7 @@ u64 Solver::partition(RecView in, RecView out, u32 pivot, bool write_heavy) {
 // same-component edges, sort + keep the lightest copies, recurse
 // (boruvka_recursive_setup, hybrid_boruvka_edgefilter.hpp:173-225).
 bool Solver::one_component() {
-  count_roots(st, gpar, seg_end, n0, ctr + C_NNEXT);
+  if (!roots_counted) count_roots(st, gpar, seg_end, n0, ctr + C_ROOTS);
   const u32 k = std::min<u32>(GPAR_SAMPLES, n0);
   sample_weights(st, nullptr, gpar, n0, k, 32, samples);
   KMST_HIP(hipMemcpyAsync(C.host_counters, ctr, 256, hipMemcpyDeviceToHost, st));
@@ -798,7 +807,7 @@ bool Solver::one_component() {
     i = j;
   }
   giant_share = k ? double(best) / double(k) : 0.0;
-  return C.host_counters[C_NNEXT] <= 1;
+  return C.host_counters[C_ROOTS] <= 1;
 }
 
 void Solver::filter_heavy(RecView heavy, RecView scratch, RecView alt, int depth, u64 key_min,
