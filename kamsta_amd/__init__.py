@@ -162,6 +162,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run `python -m kamsta_amd.build` first")
+    # torch ships its own HIP runtime under the same soname: when torch is
+    # installed it is loaded first, so that libkmst binds to that runtime
+    # rather than starting the system one, after which torch finds no GPU
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
     pu64 = ctypes.POINTER(ctypes.c_uint64)
