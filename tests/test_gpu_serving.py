@@ -81,6 +81,42 @@ def test_batched_wide_weights(gpu):
             assert np.array_equal(x, y)
 
 
+def test_batch_of_mixed_upload_forms(gpu):
+    """A batch mixing requests of byte weights and of wide weights, with
+    isolated vertices, an empty one, and one whose records are out of order
+    (refused when it is solved alone; every other request is exact)."""
+    rng = np.random.default_rng(23)
+    reqs = []
+    for i in range(24):
+        n = int(2 ** rng.uniform(6, 12))
+        m = int(n * rng.uniform(2, 8) / 2)
+        mw = 254 if i % 3 else (1 << 20)
+        s, d, w = k.gen_gnm_host(n, m, mw, 9000 + i)
+        reqs.append((s, d, w, n + (7 if i % 5 == 0 else 0)))  # some with isolated ids
+    reqs[10] = (np.zeros(0, np.uint32),) * 3 + (16,)
+    bad = 17
+    s, d, w, n = reqs[bad]
+    s, d = s.copy(), d.copy()
+    s[[3, len(s) - 3]] = s[[len(s) - 3, 3]]
+    d[[3, len(d) - 3]] = d[[len(d) - 3, 3]]
+    reqs[bad] = (s, d, w, n)
+    srv = k.Server(algorithm=k.FILTER_BORUVKA, max_batch_requests=4096)
+    res = solve_all(srv, reqs)
+    srv.close()
+    for i, ((s, d, w, n), (r, e)) in enumerate(zip(reqs, res)):
+        if i == bad:
+            assert r["status"] in (k.KMST_EUNSORTED, k.KMST_EASYMMETRIC), r["status"]
+            continue
+        assert r["status"] == 0, (i, r["status"])
+        if len(s) == 0:
+            assert r["mst_edges"] == 0
+            continue
+        exp, exp_w = oracle.kruskal(k.to_wedges(s, d, w))
+        assert r["total_weight"] == exp_w
+        assert np.array_equal(e[0], exp["src"]) and np.array_equal(e[1], exp["dst"])
+        assert np.array_equal(e[2], exp["weight"])
+
+
 def test_bad_request_does_not_poison_batch(gpu):
     reqs = requests(12, 5)
     s, d, w, n = reqs[4]
