@@ -231,8 +231,8 @@ __device__ __forceinline__ u64 lookback_exclusive(u64* status, u32 epoch, u32 ti
 // (w << 2b) | (min(s, d) << b) | max(s, d) with b = vbits.  With twin_lo set
 // (positional mode on the read-only input, taken only by the kernels that
 // read the input directly: the class scatter and the outside gather) the key
-// is (w << 32) | the position of the (min, max) twin, found for a record
-// (s, d) with s > d by a search in d's record range [twin_lo[d], twin_hi[d]).
+// is (w << 32) | the position of the first record of the pair (min, max),
+// found by a search in min's record range [twin_lo[min], twin_hi[min]).
 struct KeySrc {
   const u64* v = nullptr;
   const u32* w = nullptr;

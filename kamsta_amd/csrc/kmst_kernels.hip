@@ -2667,7 +2667,7 @@ void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u
 // pass (whose reads hit the cache) writes it.  Output order is arbitrary;
 // positions at or past cap raise ERR_BOUNDS.  With lazy positional keys
 // (keys.twin_lo) [kmin, kmax] are weight bounds shifted by 32, and only the
-// emitted records' keys are made (their twin search in the emitting pass).
+// emitted records' keys are made (their pair_position search in the emitting pass).
 __global__ __launch_bounds__(BLOCK) void outside_gather_kernel(
     const u32* __restrict__ list, const u32* __restrict__ seg_lo, const u32* __restrict__ seg_hi,
     u32 n_list, const u32* __restrict__ in_s, const u32* __restrict__ in_d, const KeySrc keys,
@@ -3086,7 +3086,7 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
     }
     __syncthreads();
     if (LAZY) {
-      // the wave's twin searches, one per lane
+      // the wave's pair_position searches, one per lane
       for (u32 t = lane; t < ntask; t += WAVE) {
         const u32 p = s_task[wv][t];
         if (p == NONE32) continue;

@@ -72,7 +72,7 @@ void build_offsets(hipStream_t st, const u32* src, u64 m, u64 n, u32* off);
 // Record ranges [seg_start[v], seg_end[v]) of records sorted by src, written
 // for the vertices with records only; bounded work on an unchecked input.
 void record_ranges(hipStream_t st, const u32* src, u64 m, u64 n, u32* seg_start, u32* seg_end);
-// Positional keys of a validated input (twin search in the record ranges
+// Positional keys of a validated input (pair_position search in the record ranges
 // [row_lo[v], row_hi[v]): the offsets off / off + 1, or seg_start / seg_end).
 void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u64 m,
                 const u32* row_lo, const u32* row_hi, u64* val, u32* err);
@@ -156,7 +156,7 @@ void partition_giant_bits(hipStream_t st, const u32* gpar, const u32* seg_end, u
 // cross-component records in both directions (see the kernel), at most
 // 2 * total of them.  With lazy positional keys (keys.twin_lo / twin_hi the
 // validated record ranges) [kmin, kmax] bound the weight << 32 and the keys
-// of the emitted records are made by their twin search.
+// of the emitted records are made by their pair_position search.
 void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u32* seg_end,
                   u64 n, u32 root, const u64* inside, u32* list, u32* seg_lo, u32* seg_hi,
                   u32* counters);

@@ -279,7 +279,7 @@ class Solver {
   u64 partition(RecView in, RecView out, u32 pivot, bool write_heavy = true);
   // keys of r: stored, packed from the weights, or (positional mode, r the
   // input without stored keys) made by the kernels that read the input, with
-  // the twin search in the validated record ranges
+  // the pair_position search in the validated record ranges
   KeySrc keys_of(const RecView& r) const {
     KeySrc k{r.v, r.w, vbits};
     if (!r.v && !packed) {
@@ -385,7 +385,7 @@ void Solver::setup(bool filter, bool plan_classes) {
       return;
     }
     // positional keys: the class scatter validates the input and makes the
-    // keys of the records it writes, by a twin search in the record ranges
+    // keys of the records it writes, by a pair_position search in the record ranges
     // (clamped to the arrays while the input is unchecked), recorded first
     record_ranges(st, in_s, m0, n0, seg_start, seg_end);
     check_in_scatter = true;
