@@ -78,6 +78,7 @@ struct KernelTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
   std::vector<Mark> marks;
   size_t used = 0;
+  int depth = 0;  // open scopes: a scope opened inside another is not timed
   std::map<int, KernelTotals> totals;
   void begin(hipStream_t st, int kind, double bytes);
   void end(hipStream_t st) noexcept;

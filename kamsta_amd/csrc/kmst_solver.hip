@@ -38,6 +38,10 @@ thread_local KernelTimer* g_ktimer = nullptr;
 
 // ------------------------------------------------------------ kernel timer
 void KernelTimer::begin(hipStream_t st, int kind, double bytes) {
+  if (depth > 0) {  // inside an open scope: its launches count for that scope
+    ++depth;
+    return;
+  }
   if (used == pool.size()) {
     hipEvent_t a, b;
     KMST_HIP(hipEventCreate(&a));
@@ -46,8 +50,10 @@ void KernelTimer::begin(hipStream_t st, int kind, double bytes) {
   }
   KMST_HIP(hipEventRecord(pool[used].first, st));
   marks.push_back({kind, bytes});
+  depth = 1;
 }
 void KernelTimer::end(hipStream_t st) noexcept {
+  if (depth == 0 || --depth > 0) return;
   (void)hipEventRecord(pool[used].second, st);
   ++used;
 }
@@ -61,6 +67,7 @@ void KernelTimer::collect() {
     acc.bytes += marks[i].bytes;
   }
   used = 0;
+  depth = 0;
   marks.clear();
 }
 KernelTimer::~KernelTimer() {
