@@ -5,6 +5,7 @@
 //   src[i], dst[i]  u32 endpoint ids in the current (compacted) numbering
 //   val[i]          u64 strict key (weight << 32) | uid, uid = input position
 //                   of the (min, max) twin; VAL_NONE marks a dead record.
+#include <type_traits>
 #include "kmst_kernels.hpp"
 
 namespace kmst {
@@ -169,18 +170,17 @@ void scan_exclusive_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* scr
 // self-loops, ids < n (include/util/utils.hpp:391-395).  Also gathers the
 // largest weight (to size the packed key), the number of min<max records,
 // and hash sums of (min, max, w) over min<max and over max<min records:
-// the input is symmetric as a multiset iff both hash sums and counts agree
-// (up to a 2^-64 collision chance).
-// Multiplications by odd 32-bit constants are bijections mod 2^64 and cost
-// three 32-bit multiplies each; the xor-shifts carry the high bits down.
+// the input is symmetric as a multiset iff both counts agree and the hash
+// sums agree, up to a collision of the sums.
+// The hash costs two 32 x 32 -> 64 multiply-adds (the passes that take it
+// are bound by instruction issue): the first mixes the pair non-linearly, its
+// addend (b, a) keeping distinct pairs apart where a factor is zero; the
+// second folds the weight in, so that for a fixed pair the hash is injective
+// in w (a reverse record of another weight always changes the sums).
 __device__ __forceinline__ u64 sym_hash(u32 a, u32 b, u32 w) {
-  u64 x = (u64(a) << 32 | b) * 0x9E3779B1ull;
-  x ^= (x >> 29) ^ (u64(w) << 17) ^ w;
-  x *= 0x85EBCA77ull;
-  x ^= x >> 32;
-  x *= 0xC2B2AE3Dull;
-  x ^= x >> 29;
-  return x;
+  const u64 p = u64(a ^ 0x9E3779B9u) * u64(b ^ 0x85EBCA77u) + ((u64(b) << 32) | a);
+  const u32 c = static_cast<u32>(p) ^ static_cast<u32>(p >> 32) ^ w;
+  return u64(c) * 0xC2B2AE3Du + p;
 }
 
 __global__ __launch_bounds__(BLOCK) void fingerprint_kernel(const u32* __restrict__ s,
@@ -453,7 +453,10 @@ void unit_class_counts(hipStream_t st, const u32* w, u64 m, u32 q0, u32 q1, u32*
 // class and the class 0 / class 1 counts of every 256-record unit.  A wave
 // takes WC_UNITS consecutive units per step (lane l: records 4l .. 4l + 3 of
 // each, one 16-byte load), loads them all before counting, and writes their
-// unit counts with one store.
+// unit counts with one store.  The pass is bound by instruction issue, not by
+// HBM, unless it stays lean: a step whose units are all in range counts
+// classes 0 and 1 with one compare and a ballot per 64 records (the counts
+// stay wave-uniform), and only the pivots that exist are compared.
 constexpr int WC_UNITS = 4;
 
 template <bool VEC>
@@ -465,15 +468,60 @@ __global__ __launch_bounds__(BLOCK) void weight_classes_kernel(
   const u64 units = record_units_dev(m);
   const u64 steps = (units + WC_UNITS - 1) / WC_UNITS;
   const u64 wstride = u64(gridDim.x) * WAVES;
-  // above[j]: records with w > q[j]; for j < 2 they follow from the class 0
-  // and class 1 tests (q0 = q[0], q1 = q[1]) every record takes anyway
+  // per lane: the largest weight, and (partial steps) records seen and
+  // class 0 / class 1 totals; wave-uniform: the same totals of full steps
   u32 mw = 0, seen = 0, tot0 = 0, tot1 = 0;
+  u32 useen = 0, utot0 = 0, utot1 = 0;
+  // above[j]: records with w > q[j]; for j < 2 they follow from the class 0
+  // and class 1 totals
   u32 above[MAX_PIVOTS];
 #pragma unroll
   for (int j = 0; j < MAX_PIVOTS; ++j) above[j] = 0;
   for (u64 step = u64(blockIdx.x) * WAVES + wave_id(); step < steps; step += wstride) {
     const u64 u0 = step * WC_UNITS;
     u32 w4[WC_UNITS][4];
+    if (VEC && (u0 + WC_UNITS) * UNIT_RECORDS <= m) {  // wave-uniform
+#pragma unroll
+      for (int u = 0; u < WC_UNITS; ++u) {
+        const uint4 a =
+            *reinterpret_cast<const uint4*>(wt + (u0 + u) * UNIT_RECORDS + 4 * u64(lane));
+        w4[u][0] = a.x; w4[u][1] = a.y; w4[u][2] = a.z; w4[u][3] = a.w;
+      }
+      u32 mine0 = 0, mine1 = 0;  // lane u < WC_UNITS keeps unit u0 + u's counts
+#pragma unroll
+      for (int u = 0; u < WC_UNITS; ++u) {
+        u32 t0 = 0, t01 = 0;  // wave-uniform
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const u32 w = w4[u][e];
+          mw = w > mw ? w : mw;
+          t0 += static_cast<u32>(__popcll(__ballot(w <= q0)));
+          t01 += static_cast<u32>(__popcll(__ballot(w <= q1)));
+        }
+        if (lane == u) {
+          mine0 = t0;
+          mine1 = t01 - t0;
+        }
+        utot0 += t0;
+        utot1 += t01 - t0;
+      }
+#pragma unroll
+      for (u32 j = 2; j < MAX_PIVOTS; ++j) {
+        if (j < P.K) {  // uniform
+#pragma unroll
+          for (int u = 0; u < WC_UNITS; ++u)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) above[j] += (w4[u][e] > P.q[j]) ? 1u : 0u;
+        }
+      }
+      useen += WC_UNITS * static_cast<u32>(UNIT_RECORDS);
+      if (lane < WC_UNITS) {
+        unit_c0[u0 + lane] = mine0;
+        unit_c1[u0 + lane] = mine1;
+      }
+      continue;
+    }
+    // a step past the end of the records (or unaligned weights)
 #pragma unroll
     for (int u = 0; u < WC_UNITS; ++u) {
       const u64 r0 = (u0 + u) * UNIT_RECORDS + 4 * u64(lane);
@@ -485,7 +533,7 @@ __global__ __launch_bounds__(BLOCK) void weight_classes_kernel(
         for (int e = 0; e < 4; ++e) w4[u][e] = (r0 + e < m) ? wt[r0 + e] : 0u;
       }
     }
-    u32 mine0 = 0, mine1 = 0;  // lane u < WC_UNITS keeps unit u0 + u's counts
+    u32 mine0 = 0, mine1 = 0;
 #pragma unroll
     for (int u = 0; u < WC_UNITS; ++u) {
       const u64 unit = u0 + u;  // wave-uniform
@@ -522,6 +570,12 @@ __global__ __launch_bounds__(BLOCK) void weight_classes_kernel(
       unit_c0[u0 + lane] = mine0;
       unit_c1[u0 + lane] = mine1;
     }
+  }
+  // the wave-uniform totals count once, on lane 0
+  if (lane == 0) {
+    seen += useen;
+    tot0 += utot0;
+    tot1 += utot1;
   }
   if (P.K >= 1) above[0] = seen - tot0;
   if (P.K >= 2) above[1] = seen - tot0 - tot1;
@@ -2900,13 +2954,22 @@ void component_bits(hipStream_t st, const u32* gpar, u64 n, u32 root, u64* bits)
 // A block takes a chunk of consecutive units per step (two per wave, one
 // with the input check, which needs the registers).  The chunk's class 0 and
 // class 1 records are contiguous in the output, so they are staged in LDS and
-// leave in coalesced runs.
+// leave in coalesced runs.  With packed keys the raw weight is staged and the
+// key made on the way out, for the written records only: the per-record work
+// of this pass (the checks, the symmetry hash, the class tests) costs about as
+// much issue time as its HBM traffic takes.
 template <bool CHECK>
 struct ScatterChunk {
   static constexpr int WAVE_UNITS = CHECK ? 1 : 2;
   static constexpr int UNITS = WAVES * WAVE_UNITS;
   static constexpr int RECORDS = UNITS * int(UNIT_RECORDS);
 };
+
+// popcount(mask & lanes-below-me) + acc
+__device__ __forceinline__ u32 mbcnt_acc(u64 mask, u32 acc) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<u32>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<u32>(mask), acc));
+}
 
 template <bool VEC, bool CHECK, bool LAZY>
 __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
@@ -2930,11 +2993,15 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
   const u64 total0 = base1, total1 = out_cap - base1;  // class sizes
   u32 bad = 0, fwd = 0;
   u64 hf = 0, hr = 0;
+  // stored keys (never with the input check, which needs the weights)
+  const bool stored = !CHECK && keys.v != nullptr;
+  // packed keys: the raw weight is staged, the key made on the way out
+  const bool late_key = !LAZY && !stored;
   // a chunk's records of this wave.  Without the input check the next
   // chunk's are loaded before the current one is processed, so their latency
-  // overlaps the staging and the workgroup barriers (with the check the extra
-  // registers cost more occupancy than the overlap gains: 2.16 -> 2.33 ms at
-  // GNM 2^24 / 2^28, against 1.33 -> 1.07 ms without it at RGG2D 2^24)
+  // overlaps the staging and the workgroup barriers (1.33 -> 1.07 ms at RGG2D
+  // 2^24).  With the check it gains nothing (GNM 2^24 / 2^28: 1.91 against
+  // 1.93 ms) and takes 70 registers instead of 49.
   constexpr bool PREFETCH = !CHECK;
   auto load_chunk = [&](u64 chunk, u32 (&ls)[CS_WAVE_UNITS][4], u32 (&ld)[CS_WAVE_UNITS][4],
                         u32 (&lw)[CS_WAVE_UNITS][4], u64 (&lv)[CS_WAVE_UNITS][4]) {
@@ -2949,7 +3016,7 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
         const uint4 b = *reinterpret_cast<const uint4*>(d_in + r0);
         ls[u][0] = a.x; ls[u][1] = a.y; ls[u][2] = a.z; ls[u][3] = a.w;
         ld[u][0] = b.x; ld[u][1] = b.y; ld[u][2] = b.z; ld[u][3] = b.w;
-        if (keys.v) {
+        if (stored) {
           const uint4 c = *reinterpret_cast<const uint4*>(keys.v + r0);
           const uint4 e = *reinterpret_cast<const uint4*>(keys.v + r0 + 2);
           lv[u][0] = (u64(c.y) << 32) | c.x;
@@ -2966,7 +3033,7 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
           const u64 i = min(r0 + e, m - 1);
           ls[u][e] = s_in[i];
           ld[u][e] = d_in[i];
-          if (keys.v) lv[u][e] = keys.v[i];
+          if (stored) lv[u][e] = keys.v[i];
           else lw[u][e] = keys.w[i];
         }
       }
@@ -2994,10 +3061,6 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
       const u64 unit = first + wv * CS_WAVE_UNITS + u;  // wave-uniform
       if (unit >= units) break;
       const u64 r0 = unit * UNIT_RECORDS + 4 * u64(lane);
-      if (!keys.v && !LAZY) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) vv[u][e] = pack_key(ww[u][e], ss[u][e], dd[u][e], keys.vbits);
-      }
       bool dup[4] = {false, false, false, false};  // a copy of the record precedes it
       if (CHECK) {
         // the record before lane l's first one is lane l - 1's last
@@ -3042,7 +3105,7 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
       u64 b0[4], b1[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const u32 w = keys.v ? static_cast<u32>(vv[u][e] >> wshift) : ww[u][e];
+        const u32 w = stored ? static_cast<u32>(vv[u][e] >> wshift) : ww[u][e];
         cls[e] = (r0 + e >= m) ? 2u : (w <= q0 ? 0u : (w <= q1 ? 1u : 2u));
         b0[e] = __ballot(cls[e] == 0);
         b1[e] = __ballot(cls[e] == 1);
@@ -3064,16 +3127,20 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
       }
       u32 tslot = ntask;
       // stage positions: lane l's records follow those of lanes < l
-      u64 p0 = off0[unit] - a0 + mbcnt(b0[0]) + mbcnt(b0[1]) + mbcnt(b0[2]) + mbcnt(b0[3]);
-      u64 p1 = n0 + off1[unit] - a1 + mbcnt(b1[0]) + mbcnt(b1[1]) + mbcnt(b1[2]) + mbcnt(b1[3]);
+      u32 p0 = static_cast<u32>(off0[unit] - a0), p1 = static_cast<u32>(n0 + off1[unit] - a1);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        p0 = mbcnt_acc(b0[e], p0);
+        p1 = mbcnt_acc(b1[e], p1);
+      }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (cls[e] < 2) {
-          const u64 p = cls[e] == 0 ? p0++ : p1++;
-          if (p < u64(CS_CHUNK_RECORDS)) {
+          const u32 p = cls[e] == 0 ? p0++ : p1++;
+          if (p < u32(CS_CHUNK_RECORDS)) {
             st_s[p] = ss[u][e];
             st_d[p] = dd[u][e];
-            st_v[p] = vv[u][e];
+            st_v[p] = late_key ? u64(ww[u][e]) : vv[u][e];
             if (LAZY && need[e]) s_task[wv][tslot + mbcnt(bt[e])] = static_cast<u32>(p);
           } else {
             atomicOr(err, ERR_BOUNDS);
@@ -3113,7 +3180,8 @@ __global__ __launch_bounds__(BLOCK) void class_scatter_kernel(
       if (p < out_cap) {
         s_out[p] = st_s[j];
         d_out[p] = st_d[j];
-        v_out[p] = st_v[j];
+        v_out[p] = late_key ? pack_key(static_cast<u32>(st_v[j]), st_s[j], st_d[j], keys.vbits)
+                            : st_v[j];
       } else {
         atomicOr(err, ERR_BOUNDS);
       }
