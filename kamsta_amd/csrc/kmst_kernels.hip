@@ -36,6 +36,30 @@ void fill_u32(hipStream_t st, u32* p, u64 n, u32 value) {
   KMST_HIP(hipGetLastError());
 }
 
+// Counters (and optionally a block of staged words) written straight into
+// coherent pinned host memory, then the sequence word with a system-scope
+// release: the host spins on the sequence word instead of a device-to-host
+// copy and a stream synchronisation.  One workgroup; every store is a vector
+// store.
+__global__ __launch_bounds__(BLOCK) void publish_kernel(const u32* __restrict__ ctr, u32 ctr_words,
+                                                        const u32* __restrict__ extra,
+                                                        u32 extra_words, u32* host,
+                                                        u32 extra_at, u32 seq_at, u32 seq) {
+  for (u32 i = threadIdx.x; i < ctr_words; i += BLOCK) host[i] = ctr[i];
+  for (u32 i = threadIdx.x; i < extra_words; i += BLOCK) host[extra_at + i] = extra[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(host + seq_at, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void publish_words(hipStream_t st, const u32* ctr, u32 ctr_words, const u32* extra,
+                   u32 extra_words, u32* host, u32 extra_at, u32 seq_at, u32 seq) {
+  publish_kernel<<<1, BLOCK, 0, st>>>(ctr, ctr_words, extra, extra_words, host, extra_at, seq_at,
+                                      seq);
+  KMST_HIP(hipGetLastError());
+}
+
 // ============================================================== scan
 // Three-phase exclusive scan: per-tile reduce, scan of tile sums (recursive),
 // per-tile downsweep.  Loads and stores are striped (element k * BLOCK + t of

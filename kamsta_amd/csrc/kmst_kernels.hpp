@@ -30,6 +30,11 @@ struct ArgPiece {
 
 void fill_u64(hipStream_t st, u64* p, u64 n, u64 value);
 void fill_u32(hipStream_t st, u32* p, u64 n, u32 value);
+// ctr[0, ctr_words) to host[0, ...), extra[0, extra_words) to host[extra_at, ...),
+// then host[seq_at] = seq with a system-scope release (host: coherent pinned
+// memory the device writes directly).
+void publish_words(hipStream_t st, const u32* ctr, u32 ctr_words, const u32* extra,
+                   u32 extra_words, u32* host, u32 extra_at, u32 seq_at, u32 seq);
 
 // Exclusive prefix sum of in[0,n) into out (may alias in); *total_dev gets the
 // sum when non-null.  `scratch` needs scan_scratch_elems(n) entries.

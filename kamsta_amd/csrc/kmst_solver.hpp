@@ -86,7 +86,9 @@ kmst_status guarded_call(F&& f) {
 }
 
 // pinned host words: counters in [0, 64), then HOST_STAGE_WORDS of staging
-// (vertex samples, or the persistent rounds' control words and log)
+// (vertex samples, or the persistent rounds' control words and log), then
+// the sequence word of the last publication (contexts with coherent pinned
+// words, which the device writes directly)
 constexpr u32 GPAR_SAMPLES = 256;
 constexpr u32 PIVOT_SAMPLES = 4095;  // input weights sampled for the top-level pivots
 constexpr u32 LEVEL_SAMPLES = 1023;  // weights sampled for a deeper level's pivot
@@ -97,7 +99,8 @@ static_assert(GPAR_SAMPLES <= HOST_STAGE_WORDS && LEVEL_SAMPLES <= HOST_STAGE_WO
               "sample staging");
 static_assert(PERSIST_CTL_WORDS + 2 * PERSIST_LOG_TOTAL <= HOST_STAGE_WORDS,
               "persistent-round staging");
-constexpr size_t HOST_COUNTER_BYTES = (HOST_SAMPLE_WORD + HOST_STAGE_WORDS) * 4;
+constexpr u32 HOST_SEQ_WORD = HOST_SAMPLE_WORD + HOST_STAGE_WORDS;
+constexpr size_t HOST_COUNTER_BYTES = (HOST_SEQ_WORD + 64) * 4;
 
 }  // namespace kmst
 
@@ -115,6 +118,10 @@ struct kmst_context {
   bool owns_resources = true;  // created the stream and the pinned counters
   hipStream_t stream = nullptr;
   kmst::u32* host_counters = nullptr;  // pinned, HOST_COUNTER_BYTES
+  // host_counters is coherent pinned memory the device writes directly
+  // (read-backs by publication); otherwise by copy and synchronisation
+  bool host_mapped = false;
+  kmst::u32 publish_seq = 0;
   bool timing = false;
   kmst::KernelTimer timer;
   kmst::SolveStats stats;
