@@ -1261,6 +1261,12 @@ __global__ __launch_bounds__(BLOCK) void jump_label_kernel(
     u32* __restrict__ rep_next, u32* err) {
   const u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x;
   if (v >= n) return;
+  // the vertex's own words are loaded before the walk, so that their
+  // latency overlaps the walk's dependent loads
+  const u64 sv = state[v];
+  const u64 pv = pos[v];
+  const u64 bv = best_val[v];
+  const u32 name = gpar ? (rep ? rep[v] : static_cast<u32>(v)) : 0u;
   u32 x = parent[v];
   u32 id;
   if (TAGGED && (x & ROOT_TAG)) {
@@ -1283,12 +1289,10 @@ __global__ __launch_bounds__(BLOCK) void jump_label_kernel(
     else id = root_id[x];
   }
   label[v] = id;
-  const u64 sv = state[v];
-  if (sv & ST_EMIT) mst[mst_base + (pos[v] >> 32)] = best_val[v];
+  if (sv & ST_EMIT) mst[mst_base + (pv >> 32)] = bv;
   if (gpar && sv != 0) {
-    const u32 name = rep ? rep[v] : static_cast<u32>(v);
     gpar[name] = rep ? rep[x] : x;
-    if (sv & ST_ROOT) rep_next[static_cast<u32>(pos[v])] = name;
+    if (sv & ST_ROOT) rep_next[static_cast<u32>(pv)] = name;
   }
 }
 

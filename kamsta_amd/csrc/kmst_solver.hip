@@ -15,6 +15,7 @@
 // include/algorithms/hybrid_boruvka_edgefilter.hpp:227-330) wraps the same
 // rounds in the pivot recursion: light/heavy stream split, recurse on light,
 // relabel heavy by the parent array, drop same-component edges, recurse.
+#include <cstdlib>
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -131,10 +132,21 @@ using namespace kmst;
 kmst_context::kmst_context(int dev) : device(dev) {
   KMST_HIP(hipSetDevice(device));
   KMST_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_counters), HOST_COUNTER_BYTES,
-                         hipHostMallocCoherent));
-  host_counters[HOST_SEQ_WORD] = 0;
-  host_mapped = true;
+  // coherent pinned words, which the device writes directly; where the
+  // system offers none (or KMST_READBACK_COPY=1 asks for it), default pinned
+  // words read back by copies
+  const char* copy_env = std::getenv("KMST_READBACK_COPY");
+  const bool publish = !(copy_env && copy_env[0] == '1');
+  if (publish && hipHostMalloc(reinterpret_cast<void**>(&host_counters), HOST_COUNTER_BYTES,
+                               hipHostMallocCoherent) == hipSuccess) {
+    host_counters[HOST_SEQ_WORD] = 0;
+    host_mapped = true;
+  } else {
+    (void)hipGetLastError();
+    host_counters = nullptr;
+    KMST_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_counters), HOST_COUNTER_BYTES,
+                           hipHostMallocDefault));
+  }
 }
 kmst_context::kmst_context(int dev, kmst::u32* host_words, kmst::Arena* arena)
     : device(dev), owns_resources(false), host_counters(host_words) {
@@ -1437,6 +1449,7 @@ size_t kmst_context_stats_json(const kmst_context* ctx, char* buf, size_t cap) {
   o << "{\"n\":" << s.n << ",\"m\":" << s.m << ",\"algorithm\":" << s.algorithm
     << ",\"packed_keys\":" << (s.packed_keys ? "true" : "false")
     << ",\"lazy_keys\":" << (s.lazy_keys ? "true" : "false")
+    << ",\"readback\":\"" << (ctx->host_mapped ? "publish" : "copy") << "\""
     << ",\"mst_edges\":" << s.mst_edges << ",\"total_weight\":" << s.total_weight
     << ",\"mst_long_groups\":" << s.mst_long_groups
     << ",\"mst_resorted\":" << (s.mst_resorted ? "true" : "false") << ",\"rounds\":[";

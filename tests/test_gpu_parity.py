@@ -280,6 +280,35 @@ def test_device_solve_and_stats(gpu, ctx):
     ctx.set_timing(False)
 
 
+def test_copy_readbacks_match_published(gpu, ctx, monkeypatch):
+    """A context reads its counters back by publication into coherent pinned
+    host words (the default) or, with KMST_READBACK_COPY=1 (and where the
+    system has no coherent pinned memory), by copies and stream
+    synchronisation; both give the oracle's MST, plain and filter."""
+    torch = gpu
+    monkeypatch.setenv("KMST_READBACK_COPY", "1")
+    cp = k.Context(0)
+    monkeypatch.delenv("KMST_READBACK_COPY")
+    try:
+        n, m = 1 << 15, 1 << 19
+        s = torch.empty(2 * m, dtype=torch.int32, device="cuda")
+        d, w = torch.empty_like(s), torch.empty_like(s)
+        ctx.gen_gnm_device(n, m, 254, 7, s, d, w)
+        rec = k.to_wedges(host(s, np.uint32), host(d, np.uint32), host(w, np.uint32))
+        exp, exp_w = oracle.kruskal(rec)
+        for c, mode in ((ctx, "publish"), (cp, "copy")):
+            for algo in (k.BORUVKA, k.FILTER_BORUVKA):
+                os_, od, ow = (torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3))
+                cnt, tw = c.solve_device(s, d, w, 2 * m, n, os_, od, ow, n, algorithm=algo)
+                assert c.stats()["readback"] == mode
+                assert tw == exp_w and cnt == len(exp)
+                assert np.array_equal(host(os_, np.uint32)[:cnt], exp["src"])
+                assert np.array_equal(host(od, np.uint32)[:cnt], exp["dst"])
+                assert np.array_equal(host(ow, np.uint32)[:cnt], exp["weight"])
+    finally:
+        cp.close()
+
+
 def test_device_solve_misaligned_inputs(gpu, ctx):
     """Input arrays at a 4-byte offset (not 16-byte aligned) and an odd
     record count exercise the scalar tail of the twin search."""
