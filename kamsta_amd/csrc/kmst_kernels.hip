@@ -2179,7 +2179,6 @@ struct MstDecode {
   bool packed;
   const u32* in_src;
   const u32* in_dst;
-  const u32* in_w;
   u32* out_s;
   u32* out_d;
   u32* out_w;
@@ -2194,11 +2193,13 @@ __device__ __forceinline__ u32 mst_write(const MstDecode& o, u64 key, u64 i) {
     b = static_cast<u32>(key & mask);
     x = static_cast<u32>(key >> (2 * o.vbits));
   } else {
+    // positional key (w << 32) | uid: the weight is the key's, the ends are
+    // gathered from the record at uid
     const u32 uid = static_cast<u32>(key);
     const u32 s = o.in_src[uid], d = o.in_dst[uid];
     a = s < d ? s : d;
     b = s < d ? d : s;
-    x = o.in_w[uid];
+    x = static_cast<u32>(key >> 32);
   }
   o.out_s[i] = a;
   o.out_d[i] = b;
@@ -2230,11 +2231,12 @@ __global__ __launch_bounds__(BLOCK) void mst_output_kernel(const u64* __restrict
 }
 
 void mst_output(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
-                const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
-                u32* out_w, u64* total_dev) {
+                const u32* in_src, const u32* in_dst, u32* out_s, u32* out_d, u32* out_w,
+                u64* total_dev) {
   if (count == 0) return;
-  KTScope t(st, K_GATHER, (packed ? 20.0 : 24.0) * count);
-  const MstDecode o{vbits, packed, in_src, in_dst, in_w, out_s, out_d, out_w};
+  // the key, the output triple, and for positional keys two 4-byte gathers
+  KTScope t(st, K_GATHER, (packed ? 20.0 : 28.0) * count);
+  const MstDecode o{vbits, packed, in_src, in_dst, out_s, out_d, out_w};
   mst_output_kernel<<<grid_for(count, 1024), BLOCK, 0, st>>>(keys, count, o, total_dev);
   KMST_HIP(hipGetLastError());
 }
@@ -2354,14 +2356,14 @@ __global__ __launch_bounds__(MST_CHUNK) void mst_rank_long_kernel(
 }
 
 void mst_order(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed, u32 lbits,
-               const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
-               u32* out_w, u64* total_dev, u32* list, u32* list_count, u32* too_long) {
+               const u32* in_src, const u32* in_dst, u32* out_s, u32* out_d, u32* out_w,
+               u64* total_dev, u32* list, u32* list_count, u32* too_long) {
   if (count == 0) return;
   if (count > 0xFFFFFFFFull) throw Error(KMST_EINVAL, "more than 2^32 - 1 MST edges");
   const u32 fbits = packed ? 2 * vbits : 32;
   const u64 fmask = fbits >= 64 ? ~0ull : ((u64{1} << fbits) - 1);
-  const MstDecode o{vbits, packed, in_src, in_dst, in_w, out_s, out_d, out_w};
-  KTScope t(st, K_GATHER, (packed ? 20.0 : 24.0) * count);
+  const MstDecode o{vbits, packed, in_src, in_dst, out_s, out_d, out_w};
+  KTScope t(st, K_GATHER, (packed ? 20.0 : 28.0) * count);
   // few workgroups: each ends with one atomic on the single total word
   mst_rank_kernel<<<grid_for(count, 1024), BLOCK, 0, st>>>(keys, count, fmask, lbits, o, list,
                                                            list_count, total_dev);

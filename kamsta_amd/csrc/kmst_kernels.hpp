@@ -228,9 +228,11 @@ bool radix_sort_u64(hipStream_t st, u64* v, u64* v_alt, u64 m, u32 bits, SortWor
 
 // K9 MST keys (sorted by their low bits: (min, max) for packed keys, the twin
 // position for positional ones) -> canonical (min, max, w) output + u64 weight.
+// A positional key's ends come from the input record at its position, its
+// weight from the key.
 void mst_output(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed,
-                const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
-                u32* out_w, u64* total_dev);
+                const u32* in_src, const u32* in_dst, u32* out_s, u32* out_d, u32* out_w,
+                u64* total_dev);
 
 // K9 without the full sort.  The order field of an MST key is (min, max) for a
 // packed key (its low 2 vbits bits), the twin position for a positional one
@@ -245,8 +247,8 @@ constexpr u32 MST_GROUP_SMALL = 64;
 constexpr u32 MST_CHUNK = 1024;
 constexpr u32 MST_GROUP_MAX = 8 * MST_CHUNK;  // one group sorted in 64 KiB of LDS
 void mst_order(hipStream_t st, const u64* keys, u64 count, u32 vbits, bool packed, u32 lbits,
-               const u32* in_src, const u32* in_dst, const u32* in_w, u32* out_s, u32* out_d,
-               u32* out_w, u64* total_dev, u32* list, u32* list_count, u32* too_long);
+               const u32* in_src, const u32* in_dst, u32* out_s, u32* out_d, u32* out_w,
+               u64* total_dev, u32* list, u32* list_count, u32* too_long);
 
 // How many of RUN_SAMPLES evenly spread records of s[0..*m_dev) start a run
 // of equal s (differ from the record before them; record 0 counts), in the

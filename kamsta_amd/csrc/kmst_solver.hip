@@ -985,7 +985,7 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
   fill_u64(st, total_dev, 1, 0);
   fill_u32(st, ctr + C_MSTLIST, 2, 0);
   // parent (n0 words) is free here: it lists the long groups' first keys
-  mst_order(st, keys, cnt, vbits, packed, lbits, in_s, in_d, in_w, out_s, out_d, out_w, total_dev,
+  mst_order(st, keys, cnt, vbits, packed, lbits, in_s, in_d, out_s, out_d, out_w, total_dev,
             parent, ctr + C_MSTLIST, ctr + C_MSTLONG);
   check_errors("finish");
   C.stats.mst_long_groups = C.host_counters[C_MSTLIST];
@@ -996,7 +996,7 @@ void Solver::run(const kmst_config& cfg, u32* out_s, u32* out_d, u32* out_w, u64
     u64* alt = keys == mst ? X.v : mst;
     if (radix_sort_u64(st, keys, alt, cnt, fbits, C.sort_work, ctr + C_ERR)) keys = alt;
     fill_u64(st, total_dev, 1, 0);
-    mst_output(st, keys, cnt, vbits, packed, in_s, in_d, in_w, out_s, out_d, out_w, total_dev);
+    mst_output(st, keys, cnt, vbits, packed, in_s, in_d, out_s, out_d, out_w, total_dev);
     check_errors("finish");
   }
   *out_n = cnt;
