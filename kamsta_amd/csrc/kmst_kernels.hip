@@ -2594,34 +2594,42 @@ void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const K
   lb.tiles_issued += tiles;
 }
 
-// Present-vertex compaction before a sparse base case: flag every src, scan,
-// and keep compact id -> original id.
-__global__ __launch_bounds__(BLOCK) void mark_present_kernel(const u32* __restrict__ s, u64 m,
-                                                             u32* __restrict__ flag) {
+// Vertex compaction before a sparse base case, in passes over the records
+// only: the records are sorted by src and symmetric, so the base case's
+// vertices are its sources, and the runs of equal src numbered in order give
+// them [0, nc) in increasing original id.  run_start_flags marks every run's
+// first record; after an exclusive scan of the marks, number_runs writes
+// new_id[src] and rep[new id] = src at every run start.
+__global__ __launch_bounds__(BLOCK) void run_start_flags_kernel(const u32* __restrict__ s, u64 m,
+                                                                u32* __restrict__ flag) {
+  for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK)
+    flag[i] = (i == 0 || s[i - 1] != s[i]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(BLOCK) void number_runs_kernel(const u32* __restrict__ s, u64 m,
+                                                            const u32* __restrict__ id,
+                                                            u32* __restrict__ new_id,
+                                                            u32* __restrict__ rep) {
   for (u64 i = u64(blockIdx.x) * BLOCK + threadIdx.x; i < m; i += u64(gridDim.x) * BLOCK) {
     const u32 v = s[i];
-    if (i == 0 || s[i - 1] != v) flag[v] = 1u;
+    if (i == 0 || s[i - 1] != v) {
+      new_id[v] = id[i];
+      rep[id[i]] = v;
+    }
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void compact_rep_kernel(const u32* __restrict__ flag,
-                                                            const u32* __restrict__ new_id, u64 n,
-                                                            u32* __restrict__ rep) {
-  for (u64 v = u64(blockIdx.x) * BLOCK + threadIdx.x; v < n; v += u64(gridDim.x) * BLOCK)
-    if (flag[v]) rep[new_id[v]] = static_cast<u32>(v);
-}
-
-void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag) {
+void run_start_flags(hipStream_t st, const u32* s, u64 m, u32* flag) {
   if (m == 0) return;
   KTScope t(st, K_OTHER, 8.0 * m);
-  mark_present_kernel<<<grid_for(m), BLOCK, 0, st>>>(s, m, flag);
+  run_start_flags_kernel<<<grid_for(m), BLOCK, 0, st>>>(s, m, flag);
   KMST_HIP(hipGetLastError());
 }
 
-void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32* rep) {
-  if (n == 0) return;
-  KTScope t(st, K_OTHER, 8.0 * n);
-  compact_rep_kernel<<<grid_for(n), BLOCK, 0, st>>>(flag, new_id, n, rep);
+void number_runs(hipStream_t st, const u32* s, u64 m, const u32* id, u32* new_id, u32* rep) {
+  if (m == 0) return;
+  KTScope t(st, K_OTHER, 8.0 * m);
+  number_runs_kernel<<<grid_for(m), BLOCK, 0, st>>>(s, m, id, new_id, rep);
   KMST_HIP(hipGetLastError());
 }
 

@@ -281,9 +281,12 @@ void split_light_heavy(hipStream_t st, const u32* s_in, const u32* d_in, const K
                        u64* v_out, LookbackWork& lb, u32* light_count_dev, u32* err);
 void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32* count_dev);
 
-// Present-vertex compaction helpers (sorted records; flag must be zeroed).
-void mark_present(hipStream_t st, const u32* s, u64 m, u32* flag);
-void compact_rep(hipStream_t st, const u32* flag, const u32* new_id, u64 n, u32* rep);
+// Vertex compaction of records sorted by src (symmetric, so the sources are
+// all the vertices): flag[i] = 1 where record i starts a run of equal src;
+// with id the exclusive scan of the flags, new_id[src] = id and rep[id] = src
+// at every run start.
+void run_start_flags(hipStream_t st, const u32* s, u64 m, u32* flag);
+void number_runs(hipStream_t st, const u32* s, u64 m, const u32* id, u32* new_id, u32* rep);
 
 // Weights-only pass over the input: *max_w, the record count of every weight
 // class into class_counts[0..P.K] (zeroed by the caller), and the class 0

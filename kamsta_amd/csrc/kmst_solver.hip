@@ -683,12 +683,14 @@ void Solver::persistent(RecView cur, RecView A, RecView B, u64 n, bool track_gpa
 void Solver::base_case(RecView P, RecView Q) {
   // a base case with few records touches few vertices: renumber them to
   // [0, n_c) (monotone, so the records stay grouped) instead of running the
-  // first round over all n0 vertex slots
+  // first round over all n0 vertex slots.  P is sorted by src and symmetric
+  // (every part the filter hands down is), so its vertices are its sources
+  // and the runs of equal src number them, in passes over P only; Q.s holds
+  // the run marks and their scan until the relabel writes Q
   if (P.m < u64(n0)) {
-    fill_u32(st, flag, n0, 0);
-    mark_present(st, P.s, P.m, flag);
-    scan_exclusive_u32(st, flag, newid, n0, scan_scratch, ctr + C_NNEXT);
-    compact_rep(st, flag, newid, n0, rep);
+    run_start_flags(st, P.s, P.m, Q.s);
+    scan_exclusive_u32(st, Q.s, Q.s, P.m, scan_scratch, ctr + C_NNEXT);
+    number_runs(st, P.s, P.m, Q.s, newid, rep);
     relabel_compact(st, P.s, P.d, keys_of(P), 0, P.m, newid, Q.s, Q.d, Q.v, C.lb_work, ctr + C_MNEXT,
                     ctr + C_ERR);
     check_errors("compact");
