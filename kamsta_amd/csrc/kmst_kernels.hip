@@ -2679,33 +2679,48 @@ void count_roots(hipStream_t st, const u32* gpar, const u32* seg_end, u64 n, u32
 // `root`, or, with the bitmap `inside` (each partition range's largest
 // component), whose bit is clear -- with their
 // record ranges [seg_start, seg_end) in the (src, dst)-sorted input (seg_end
-// is 0 for vertices without records).  Each block owns OUT_VPT * BLOCK
-// consecutive vertices, thread t those at t + k * BLOCK (coalesced loads), and
-// takes one slot range with one global atomic; the list holds a block's
-// vertices thread by thread, the blocks in the order of their atomics.
-// counters[0] = list length, counters[2..3] = u64 total range length.
+// is 0 for vertices without records).  Each block owns OUT_TILES tiles of
+// OUT_VPT * BLOCK consecutive vertices, thread t those at t + k * BLOCK of a
+// tile (coalesced loads).  A first pass keeps each thread's outside vertices
+// as one bit mask per tile; the block then takes one slot range with one
+// global atomic (a quarter of the same-address atomics of one tile per
+// block: RGG2D 2^24 lists 0.40 -> 0.37 ms, GNM unchanged) and a second pass
+// writes the marked vertices, re-reading only their ranges.
+// The list holds a block's vertices thread by thread and tile by tile, the
+// blocks in the order of their atomics.  counters[0] = list length,
+// counters[2..3] = u64 total range length.
 constexpr int OUT_VPT = 16;
+constexpr int OUT_TILES = 4;
 
 __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
     const u32* __restrict__ gpar, const u32* __restrict__ seg_start,
     const u32* __restrict__ seg_end, u64 n, u32 root, const u64* __restrict__ inside,
     u32* __restrict__ list, u32* __restrict__ seg_lo, u32* __restrict__ seg_hi, u32* counters) {
+  static_assert(OUT_VPT <= 32, "one bit per vertex of a tile in a u32 mask");
   __shared__ u32 s_scan[WAVES];
   __shared__ u32 s_base;
   __shared__ u64 s_len[WAVES];
-  const u64 v0 = u64(blockIdx.x) * (BLOCK * OUT_VPT) + threadIdx.x;
-  u32 h[OUT_VPT];
-  bool out[OUT_VPT];
-#pragma unroll
-  for (int k = 0; k < OUT_VPT; ++k) {
-    const u64 v = min(v0 + u64(k) * BLOCK, n - 1);
-    h[k] = seg_end[v];
-    const bool in = inside ? ((inside[v >> 6] >> (v & 63)) & 1ull) != 0 : gpar[v] == root;
-    out[k] = v0 + u64(k) * BLOCK < n && h[k] && !in;
-  }
+  const u64 b0 = u64(blockIdx.x) * (OUT_TILES * BLOCK * OUT_VPT) + threadIdx.x;
+  u32 mask[OUT_TILES];
   u32 mine = 0;
 #pragma unroll
-  for (int k = 0; k < OUT_VPT; ++k) mine += out[k] ? 1u : 0u;
+  for (int tt = 0; tt < OUT_TILES; ++tt) {
+    const u64 v0 = b0 + u64(tt) * (BLOCK * OUT_VPT);
+    u32 h[OUT_VPT];
+    bool in[OUT_VPT];
+#pragma unroll
+    for (int k = 0; k < OUT_VPT; ++k) {
+      const u64 v = min(v0 + u64(k) * BLOCK, n - 1);
+      h[k] = seg_end[v];
+      in[k] = inside ? ((inside[v >> 6] >> (v & 63)) & 1ull) != 0 : gpar[v] == root;
+    }
+    u32 mk = 0;
+#pragma unroll
+    for (int k = 0; k < OUT_VPT; ++k)
+      if (v0 + u64(k) * BLOCK < n && h[k] && !in[k]) mk |= 1u << k;
+    mask[tt] = mk;
+    mine += static_cast<u32>(__popc(mk));
+  }
   u32 total;
   const u32 off = block_exclusive_sum(mine, total, s_scan);
   if (threadIdx.x == 0) s_base = total ? atomicAdd(counters, total) : 0u;
@@ -2713,14 +2728,15 @@ __global__ __launch_bounds__(BLOCK) void outside_list_kernel(
   u32 slot = s_base + off;
   u64 len = 0;
 #pragma unroll
-  for (int k = 0; k < OUT_VPT; ++k) {
-    if (out[k]) {
-      const u32 v = static_cast<u32>(v0 + u64(k) * BLOCK);
-      const u32 lo = seg_start[v];
+  for (int tt = 0; tt < OUT_TILES; ++tt) {
+    const u64 v0 = b0 + u64(tt) * (BLOCK * OUT_VPT);
+    for (u32 mk = mask[tt]; mk; mk &= mk - 1) {
+      const u32 v = static_cast<u32>(v0 + u64(__ffs(mk) - 1) * BLOCK);
+      const u32 lo = seg_start[v], hi = seg_end[v];
       list[slot] = v;
       seg_lo[slot] = lo;
-      seg_hi[slot] = h[k];
-      len += h[k] - lo;
+      seg_hi[slot] = hi;
+      len += hi - lo;
       ++slot;
     }
   }
@@ -2743,7 +2759,7 @@ void outside_list(hipStream_t st, const u32* gpar, const u32* seg_start, const u
   fill_u32(st, counters, 4, 0);
   if (n == 0) return;
   KTScope t(st, K_OTHER, inside ? 4.0 * n + n / 8.0 : 8.0 * n);
-  const u32 blocks = ceil_div_u32(n, BLOCK * OUT_VPT);
+  const u32 blocks = ceil_div_u32(n, u64(OUT_TILES) * BLOCK * OUT_VPT);
   outside_list_kernel<<<blocks, BLOCK, 0, st>>>(gpar, seg_start, seg_end, n, root, inside, list,
                                                 seg_lo, seg_hi, counters);
   KMST_HIP(hipGetLastError());
