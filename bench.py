@@ -531,14 +531,22 @@ def main():
     achieved = per_launch_bytes / (avg_ms / 1000.0) / 1e9
     # HBM bytes per launch of the same kernel from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_*.md) on the same
-    # workload, or null
+    # workload, or null.  The counted runs may split the kind into more or
+    # fewer launches than this run's tuning does (the persistent tails take
+    # the small rounds), so their ratio of HBM to algorithmic bytes is applied
+    # to this run's algorithmic bytes per launch.
     traffic = None
+    traffic_ratio = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         try:
             tj = json.load(open(tfile))
             t = tj.get(best, {}).get(dom) if tj.get("workload") == workload else None
-            traffic = round(t["bytes_per_launch"]) if t else None
+            if t and t.get("traffic_ratio"):
+                traffic_ratio = t["traffic_ratio"]
+                traffic = round(traffic_ratio * per_launch_bytes)
+            elif t:
+                traffic = round(t["bytes_per_launch"])
         except Exception:
             traffic = None
     total_bytes = sum(v["bytes"] for v in kern.values()) / args.steps
@@ -549,7 +557,13 @@ def main():
     sc_gbs = sum(v["bytes"] for v in sc) / max(sum(v["ms"] for v in sc), 1e-9) / 1e6
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "algorithmic_bytes_per_launch": round(per_launch_bytes),
+                "traffic": traffic,
+                "traffic_ratio": round(traffic_ratio, 3) if traffic_ratio else None,
+                "traffic_basis": ("HBM bytes over algorithmic bytes of this kernel kind in the "
+                                  "committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs on this "
+                                  "workload (profiles/traffic_*.md), times this run's "
+                                  "algorithmic bytes per launch") if traffic_ratio else None,
+                "algorithmic_bytes_per_launch": round(per_launch_bytes),
                 "avg_launch_ms": round(avg_ms, 4),
                 "end_to_end_frac": round(total_bytes / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
                 "sort_compaction_frac": round(sc_gbs / HBM_PEAK_GBS, 4),
