@@ -635,10 +635,10 @@ void weight_classes(hipStream_t st, const u32* w, u64 m, const WeightPivots& P, 
   if (m == 0) return;
   KTScope t(st, K_VALIDATE, 4.0 * m + 8.0 * record_units(m));
   const u64 waves = (record_units(m) + WC_UNITS - 1) / WC_UNITS;
-  // 2048 workgroups (8 per CU) fill the chip; more only add block-final
-  // reductions and atomics on the few class counters (at RGG2D 2^24:
-  // 0.47 ms with 8192, 0.28 with 2048, 0.31 with 1024)
-  const u32 grid = static_cast<u32>(std::min<u64>((waves + WAVES - 1) / WAVES, 2048));
+  // 1024 workgroups (4 per CU) keep the loads in flight; more only add
+  // block-final reductions and atomics on the few class counters (GNM 2^24 /
+  // 2^28: 0.438 ms with 1024, 0.453 with 2048; RGG2D 2^24: 0.231 and 0.239)
+  const u32 grid = static_cast<u32>(std::min<u64>((waves + WAVES - 1) / WAVES, 1024));
   if ((reinterpret_cast<uintptr_t>(w) & 15) == 0)
     weight_classes_kernel<true><<<grid, BLOCK, 0, st>>>(w, m, P, q0, q1, class_counts, max_w,
                                                         unit_c0, unit_c1);
