@@ -3289,9 +3289,9 @@ void class_scatter(hipStream_t st, const u32* s_in, const u32* d_in, const KeySr
   KTScope t(st, K_SPLIT_SCATTER, (keys.v ? 16.0 : 12.0) * m + 16.0 * double(written));
   const u64 chunk = check ? ScatterChunk<true>::UNITS : ScatterChunk<false>::UNITS;
   // more workgroups than fit at once: the dispatcher then balances the
-  // chunks, whose time varies (GNM 2^24 / 2^28: 2.54 ms with 2048, 2.20 with
-  // 4096, 2.13 with 8192, 2.08 with 16384)
-  const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + chunk - 1) / chunk, 16384));
+  // chunks, whose time varies (GNM 2^24 / 2^28: 2.08 ms with 4096, 1.95 with
+  // 8192, 1.93 with 16384, 1.89 with 32768 and 65536)
+  const u32 grid = static_cast<u32>(std::min<u64>((record_units(m) + chunk - 1) / chunk, 65536));
   const bool vec = ((reinterpret_cast<uintptr_t>(s_in) | reinterpret_cast<uintptr_t>(d_in) |
                      reinterpret_cast<uintptr_t>(keys.v ? static_cast<const void*>(keys.v)
                                                         : static_cast<const void*>(keys.w))) &
