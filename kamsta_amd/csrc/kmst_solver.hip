@@ -164,7 +164,7 @@ std::vector<DevBuf*> kmst_context::all_bufs() {
   return {&xs,       &xd,         &xv,      &ys,        &yd,    &yv,      &val0,
           &off,      &best_val,   &best_dst, &parent,   &flag,  &newid,   &label,
           &rep,      &rep_next,   &gpar,    &seg_start, &seg_end, &mst,   &pieces,
-          &tiles,    &scan_scratch, &hist,  &counters,  &lookback, &in_s, &in_d,
+          &scan_scratch, &hist,  &counters,  &lookback, &in_s, &in_d,
           &in_w,     &out_s,      &out_d,   &out_w,     &samples, &gen_a, &gen_b,
           &gen_c,    &gen_d,      &tune,    &codec_bytes, &codec_off, &codec_first,
           &persist_log, &part_giant};

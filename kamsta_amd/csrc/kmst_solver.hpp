@@ -132,7 +132,7 @@ struct kmst_context {
   kmst::DevBuf off, best_val, best_dst, parent, flag, newid, label, rep, rep_next, gpar, seg_start, seg_end,
       mst;
   // kernel scratch
-  kmst::DevBuf pieces, tiles, scan_scratch, hist, counters, samples, lookback, persist_log,
+  kmst::DevBuf pieces, scan_scratch, hist, counters, samples, lookback, persist_log,
       part_giant;
   kmst::SortWork sort_work;  // radix sort state over `hist`
   kmst::LookbackWork lb_work;  // compaction look-back state over `lookback`
