@@ -372,8 +372,8 @@ def cpu_baselines(args, ctx, full, threads):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                                    "note only (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the solver runs on one GPU: sharding a solve over several "
+                                    "GPUs is designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.serving:
         serving_only(args)
