@@ -818,6 +818,11 @@ void build_keys(hipStream_t st, const u32* src, const u32* dst, const u32* w, u6
 // finished locally; runs spanning threads are combined by a segmented min
 // scan over the threads' tail pieces; runs spanning tiles go through a u64
 // atomicMin plus a fix-up pass that writes best_dst.
+// With packed keys and the tile's sources inside [first, first + ARG_STAGE)
+// (the tile's first source), the finished minima are staged in LDS by
+// source and leave in coalesced rows, best_dst unpacked from the key (the
+// end that is not the source), instead of one scattered store per run end.
+constexpr u32 ARG_STAGE = ARG_TILE;
 struct Piece {
   u64 val;
   u32 idx;
@@ -850,6 +855,7 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
   __shared__ u64 s_wave_val[WAVES];
   __shared__ u32 s_wave_idx[WAVES];
   __shared__ u32 s_wave_flag[WAVES];
+  __shared__ u64 s_best[ARG_STAGE];
 
   const int t = threadIdx.x;
   const u64 tile_base = u64(blockIdx.x) * ARG_TILE;
@@ -908,6 +914,13 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
       rv[k] = load_key(keys, i, rs[k], rd[k]);
     }
   }
+  // staged: every record of the tile has its source within ARG_STAGE of the
+  // tile's first (the ones past m - 1 repeat the last record)
+  bool near = true;
+#pragma unroll
+  for (int k = 0; k < ARG_IPT; ++k) near = near && rs[k] - tile_first_src < ARG_STAGE;
+  for (u32 i = t; i < ARG_STAGE; i += BLOCK) s_best[i] = VAL_NONE;
+  const bool staged = keys.v == nullptr && __syncthreads_and(near);
   if (has_items) {
     const u32 cnt = static_cast<u32>(min<u64>(ARG_IPT, tile_end - my_base));
     cur_src = rs[0];
@@ -925,7 +938,9 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
           head_val = cur_val;
           head_idx = cur_idx;
           head_done = true;
-        } else {  // interior run: complete inside this thread
+        } else if (staged) {  // interior run: complete inside this thread
+          s_best[cur_src - tile_first_src] = cur_val;
+        } else {
           best_val[cur_src] = cur_val;
           best_dst[cur_src] = cur_dst;
         }
@@ -979,13 +994,15 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
   s_scan_idx[t] = p.idx;
   __syncthreads();
 
-  if (!has_items) return;
+  if (!has_items && !staged) return;
 
-  // emit one finished run: direct store, or atomic + fix-up when it crosses
-  // a tile boundary
+  // emit one finished run: staged or direct store, or atomic + fix-up when
+  // it crosses a tile boundary
   auto emit = [&](u32 s, u64 v, u32 idx) {
     const bool crosses = (cross_prev && s == tile_first_src) || (cross_next && s == tile_last_src);
-    if (!crosses) {
+    if (!crosses && staged) {
+      s_best[s - tile_first_src] = v;
+    } else if (!crosses) {
       best_val[s] = v;
       best_dst[s] = dst[idx];
     } else {
@@ -997,7 +1014,7 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
   };
 
   // head run (thread has more than one run)
-  if (head_done) {
+  if (has_items && head_done) {
     if (t > 0 && prev_tail == head_src) {
       // run started in an earlier thread and ends here
       const u64 pv = s_scan_val[t - 1];
@@ -1010,7 +1027,19 @@ __global__ __launch_bounds__(BLOCK) void argmin_kernel(const u32* __restrict__ s
   // tail run: this thread finishes it when the next piece has another src
   const bool last_thread = (t == BLOCK - 1) || (my_base + ARG_IPT >= tile_end);
   const u32 next_first = last_thread ? NONE32 : s_first_src[t + 1];
-  if (last_thread || next_first != cur_src) emit(cur_src, p.val, p.idx);
+  if (has_items && (last_thread || next_first != cur_src)) emit(cur_src, p.val, p.idx);
+  if (!staged) return;
+  __syncthreads();
+  const u64 id_mask = (u64{1} << keys.vbits) - 1;
+  for (u32 i = t; i < ARG_STAGE; i += BLOCK) {
+    const u64 v = s_best[i];
+    if (v == VAL_NONE) continue;
+    const u32 s = tile_first_src + i;
+    const u32 lo = static_cast<u32>((v >> keys.vbits) & id_mask);
+    const u32 hi = static_cast<u32>(v & id_mask);
+    best_val[s] = v;
+    best_dst[s] = lo == s ? hi : lo;
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void argmin_fixup_kernel(const ArgPiece* __restrict__ pieces,
