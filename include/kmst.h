@@ -289,6 +289,13 @@ kmst_status kmst_k_segmented_argmin(kmst_context* ctx, const uint32_t* src,
                                     const uint32_t* dst, const uint64_t* val,
                                     uint64_t m, uint64_t n, uint64_t* best_val,
                                     uint32_t* best_dst);
+/* K1 with packed keys made from the weights: val = weight << 2 vbits |
+ * min(src, dst) << vbits | max(src, dst) (vbits <= 31, weight < 2^(64 - 2 vbits));
+ * otherwise as kmst_k_segmented_argmin. */
+kmst_status kmst_k_segmented_argmin_packed(kmst_context* ctx, const uint32_t* src,
+                                           const uint32_t* dst, const uint32_t* weight,
+                                           uint32_t vbits, uint64_t m, uint64_t n,
+                                           uint64_t* best_val, uint32_t* best_dst);
 /* K6: stable LSD radix sort of (src, dst, val) by (src, dst) over
  * `bits` bits of each endpoint, in place (uses context scratch). */
 kmst_status kmst_k_sort_pairs(kmst_context* ctx, uint32_t* src, uint32_t* dst,

@@ -48,7 +48,8 @@ EXPORTED_SYMBOLS = [
     "kmst_tuning_default", "kmst_context_set_tuning", "kmst_context_get_tuning", "kmst_autotune",
     "kmst_encode_host",
     "kmst_last_error", "kmst_context_stats_json", "kmst_gen_gnm_device", "kmst_gen_gnm_host",
-    "kmst_k_segmented_argmin", "kmst_k_sort_pairs", "kmst_k_pointer_jump",
+    "kmst_k_segmented_argmin", "kmst_k_segmented_argmin_packed", "kmst_k_sort_pairs",
+    "kmst_k_pointer_jump",
     "kmst_server_config_default", "kmst_server_create", "kmst_server_submit", "kmst_server_poll",
     "kmst_server_wait", "kmst_server_counters", "kmst_server_destroy",
 ]
@@ -201,6 +202,7 @@ def load():
         "kmst_gen_rhg_device": (i32, [vp, u64, ctypes.c_double, ctypes.c_double, u32, u64, vp, vp,
                                       vp, u64, pu64]),
         "kmst_k_segmented_argmin": (i32, [vp, vp, vp, vp, u64, u64, vp, vp]),
+        "kmst_k_segmented_argmin_packed": (i32, [vp, vp, vp, vp, u32, u64, u64, vp, vp]),
         "kmst_k_sort_pairs": (i32, [vp, vp, vp, vp, u64, u32]),
         "kmst_k_pointer_jump": (i32, [vp, vp, u64]),
         "kmst_server_config_default": (None, [vp]),
@@ -445,6 +447,11 @@ class Context:
     def segmented_argmin(self, src, dst, val, m, n, best_val, best_dst):
         _check(self._lib.kmst_k_segmented_argmin(self._h, _ptr(src), _ptr(dst), _ptr(val), m, n,
                                                  _ptr(best_val), _ptr(best_dst)))
+
+    def segmented_argmin_packed(self, src, dst, weight, vbits, m, n, best_val, best_dst):
+        _check(self._lib.kmst_k_segmented_argmin_packed(self._h, _ptr(src), _ptr(dst),
+                                                        _ptr(weight), vbits, m, n,
+                                                        _ptr(best_val), _ptr(best_dst)))
 
     def sort_pairs(self, src, dst, val, m, bits):
         _check(self._lib.kmst_k_sort_pairs(self._h, _ptr(src), _ptr(dst), _ptr(val), m, bits))

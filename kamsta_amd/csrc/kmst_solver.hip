@@ -1402,6 +1402,25 @@ kmst_status kmst_k_segmented_argmin(kmst_context* ctx, const uint32_t* src, cons
   });
 }
 
+kmst_status kmst_k_segmented_argmin_packed(kmst_context* ctx, const uint32_t* src,
+                                           const uint32_t* dst, const uint32_t* weight,
+                                           uint32_t vbits, uint64_t m, uint64_t n,
+                                           uint64_t* best_val, uint32_t* best_dst) {
+  return guarded([&] {
+    if (!ctx) throw kmst::Error(KMST_EINVAL, "null context");
+    if (vbits == 0 || vbits > 31) throw kmst::Error(KMST_EINVAL, "vbits outside [1, 31]");
+    if (n > (uint64_t{1} << vbits)) throw kmst::Error(KMST_EINVAL, "n exceeds 2^vbits");
+    KMST_HIP(hipSetDevice(ctx->device));
+    const uint64_t tiles = (m + kmst::ARG_TILE - 1) / kmst::ARG_TILE;
+    auto* pieces = static_cast<kmst::ArgPiece*>(
+        ctx->pieces.get(2 * tiles * sizeof(kmst::ArgPiece) + 64));
+    auto* ctr = static_cast<uint32_t*>(ctx->counters.get(256));
+    kmst::segmented_argmin(ctx->stream, src, dst, kmst::KeySrc{nullptr, weight, vbits}, m, n,
+                           best_val, best_dst, pieces, ctr + kmst::C_PIECES);
+    KMST_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 kmst_status kmst_k_sort_pairs(kmst_context* ctx, uint32_t* src, uint32_t* dst, uint64_t* val,
                               uint64_t m, uint32_t bits) {
   return guarded([&] {

@@ -77,6 +77,70 @@ def test_segmented_argmin_kernel(gpu, ctx, case):
     assert np.array_equal(gd[live], exp_d[live])
 
 
+@pytest.mark.parametrize("case", ["uniform", "hubs", "sparse", "grouped"])
+def test_segmented_argmin_packed_kernel(gpu, ctx, case):
+    """Packed keys made from the weights: tiles whose sources lie within
+    4096 ids of their first stage the minima in LDS and unpack best_dst from
+    the key; sparse ids and groups out of order take the direct stores."""
+    torch = gpu
+    rng = np.random.default_rng(12)
+    if case == "uniform":  # 0-39 records per vertex, empty vertices among them
+        n = 20000
+        deg = rng.integers(0, 40, n)
+        order = np.arange(n)
+    elif case == "hubs":  # runs across many 4096-record tiles
+        n = 300
+        deg = rng.integers(1, 5, n)
+        deg[[0, 150, 299]] = [70000, 5000, 9000]
+        order = np.arange(n)
+    elif case == "sparse":  # sources far apart: a tile spans more than 4096 ids
+        n = 1 << 22
+        deg = np.zeros(n, np.int64)
+        ids = rng.choice(n, 30000, replace=False)
+        deg[ids] = rng.integers(1, 9, len(ids))
+        order = np.arange(n)
+    else:  # grouped by source, groups in random order
+        n = 20000
+        deg = rng.integers(1, 12, n)
+        order = rng.permutation(n)
+    src = np.repeat(order.astype(np.uint32), deg[order])
+    m = len(src)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    vbits = max(1, int(n - 1).bit_length())
+    w = rng.integers(0, 1 << 19, m).astype(np.uint32)  # no key is all ones
+    assert 2 * vbits + 20 <= 64
+    lo = np.minimum(src, dst).astype(np.uint64)
+    hi = np.maximum(src, dst).astype(np.uint64)
+    key = (w.astype(np.uint64) << np.uint64(2 * vbits)) | (lo << np.uint64(vbits)) | hi
+    bv = dev(torch, np.zeros(n, np.uint64), np.uint64)
+    bd = dev(torch, np.zeros(n, np.uint32), np.uint32)
+    ctx.segmented_argmin_packed(dev(torch, src, np.uint32), dev(torch, dst, np.uint32),
+                                dev(torch, w, np.uint32), vbits, m, n, bv, bd)
+    torch.cuda.synchronize()
+    gv, gd = host(bv, np.uint64), host(bd, np.uint32)
+    exp_v = np.full(n, np.uint64(2**64 - 1))
+    exp_d = np.zeros(n, np.uint32)
+    srt = np.lexsort((key, src))
+    first = np.ones(m, bool)
+    first[1:] = src[srt][1:] != src[srt][:-1]
+    idx = srt[first]
+    exp_v[src[idx]] = key[idx]
+    exp_d[src[idx]] = dst[idx]
+    assert np.array_equal(gv, exp_v)
+    live = exp_v != np.uint64(2**64 - 1)
+    assert np.array_equal(gd[live], exp_d[live])
+
+
+def test_segmented_argmin_packed_rejects_bad_vbits(gpu, ctx):
+    torch = gpu
+    z32 = dev(torch, np.zeros(4, np.uint32), np.uint32)
+    z64 = dev(torch, np.zeros(4, np.uint64), np.uint64)
+    with pytest.raises(k.KmstError):
+        ctx.segmented_argmin_packed(z32, z32, z32, 0, 4, 4, z64, z32)
+    with pytest.raises(k.KmstError):
+        ctx.segmented_argmin_packed(z32, z32, z32, 1, 4, 4, z64, z32)  # n > 2^vbits
+
+
 @pytest.mark.parametrize("m,bits", [(1, 3), (5000, 4), (100003, 17), (300000, 24), (70000, 32)])
 def test_radix_sort_kernel_stable(gpu, ctx, m, bits):
     torch = gpu
