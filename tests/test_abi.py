@@ -53,6 +53,14 @@ def test_host_api_rejects_wide_ids_without_gpu():
     assert err.value.status == k.KMST_EINVAL
 
 
+def test_kernel_entries_reject_null_context_without_gpu():
+    lib = k.load()
+    assert lib.kmst_k_segmented_argmin(None, None, None, None, 0, 0, None, None) == k.KMST_EINVAL
+    assert lib.kmst_k_segmented_argmin_packed(None, None, None, None, 8, 0, 0, None,
+                                              None) == k.KMST_EINVAL
+    assert b"null context" in lib.kmst_last_error()
+
+
 @pytest.mark.parametrize("n,m,mw,seed", [(2, 1, 254, 0), (16, 120, 3, 1), (300, 2000, 254, 2),
                                          (1 << 14, 1 << 16, 254, 0x4B414D5354410001),
                                          (256, 4000, 254, 9)])
