@@ -1950,8 +1950,12 @@ bool radix_sort(hipStream_t st, u32* s, u32* d, u64* v, u32* s_alt, u32* d_alt, 
   {
     const bool use_d = d && dbits;  // d is only carried along otherwise
     KTScope t(st, K_SORT_HIST, (use_d ? 8.0 : 4.0) * m);
-    // few, fat blocks: every block adds its histograms to the same global bins
-    const u32 grid = grid_for((m + UP_RPT - 1) / UP_RPT, 512);
+    // few, fat blocks: every block adds its histograms to the same global bins.
+    // From 2^26 records on, 1024 blocks hide more load latency than their
+    // extra global adds cost (plain Boruvka's 2^29-record sorts: 1.65 -> 1.15
+    // ms of histograms per solve); below, 512 (filter-Boruvka at GNM 2^28:
+    // 0.29 ms with 512, 0.30 with 1024, 0.38 with 2048)
+    const u32 grid = grid_for((m + UP_RPT - 1) / UP_RPT, m >= (u64{1} << 26) ? 1024 : 512);
     const bool vec = ((reinterpret_cast<uintptr_t>(s) |
                        (use_d ? reinterpret_cast<uintptr_t>(d) : 0)) & 15) == 0;
     if (use_d) radix_upsweep_kernel<true><<<grid, BLOCK, 0, st>>>(s, d, m, dbits, pl, ghist, vec);
